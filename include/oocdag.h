@@ -1,0 +1,138 @@
+/*
+ * oocdag.h -- C ABI of liboocdag, the MI355X out-of-core Cholesky library.
+ *
+ * Every entry point is plain C: pointers, sizes, a HIP stream passed as
+ * void*.  Each one names the reference interface it replaces.  All return
+ * an int status (OOCDAG_OK or a negative OOCDAG_E*) unless noted; the
+ * factorization entry points additionally report LAPACK-style *info.
+ *
+ * Storage conventions
+ *   host A      column-major N x N, leading dimension LDA >= N, lower
+ *               triangle used (the reference's test_A, oocdag.cpp:20,251).
+ *   tiles       "packed lower tiles": the reference's device panel Y
+ *               (oocquark.cpp:256) for the whole matrix: tile (i,j), i >= j,
+ *               of edge NB at element offset (i + j*bb - j*(j+1)/2) * NB*NB,
+ *               each tile column-major with ld = NB, bb = ceil(N/NB).  Rows and
+ *               columns past N are padding (0, with 1 on the diagonal).
+ */
+#ifndef OOCDAG_H
+#define OOCDAG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OOCDAG_OK 0
+#define OOCDAG_EARG (-1)    /* invalid argument */
+#define OOCDAG_EBUDGET (-2) /* budget below two tile columns (oocdag.cpp:249-250) */
+#define OOCDAG_ENOMEM (-3)  /* device or pinned host allocation failed */
+#define OOCDAG_EHIP (-4)    /* HIP runtime error */
+#define OOCDAG_ENODEV (-5)  /* no usable gfx950 device */
+
+/* ---------------------------------------------------------------------------
+ * Factorization, host array in / host array out.
+ * Replaces: double Cholesky(Quark *quark, double *A, int N, int NB, int LDA,
+ *                           size_t memsize)                  oocdag.cpp:18-112
+ * Same arguments and meaning (the QUARK handle is gone: the library runs its
+ * own HIP streams).  The lower triangle of A is overwritten with L, the strict
+ * upper triangle is untouched.  memsize is the device budget in bytes; the
+ * panel widths follow the reference planner (find_Yn, oocdag.cpp:478-492)
+ * with room for two staged X tile columns instead of one.  Returns the
+ * wall-clock seconds of the factorization (oocdag.cpp:51,90,110), or a
+ * negative value on error.  *info: 0 success; k > 0 the leading minor of
+ * order k is not positive definite (LAPACK dpotrf convention; the reference
+ * only printed it, oocquark.cpp:95); < 0 an OOCDAG_E* code.  NB must be a
+ * multiple of 64.
+ * ------------------------------------------------------------------------- */
+double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *info);
+
+/* Panel widths (tile columns per Y panel) that oocdag_cholesky will use for
+ * (N, NB, memsize): widths[0..return-1].  Returns the panel count, or < 0. */
+int oocdag_panel_plan(int N, int NB, size_t memsize, int *widths, int maxp);
+/* Same with the number of staged X tile columns fixed: nx = 1 is the
+ * reference's find_Yn schedule (oocdag.cpp:478-492) exactly, nx = 2 the
+ * double-buffered one; nx = 0 lets the library choose. */
+int oocdag_panel_plan_nx(int N, int NB, size_t memsize, int nx, int *widths, int maxp);
+
+/* ---------------------------------------------------------------------------
+ * Input generator.
+ * Replaces: pdmatgen(ICTXT, AFORM, DIAG, M, N, MB, NB, A, LDA, IAROW, IACOL,
+ *   ISEED, IROFF, IRNUM, ICOFF, ICNUM, MYROW, MYCOL, NPROW, NPCOL) BIND(C)
+ *   oocdag.h:85-90, LIN/pdmatgen.f:1-3, called at oocdag.cpp:298-302
+ * Specialised to the 1x1 grid call of the reference (offsets 0, whole
+ * matrix).  Values are bit-identical to PDMATGEN.  Computed on the GPU and
+ * copied to A.  Returns the PDMATGEN INFO (0 ok, > 0 bad argument position)
+ * or an OOCDAG_E* code.  AFORM: 'S'/'H' symmetric, 'F' hard-coded SPD form,
+ * 'T'/'C' transposed, other general; DIAG: 'D' or 'N'.
+ * ------------------------------------------------------------------------- */
+int oocdag_pdmatgen(char aform, char diag, int M, int N, int MB, int NB, double *A, int LDA,
+                    int iseed);
+
+/* ---------------------------------------------------------------------------
+ * HBM-resident factorization on packed lower tiles (no host traffic).
+ * Replaces the in-core part of Cholesky() when the whole lower triangle fits
+ * in the device budget (one Y panel: ooc_incore, oocquark.cpp:358-421).
+ * dY: device pointer to oocdag_tiles_elems(N, NB) doubles.  stream: a
+ * hipStream_t or NULL for the library's own stream.  Synchronous: returns
+ * after the factorization finished; *info as for oocdag_cholesky.
+ * ------------------------------------------------------------------------- */
+int oocdag_potrf_tiles(double *dY, int N, int NB, void *stream, int *info);
+
+/* Number of doubles of the packed lower tile storage for (N, NB). */
+size_t oocdag_tiles_elems(int N, int NB);
+
+/* PDMATGEN('S'|'H'|'F', DIAG, ISEED) written straight into packed lower tiles
+ * in HBM (values identical to oocdag_pdmatgen; padding as described above). */
+int oocdag_pdmatgen_tiles(double *dY, int N, int NB, char aform, char diag, int iseed,
+                          void *stream);
+
+/* Copy between packed lower tiles (device) and a column-major host array
+ * (lower triangle incl. diagonal blocks' full NB x NB squares, like the
+ * reference's A2Y / Y2A tile copies, oocquark.cpp:253-299). */
+int oocdag_tiles_from_host(double *dY, const double *A, int N, int NB, int LDA);
+int oocdag_tiles_to_host(const double *dY, double *A, int N, int NB, int LDA);
+
+/* ---------------------------------------------------------------------------
+ * Tile kernels (device pointers, column-major, explicit leading dimensions).
+ * lda/ldb/ldc must be even and pointers 16-byte aligned.
+ * Replaces: cublasDgemm_v2(N,T)   oocquark.cpp:222  (CORE_incore_dgemm)
+ *           cublasDsyrk_v2(L,N)   oocquark.cpp:176  (CORE_incore_dsyrk)
+ *           cublasDtrsm_v2(R,L,T,N) oocquark.cpp:130 (CORE_incore_dtrsm)
+ *           dpotrf_('L')          oocquark.cpp:87   (CORE_incore_dpotrf)
+ * ------------------------------------------------------------------------- */
+int oocdag_dgemm_nt(int m, int n, int k, double alpha, const double *A, int lda,
+                    const double *B, int ldb, double beta, double *C, int ldc, void *stream);
+int oocdag_dsyrk_ln(int n, int k, double alpha, const double *A, int lda, double beta,
+                    double *C, int ldc, void *stream);
+/* B(m x n) <- B * L^-T, L n x n lower.  n % 64 == 0. */
+int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb, void *stream);
+/* In-place lower Cholesky of an n x n matrix, n % 64 == 0; strict upper
+ * triangle untouched.  Synchronous; *info LAPACK-style. */
+int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Device memory helpers for bindings without a GPU array library.
+ * ------------------------------------------------------------------------- */
+void *oocdag_dev_alloc(size_t bytes);
+int oocdag_dev_free(void *p);
+int oocdag_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int oocdag_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int oocdag_memset_d(void *dst, int value, size_t bytes);
+int oocdag_device_sync(void);
+/* Number of visible HIP devices (0 without a GPU); does not fail. */
+int oocdag_device_count(void);
+/* Last error message of this thread ("" if none). */
+const char *oocdag_last_error(void);
+
+/* Timing of the last oocdag_cholesky / oocdag_potrf_tiles call on this
+ * thread: seconds in the factorization, bytes moved host->device and
+ * device->host, and seconds the copy streams were busy (0 if no copies). */
+int oocdag_last_stats(double *seconds, double *h2d_bytes, double *d2h_bytes,
+                      double *h2d_seconds, double *d2h_seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,232 @@
+"""oocdag_amd -- MI355X out-of-core left-looking tiled Cholesky.
+
+Python view of liboocdag.so (the C ABI declared in include/oocdag.h).  It
+mirrors the reference entry point ``double Cholesky(Quark*, double *A, int N,
+int NB, int LDA, size_t memsize)`` (oocdag.cpp:18) as :func:`cholesky`, the
+generator ``pdmatgen`` (LIN/pdmatgen.f, oocdag.cpp:298-302) as
+:func:`pdmatgen`, and exposes the HBM-resident tile path and the tile
+kernels.
+
+The library is HIP-only: every call runs on the GPU, and a missing or
+unbuildable library raises instead of falling back to any CPU code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboocdag.so")
+
+OK, EARG, EBUDGET, ENOMEM, EHIP, ENODEV = 0, -1, -2, -3, -4, -5
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+# name -> (restype, argtypes); every symbol declared in include/oocdag.h
+SIGNATURES = {
+    "oocdag_cholesky": (C.c_double, [_dp, C.c_int, C.c_int, C.c_int, C.c_size_t, _ip]),
+    "oocdag_panel_plan": (C.c_int, [C.c_int, C.c_int, C.c_size_t, _ip, C.c_int]),
+    "oocdag_panel_plan_nx": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.c_int, _ip, C.c_int]),
+    "oocdag_pdmatgen": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
+                                  C.c_int, C.c_int]),
+    "oocdag_potrf_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _ip]),
+    "oocdag_tiles_elems": (C.c_size_t, [C.c_int, C.c_int]),
+    "oocdag_pdmatgen_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char, C.c_char,
+                                        C.c_int, C.c_void_p]),
+    "oocdag_tiles_from_host": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int]),
+    "oocdag_tiles_to_host": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int]),
+    "oocdag_dgemm_nt": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int,
+                                  C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int,
+                                  C.c_void_p]),
+    "oocdag_dsyrk_ln": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_double,
+                                  C.c_void_p, C.c_int, C.c_void_p]),
+    "oocdag_dtrsm_rltn": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                    C.c_void_p]),
+    "oocdag_dpotrf_l": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _ip, C.c_void_p]),
+    "oocdag_dev_alloc": (C.c_void_p, [C.c_size_t]),
+    "oocdag_dev_free": (C.c_int, [C.c_void_p]),
+    "oocdag_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "oocdag_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "oocdag_memset_d": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
+    "oocdag_device_sync": (C.c_int, []),
+    "oocdag_device_count": (C.c_int, []),
+    "oocdag_last_error": (C.c_char_p, []),
+    "oocdag_last_stats": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
+}
+
+
+class OocdagError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load liboocdag.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OocdagError(f"{LIB_PATH} is not built: run python -m oocdag_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().oocdag_last_error().decode()
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != OK:
+        raise OocdagError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.f_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def device_count() -> int:
+    return lib().oocdag_device_count()
+
+
+# ---------------------------------------------------------------------------
+# reference entry points
+# ---------------------------------------------------------------------------
+
+def cholesky(A: np.ndarray, nb: int, memsize: int):
+    """Cholesky(A, N, NB, LDA, memsize) (oocdag.cpp:18): factors A in place.
+
+    Returns (seconds, info) like the reference returns seconds; info follows
+    LAPACK dpotrf (k > 0: leading minor k not positive definite)."""
+    info = C.c_int(0)
+    n = A.shape[1]
+    t = lib().oocdag_cholesky(_ptr(A), n, nb, A.shape[0], memsize, C.byref(info))
+    if info.value < 0:
+        raise OocdagError(f"oocdag_cholesky failed ({info.value}): {last_error()}")
+    return t, info.value
+
+
+def panel_plan(n: int, nb: int, memsize: int, nx: int | None = None):
+    w = (C.c_int * 8192)()
+    if nx is None:
+        k = lib().oocdag_panel_plan(n, nb, memsize, w, 8192)
+    else:
+        k = lib().oocdag_panel_plan_nx(n, nb, memsize, nx, w, 8192)
+    if k < 0:
+        return None
+    return [w[i] for i in range(k)]
+
+
+def pdmatgen(n: int, nb: int = 1, aform: str = "S", diag: str = "D", iseed: int = 100,
+             m: int | None = None) -> np.ndarray:
+    """pdmatgen(...) of oocdag.cpp:298-302 (1x1 grid), computed on the GPU."""
+    m = n if m is None else m
+    A = np.zeros((m, n), dtype=np.float64, order="F")
+    info = lib().oocdag_pdmatgen(aform.encode(), diag.encode(), m, n, nb, nb, _ptr(A),
+                                 max(1, m), iseed)
+    if info != 0:
+        raise OocdagError(f"pdmatgen info={info}: {last_error()}")
+    return A
+
+
+def last_stats() -> dict:
+    v = [C.c_double(0) for _ in range(5)]
+    lib().oocdag_last_stats(*[C.byref(x) for x in v])
+    keys = ["seconds", "h2d_bytes", "d2h_bytes", "h2d_seconds", "d2h_seconds"]
+    return {k: x.value for k, x in zip(keys, v)}
+
+
+# ---------------------------------------------------------------------------
+# device buffers and the HBM-resident path
+# ---------------------------------------------------------------------------
+
+class DeviceBuffer:
+    """A hipMalloc'ed block owned by Python."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = lib().oocdag_dev_alloc(self.nbytes)
+        if not p:
+            raise OocdagError(f"device allocation of {nbytes} bytes failed: {last_error()}")
+        self.ptr = p
+
+    def free(self):
+        if self.ptr:
+            lib().oocdag_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.asfortranarray(a, dtype=np.float64)
+        b = cls(a.nbytes)
+        _check(lib().oocdag_memcpy_h2d(b.ptr, a.ctypes.data, a.nbytes), "h2d")
+        return b
+
+    def to_array(self, shape, order="F") -> np.ndarray:
+        out = np.empty(shape, dtype=np.float64, order=order)
+        _check(lib().oocdag_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+
+def tiles_elems(n: int, nb: int) -> int:
+    return int(lib().oocdag_tiles_elems(n, nb))
+
+
+def pdmatgen_tiles(buf: DeviceBuffer, n: int, nb: int, aform="S", diag="D", iseed=100):
+    _check(lib().oocdag_pdmatgen_tiles(buf.ptr, n, nb, aform.encode(), diag.encode(), iseed, None),
+           "pdmatgen_tiles")
+
+
+def potrf_tiles(buf: DeviceBuffer, n: int, nb: int) -> int:
+    info = C.c_int(0)
+    _check(lib().oocdag_potrf_tiles(buf.ptr, n, nb, None, C.byref(info)), "potrf_tiles")
+    return info.value
+
+
+def tiles_from_host(buf: DeviceBuffer, A: np.ndarray, nb: int):
+    _check(lib().oocdag_tiles_from_host(buf.ptr, _ptr(A), A.shape[1], nb, A.shape[0]),
+           "tiles_from_host")
+
+
+def tiles_to_host(buf: DeviceBuffer, A: np.ndarray, nb: int):
+    _check(lib().oocdag_tiles_to_host(buf.ptr, _ptr(A), A.shape[1], nb, A.shape[0]),
+           "tiles_to_host")
+
+
+# ---------------------------------------------------------------------------
+# tile kernels (cuBLAS / MKL call sites of oocquark.cpp replaced)
+# ---------------------------------------------------------------------------
+
+def dgemm_nt(alpha, A: DeviceBuffer, lda, B: DeviceBuffer, ldb, beta, Cb: DeviceBuffer, ldc,
+             m, n, k):
+    _check(lib().oocdag_dgemm_nt(m, n, k, alpha, A.ptr, lda, B.ptr, ldb, beta, Cb.ptr, ldc, None),
+           "dgemm_nt")
+    _check(lib().oocdag_device_sync(), "sync")
+
+
+def dsyrk_ln(alpha, A: DeviceBuffer, lda, beta, Cb: DeviceBuffer, ldc, n, k):
+    _check(lib().oocdag_dsyrk_ln(n, k, alpha, A.ptr, lda, beta, Cb.ptr, ldc, None), "dsyrk_ln")
+    _check(lib().oocdag_device_sync(), "sync")
+
+
+def dtrsm_rltn(L: DeviceBuffer, ldl, B: DeviceBuffer, ldb, m, n):
+    _check(lib().oocdag_dtrsm_rltn(m, n, L.ptr, ldl, B.ptr, ldb, None), "dtrsm_rltn")
+
+
+def dpotrf_l(A: DeviceBuffer, lda, n) -> int:
+    info = C.c_int(0)
+    _check(lib().oocdag_dpotrf_l(n, A.ptr, lda, C.byref(info), None), "dpotrf_l")
+    return info.value

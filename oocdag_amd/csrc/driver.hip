@@ -1,0 +1,838 @@
+// Host side of liboocdag: the C ABI, the panel planner and the out-of-core
+// scheduler on HIP streams and events.
+//
+// The reference runs Cholesky() (oocdag.cpp:18-112) as a QUARK task DAG on
+// four host threads with cuBLAS on the legacy NULL stream and pageable
+// synchronous copies (oocquark.cpp:16-63, oocdag.cpp:279-289).  Here one host
+// thread drives three HIP streams:
+//   h2d  : pinned-slot -> HBM copies of Y panels and X tile columns,
+//   comp : all kernels (updates, diagonal factorization, solves),
+//   d2h  : factored tile columns back to pinned slots,
+// ordered by events: an X buffer is refilled only after the update that read
+// it (double-buffered X), a factored tile column is copied out as soon as it
+// is final while later columns are still being computed, and host packing of
+// the next X column overlaps the update running on the current one.
+#include "ooc_internal.h"
+#include "../../include/oocdag.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+using namespace ooc;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local double g_stat[5];
+
+void set_err(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+#define HIPTRY(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            set_err("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_));   \
+            return OOCDAG_EHIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+int device_ok() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_err("no HIP device");
+        return 0;
+    }
+    return 1;
+}
+
+// Tiles of the packed lower trapezoid with Ym tile rows and Yn tile columns.
+long long trap_tiles(int Ym, int Yn) { return (long long)Yn * Ym - ((long long)Yn * (Yn - 1)) / 2; }
+
+// find_Yn (oocdag.cpp:478-492) with the reserved X space generalised from one
+// tile column (nx = 1, the reference) to nx columns.  The function-static of
+// the reference becomes the explicit 'prev'.
+int find_width(int bb, long long memBlock, int jb, int nx, int prev) {
+    int Ym = bb - jb;
+    long long room = memBlock - (long long)nx * Ym;  // tiles left for the panel
+    if ((long long)Ym * (Ym + 1) <= room * 2) return Ym;
+    int Yn = prev;
+    while (room * 2 >= (long long)(2 * Ym - Yn) * (Yn + 1)) {
+        if (Yn == Ym) break;
+        ++Yn;
+    }
+    return Yn;
+}
+
+struct Plan {
+    std::vector<int> width;  // tile columns per panel
+    long long tiles_needed = 0;
+    int nx = 1;
+};
+
+// Builds the panel schedule.  Two X buffers when the budget leaves room for
+// at least one panel column next to them, else the reference's single one.
+// force_nx > 0 fixes the X buffer count (1 reproduces find_Yn exactly).
+bool make_plan(int N, int NB, size_t memsize, Plan &pl, int force_nx = 0) {
+    int bb = (N + NB - 1) / NB;
+    // w = memsize/sizeof(double)/BB/NB/NB - 1; assert(w > 0)  (oocdag.cpp:248-250)
+    long long w = (long long)(memsize / sizeof(double) / (size_t)bb / (size_t)NB / (size_t)NB) - 1;
+    if (w <= 0) return false;
+    long long memBlock = (long long)(memsize / sizeof(double) / (size_t)NB / (size_t)NB);
+    for (int nx = 2; nx >= 1; --nx) {
+        if (force_nx > 0 && nx != force_nx) continue;
+        pl.width.clear();
+        pl.tiles_needed = 0;
+        pl.nx = nx;
+        bool ok = true;
+        int prev = 0;
+        for (int jb = 0; jb < bb;) {
+            int Yn = find_width(bb, memBlock, jb, nx, prev);
+            if (Yn <= 0) {
+                ok = false;
+                break;
+            }
+            int Ym = bb - jb;
+            long long need = trap_tiles(Ym, Yn) + (jb > 0 ? (long long)nx * Ym : 0);
+            pl.tiles_needed = std::max(pl.tiles_needed, need);
+            pl.width.push_back(Yn);
+            prev = Yn;
+            jb += Yn;
+        }
+        if (ok) return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// host packing between column-major A (LDA) and NB x NB tiles
+// ---------------------------------------------------------------------------
+
+// Packs tiles (ti0 .. bb-1, tj) of A into dst (consecutive NB^2 tiles),
+// zero-padding past N and putting 1 on the padded diagonal of tile (tj,tj).
+void pack_column(const double *A, int N, int NB, int LDA, int tj, int ti0, double *dst) {
+    const int bb = (N + NB - 1) / NB;
+    const int ntile = bb - ti0;
+    const size_t nb2 = (size_t)NB * NB;
+#pragma omp parallel for schedule(static) collapse(2)
+    for (int t = 0; t < ntile; ++t) {
+        for (int c = 0; c < NB; ++c) {
+            const int ti = ti0 + t;
+            const int gr0 = ti * NB, gc = tj * NB + c;
+            double *d = dst + (size_t)t * nb2 + (size_t)c * NB;
+            const int vr = std::max(0, std::min(NB, N - gr0));
+            if (gc < N) {
+                if (vr > 0) memcpy(d, A + (size_t)gc * LDA + gr0, (size_t)vr * sizeof(double));
+                for (int r = vr; r < NB; ++r) d[r] = 0.0;
+            } else {
+                for (int r = 0; r < NB; ++r) d[r] = (gr0 + r == gc) ? 1.0 : 0.0;
+            }
+        }
+    }
+}
+
+// Writes tiles (ti0 .. bb-1, tj) from src back into A: valid rows/columns
+// only, and only the lower triangle of the diagonal tile.
+void unpack_column(double *A, int N, int NB, int LDA, int tj, int ti0, const double *src) {
+    const int bb = (N + NB - 1) / NB;
+    const int ntile = bb - ti0;
+    const size_t nb2 = (size_t)NB * NB;
+#pragma omp parallel for schedule(static) collapse(2)
+    for (int t = 0; t < ntile; ++t) {
+        for (int c = 0; c < NB; ++c) {
+            const int ti = ti0 + t;
+            const int gr0 = ti * NB, gc = tj * NB + c;
+            if (gc >= N) continue;
+            const double *s = src + (size_t)t * nb2 + (size_t)c * NB;
+            int r0 = (ti == tj) ? c : 0;
+            int vr = std::max(0, std::min(NB, N - gr0));
+            if (vr > r0)
+                memcpy(A + (size_t)gc * LDA + gr0 + r0, s + r0, (size_t)(vr - r0) * sizeof(double));
+        }
+    }
+}
+
+struct Streams {
+    hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr;
+    bool own_comp = true;
+    ~Streams() {
+        if (h2d) (void)hipStreamDestroy(h2d);
+        if (d2h) (void)hipStreamDestroy(d2h);
+        if (comp && own_comp) (void)hipStreamDestroy(comp);
+    }
+};
+
+// Factors tile columns [0, Yn) of a packed panel already resident in HBM
+// (ooc_incore, oocquark.cpp:358-421).  gcol0 = global column of panel
+// column 0.  on_column(j) is called after column j's factor kernels were
+// queued (so the caller can record an event and start its copy-out).
+template <class F>
+int factor_panel(double *dY, int NB, int Ym, int Yn, int gcol0, double *dinv, int *dinfo,
+                 hipStream_t s, F on_column) {
+    Opnd P = panel(dY, NB, Ym, 1);
+    for (int j = 0; j < Yn; ++j) {
+        HIPTRY(factor_column(P, j * NB, (Ym - j) * NB, NB, gcol0 + j * NB, dinv, dinfo, s));
+        int rc = on_column(j);
+        if (rc) return rc;
+        if (j + 1 < Yn) {
+            // right-looking update of the rest of the panel from column j
+            // (ooc_syrk(&Y(jb+1,jb), &Y(jb+1,jb+1), ...), oocquark.cpp:414-418)
+            Gemm g;
+            g.A = P; g.A.r0 = (j + 1) * NB; g.A.c0 = j * NB;
+            g.B = g.A;
+            g.C = P; g.C.r0 = (j + 1) * NB; g.C.c0 = (j + 1) * NB;
+            g.M = (Ym - j - 1) * NB;
+            g.N = (Yn - j - 1) * NB;
+            g.K = NB;
+            g.alpha = -1.0;
+            g.beta = 1.0;
+            g.lower = 1;
+            HIPTRY(launch_gemm(g, GEMM_UPDATE, s));
+        }
+    }
+    return OOCDAG_OK;
+}
+
+int read_info(int *dinfo, hipStream_t s, int N, int *info) {
+    int h = INT_MAX;
+    HIPTRY(hipMemcpyAsync(&h, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    *info = (h == INT_MAX) ? 0 : std::min(h, N);
+    return OOCDAG_OK;
+}
+
+// Device scratch for the diagonal-block inverses and the info flag.
+struct Scratch {
+    double *dinv = nullptr;
+    int *dinfo = nullptr;
+    ~Scratch() {
+        if (dinv) (void)hipFree(dinv);
+        if (dinfo) (void)hipFree(dinfo);
+    }
+    int init(int NB, hipStream_t s) {
+        HIPTRY(hipMalloc(&dinv, (size_t)(NB / 64) * 64 * 64 * sizeof(double)));
+        HIPTRY(hipMalloc(&dinfo, sizeof(int)));
+        int big = INT_MAX;
+        HIPTRY(hipMemcpyAsync(dinfo, &big, sizeof(int), hipMemcpyHostToDevice, s));
+        return OOCDAG_OK;
+    }
+};
+
+int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, double *secs) {
+    Plan pl;
+    if (!make_plan(N, NB, memsize, pl)) {
+        set_err("budget %zu bytes below two tile columns for N=%d NB=%d", memsize, N, NB);
+        return OOCDAG_EBUDGET;
+    }
+    const int bb = (N + NB - 1) / NB;
+    const size_t nb2 = (size_t)NB * NB;
+    const size_t col_bytes = (size_t)bb * nb2 * sizeof(double);
+
+    Streams st;
+    HIPTRY(hipStreamCreateWithFlags(&st.h2d, hipStreamNonBlocking));
+    HIPTRY(hipStreamCreateWithFlags(&st.comp, hipStreamNonBlocking));
+    HIPTRY(hipStreamCreateWithFlags(&st.d2h, hipStreamNonBlocking));
+
+    double *dY = nullptr;
+    if (hipMalloc(&dY, (size_t)pl.tiles_needed * nb2 * sizeof(double)) != hipSuccess) {
+        set_err("device allocation of %lld tiles failed", pl.tiles_needed);
+        return OOCDAG_ENOMEM;
+    }
+    struct FreeDev {
+        double *p;
+        ~FreeDev() { (void)hipFree(p); }
+    } fy{dY};
+    Scratch sc;
+    int rc = sc.init(NB, st.comp);
+    if (rc) return rc;
+
+    // pinned slots: 2 for host->device, 2 for device->host
+    double *slot[4] = {nullptr, nullptr, nullptr, nullptr};
+    struct FreeHost {
+        double **s;
+        ~FreeHost() {
+            for (int i = 0; i < 4; ++i)
+                if (s[i]) (void)hipHostFree(s[i]);
+        }
+    } fh{slot};
+    for (int i = 0; i < 4; ++i)
+        if (hipHostMalloc(&slot[i], col_bytes, hipHostMallocDefault) != hipSuccess) {
+            set_err("pinned host allocation of %zu bytes failed", col_bytes);
+            return OOCDAG_ENOMEM;
+        }
+    hipEvent_t ev_slot[4], ev_xready[2], ev_xfree[2], ev_y, ev_col, ev_t0, ev_t1;
+    for (auto &e : ev_slot) HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &e : ev_xready) HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &e : ev_xfree) HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPTRY(hipEventCreateWithFlags(&ev_y, hipEventDisableTiming));
+    HIPTRY(hipEventCreateWithFlags(&ev_col, hipEventDisableTiming));
+    HIPTRY(hipEventCreate(&ev_t0));
+    HIPTRY(hipEventCreate(&ev_t1));
+    bool slot_used[4] = {false, false, false, false};
+    HIPTRY(hipStreamSynchronize(st.comp));
+
+    double h2d_bytes = 0, d2h_bytes = 0;
+    const double t0 = now();
+    int hslot = 0;
+    int nxb = pl.nx;
+
+    auto take_slot = [&](int s) -> int {
+        if (slot_used[s]) HIPTRY(hipEventSynchronize(ev_slot[s]));
+        slot_used[s] = true;
+        return OOCDAG_OK;
+    };
+
+    for (size_t p = 0, jb = 0; p < pl.width.size(); jb += pl.width[p], ++p) {
+        const int Yn = pl.width[p];
+        const int Ym = bb - (int)jb;
+        // --- A2Y (oocquark.cpp:253-275): the panel's tile columns -> Y
+        for (int jj = 0; jj < Yn; ++jj) {
+            if ((rc = take_slot(hslot))) return rc;
+            const int tj = (int)jb + jj;
+            pack_column(A, N, NB, LDA, tj, tj, slot[hslot]);
+            size_t bytes = (size_t)(bb - tj) * nb2 * sizeof(double);
+            double *dst = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
+            HIPTRY(hipMemcpyAsync(dst, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
+            HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
+            h2d_bytes += bytes;
+            hslot ^= 1;
+        }
+        HIPTRY(hipEventRecord(ev_y, st.h2d));
+        HIPTRY(hipStreamWaitEvent(st.comp, ev_y, 0));
+
+        // --- left-looking updates from every prior tile column (oocdag.cpp:73-77)
+        const long long ytiles = trap_tiles(Ym, Yn);
+        for (int jjb = 0; jjb < (int)jb; ++jjb) {
+            const int b = jjb % nxb;
+            double *X = dY + (size_t)(ytiles + (long long)b * Ym) * nb2;
+            if (jjb >= nxb) HIPTRY(hipStreamWaitEvent(st.h2d, ev_xfree[b], 0));
+            if ((rc = take_slot(hslot))) return rc;
+            // A2X (oocquark.cpp:301-319): rows jb.. of tile column jjb
+            pack_column(A, N, NB, LDA, jjb, (int)jb, slot[hslot]);
+            size_t bytes = (size_t)Ym * nb2 * sizeof(double);
+            HIPTRY(hipMemcpyAsync(X, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
+            HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
+            HIPTRY(hipEventRecord(ev_xready[b], st.h2d));
+            h2d_bytes += bytes;
+            hslot ^= 1;
+            HIPTRY(hipStreamWaitEvent(st.comp, ev_xready[b], 0));
+            // ooc_syrk(X, Y, YM, YN, NB) (oocquark.cpp:321-356) as one launch
+            Gemm g;
+            g.A = panel(X, NB, Ym, 0);
+            g.B = g.A;
+            g.C = panel(dY, NB, Ym, 1);
+            g.M = Ym * NB;
+            g.N = Yn * NB;
+            g.K = NB;
+            g.alpha = -1.0;
+            g.beta = 1.0;
+            g.lower = 1;
+            HIPTRY(launch_gemm(g, GEMM_UPDATE, st.comp));
+            HIPTRY(hipEventRecord(ev_xfree[b], st.comp));
+        }
+
+        // --- in-core factorization of the panel, each finished tile column
+        // copied back while the next ones are computed (oocquark.cpp:389-419)
+        std::vector<int> pending;
+        auto drain_one = [&]() -> int {
+            // oldest pending column: wait for its copy and unpack it into A
+            int jj = pending.front();
+            pending.erase(pending.begin());
+            int s = 2 + (jj & 1);
+            HIPTRY(hipEventSynchronize(ev_slot[s]));
+            unpack_column(A, N, NB, LDA, (int)jb + jj, (int)jb + jj, slot[s]);
+            slot_used[s] = false;
+            return OOCDAG_OK;
+        };
+        rc = factor_panel(dY, NB, Ym, Yn, (int)jb * NB, sc.dinv, sc.dinfo, st.comp,
+                          [&](int jj) -> int {
+                              int s = 2 + (jj & 1);
+                              if (slot_used[s]) {
+                                  int r = drain_one();
+                                  if (r) return r;
+                              }
+                              HIPTRY(hipEventRecord(ev_col, st.comp));
+                              HIPTRY(hipStreamWaitEvent(st.d2h, ev_col, 0));
+                              size_t bytes = (size_t)(Ym - jj) * nb2 * sizeof(double);
+                              const double *src = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
+                              HIPTRY(hipMemcpyAsync(slot[s], src, bytes, hipMemcpyDeviceToHost,
+                                                    st.d2h));
+                              HIPTRY(hipEventRecord(ev_slot[s], st.d2h));
+                              slot_used[s] = true;
+                              pending.push_back(jj);
+                              d2h_bytes += bytes;
+                              return OOCDAG_OK;
+                          });
+        if (rc) return rc;
+        while (!pending.empty())
+            if ((rc = drain_one())) return rc;
+        // the next panel overwrites Y: everything on this panel must be done
+        HIPTRY(hipStreamSynchronize(st.comp));
+        HIPTRY(hipStreamSynchronize(st.d2h));
+    }
+    HIPTRY(hipStreamSynchronize(st.h2d));
+    rc = read_info(sc.dinfo, st.comp, N, info);
+    *secs = now() - t0;
+    g_stat[0] = *secs;
+    g_stat[1] = h2d_bytes;
+    g_stat[2] = d2h_bytes;
+    for (auto &e : ev_slot) (void)hipEventDestroy(e);
+    for (auto &e : ev_xready) (void)hipEventDestroy(e);
+    for (auto &e : ev_xfree) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ev_y);
+    (void)hipEventDestroy(ev_col);
+    (void)hipEventDestroy(ev_t0);
+    (void)hipEventDestroy(ev_t1);
+    return rc;
+}
+
+int check_nb(int N, int NB) {
+    if (N < 0 || NB < 64 || NB % 64 != 0) {
+        set_err("N=%d NB=%d: NB must be a positive multiple of 64", N, NB);
+        return OOCDAG_EARG;
+    }
+    return OOCDAG_OK;
+}
+
+// ---- general-matrix generator for the host-array entry point -------------
+
+struct GenColArgs {
+    unsigned pow2[40][2];
+    unsigned x1;
+    double *D;       // column-major M x ncols, ld M
+    int M, N, col0, ncols;
+    int mode;        // 0 general (j*M+i), 1 transposed (i*N+j), 2 symmetric, 3 'F'
+};
+
+__global__ void k_gen_cols(GenColArgs g) {
+    long long total = (long long)g.M * g.ncols;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(idx % g.M), jl = (int)(idx / g.M);
+        int j = g.col0 + jl;
+        double v;
+        if (g.mode == 3) {
+            v = (i == j) ? 10.0 : (double)0.1f;
+        } else {
+            unsigned long long p;
+            if (g.mode == 0)
+                p = (unsigned long long)j * g.M + i;
+            else if (g.mode == 1)
+                p = (unsigned long long)i * g.N + j;
+            else
+                p = (i >= j) ? (unsigned long long)j * g.M + i : (unsigned long long)i * g.N + j;
+            unsigned x = g.x1;
+            for (int k = 0; p; ++k, p >>= 1)
+                if (p & 1ull) x = (g.pow2[k][0] * x + g.pow2[k][1]) & 0x7fffffffu;
+            v = 1.0 - 2.0 * ((double)x / 2147483648.0);
+        }
+        g.D[idx] = v;
+    }
+}
+
+void lcg_pow_host(unsigned long long n, unsigned &a, unsigned &c) {
+    unsigned ra = 1, rc = 0, ba = 1103515245u, bc = 12345u;
+    while (n) {
+        if (n & 1ull) {
+            unsigned na = (ba * ra) & 0x7fffffffu;
+            unsigned nc = (ba * rc + bc) & 0x7fffffffu;
+            ra = na;
+            rc = nc;
+        }
+        unsigned na = (ba * ba) & 0x7fffffffu;
+        unsigned nc = (ba * bc + bc) & 0x7fffffffu;
+        ba = na;
+        bc = nc;
+        n >>= 1;
+    }
+    a = ra;
+    c = rc;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *oocdag_last_error(void) { return g_err.c_str(); }
+
+int oocdag_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int oocdag_last_stats(double *seconds, double *h2d_bytes, double *d2h_bytes, double *h2d_seconds,
+                      double *d2h_seconds) {
+    if (seconds) *seconds = g_stat[0];
+    if (h2d_bytes) *h2d_bytes = g_stat[1];
+    if (d2h_bytes) *d2h_bytes = g_stat[2];
+    if (h2d_seconds) *h2d_seconds = g_stat[3];
+    if (d2h_seconds) *d2h_seconds = g_stat[4];
+    return OOCDAG_OK;
+}
+
+int oocdag_panel_plan(int N, int NB, size_t memsize, int *widths, int maxp) {
+    return oocdag_panel_plan_nx(N, NB, memsize, 0, widths, maxp);
+}
+
+int oocdag_panel_plan_nx(int N, int NB, size_t memsize, int nx, int *widths, int maxp) {
+    if (N <= 0 || NB <= 0 || nx < 0 || nx > 2) return OOCDAG_EARG;
+    Plan pl;
+    if (!make_plan(N, NB, memsize, pl, nx)) return OOCDAG_EBUDGET;
+    for (size_t i = 0; i < pl.width.size() && (int)i < maxp; ++i) widths[i] = pl.width[i];
+    return (int)pl.width.size();
+}
+
+double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *info) {
+    g_err.clear();
+    for (double &s : g_stat) s = 0;
+    int dummy;
+    if (!info) info = &dummy;
+    *info = 0;
+    if (!A || N < 0 || LDA < std::max(1, N) || check_nb(N, NB)) {
+        if (g_err.empty()) set_err("invalid argument");
+        *info = OOCDAG_EARG;
+        return -1.0;
+    }
+    if (N == 0) return 0.0;
+    if (!device_ok()) {
+        *info = OOCDAG_ENODEV;
+        return -1.0;
+    }
+    double secs = 0;
+    int rc = cholesky_ooc(A, N, NB, LDA, memsize, info, &secs);
+    if (rc) {
+        *info = rc;
+        return -1.0;
+    }
+    return secs;
+}
+
+size_t oocdag_tiles_elems(int N, int NB) {
+    if (N <= 0 || NB <= 0) return 0;
+    size_t bb = (size_t)((N + NB - 1) / NB);
+    return bb * (bb + 1) / 2 * (size_t)NB * NB;
+}
+
+int oocdag_potrf_tiles(double *dY, int N, int NB, void *stream, int *info) {
+    g_err.clear();
+    int dummy;
+    if (!info) info = &dummy;
+    *info = 0;
+    if (!dY || check_nb(N, NB)) return OOCDAG_EARG;
+    if (N == 0) return OOCDAG_OK;
+    if (!device_ok()) return OOCDAG_ENODEV;
+    const int bb = (N + NB - 1) / NB;
+    Streams st;
+    if (stream) {
+        st.comp = (hipStream_t)stream;
+        st.own_comp = false;
+    } else {
+        HIPTRY(hipStreamCreateWithFlags(&st.comp, hipStreamNonBlocking));
+    }
+    Scratch sc;
+    int rc = sc.init(NB, st.comp);
+    if (rc) return rc;
+    const double t0 = now();
+    rc = factor_panel(dY, NB, bb, bb, 0, sc.dinv, sc.dinfo, st.comp, [](int) { return 0; });
+    if (rc) return rc;
+    rc = read_info(sc.dinfo, st.comp, N, info);
+    g_stat[0] = now() - t0;
+    g_stat[1] = g_stat[2] = g_stat[3] = g_stat[4] = 0;
+    return rc;
+}
+
+int oocdag_pdmatgen_tiles(double *dY, int N, int NB, char aform, char diag, int iseed,
+                          void *stream) {
+    g_err.clear();
+    if (!dY || N <= 0 || NB <= 0) return OOCDAG_EARG;
+    char af = (char)(aform & ~0x20), dg = (char)(diag & ~0x20);
+    if (!(af == 'S' || af == 'H' || af == 'F') || !(dg == 'D' || dg == 'N')) {
+        set_err("tile generator supports AFORM S/H/F and DIAG D/N");
+        return OOCDAG_EARG;
+    }
+    if (!device_ok()) return OOCDAG_ENODEV;
+    hipStream_t s = (hipStream_t)stream;
+    const int bb = (N + NB - 1) / NB;
+    // one launch per group of tile columns keeps the grid within 2^31 blocks
+    for (int j0 = 0; j0 < bb; j0 += 16) {
+        int j1 = std::min(bb, j0 + 16);
+        HIPTRY(launch_gen_tiles(dY, N, NB, bb, j0, j1, aform, diag, iseed, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    return OOCDAG_OK;
+}
+
+int oocdag_tiles_from_host(double *dY, const double *A, int N, int NB, int LDA) {
+    g_err.clear();
+    if (!dY || !A || N <= 0 || NB <= 0 || LDA < N) return OOCDAG_EARG;
+    const int bb = (N + NB - 1) / NB;
+    const size_t nb2 = (size_t)NB * NB;
+    std::vector<double> buf((size_t)bb * nb2);
+    for (int tj = 0; tj < bb; ++tj) {
+        pack_column(A, N, NB, LDA, tj, tj, buf.data());
+        HIPTRY(hipMemcpy(dY + (size_t)tile_index(tj, tj, bb, 1) * nb2, buf.data(),
+                         (size_t)(bb - tj) * nb2 * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return OOCDAG_OK;
+}
+
+int oocdag_tiles_to_host(const double *dY, double *A, int N, int NB, int LDA) {
+    g_err.clear();
+    if (!dY || !A || N <= 0 || NB <= 0 || LDA < N) return OOCDAG_EARG;
+    const int bb = (N + NB - 1) / NB;
+    const size_t nb2 = (size_t)NB * NB;
+    std::vector<double> buf((size_t)bb * nb2);
+    for (int tj = 0; tj < bb; ++tj) {
+        HIPTRY(hipMemcpy(buf.data(), dY + (size_t)tile_index(tj, tj, bb, 1) * nb2,
+                         (size_t)(bb - tj) * nb2 * sizeof(double), hipMemcpyDeviceToHost));
+        unpack_column(A, N, NB, LDA, tj, tj, buf.data());
+    }
+    return OOCDAG_OK;
+}
+
+int oocdag_pdmatgen(char aform, char diag, int M, int N, int MB, int NB, double *A, int LDA,
+                    int iseed) {
+    g_err.clear();
+    char af = (char)(aform & ~0x20), dg = (char)(diag & ~0x20);
+    int symm = (af == 'S' || af == 'H');
+    int tran = (af == 'T' || af == 'C');
+    // argument checks of LIN/pdmatgen.f:156-195 (1x1 grid)
+    int inf = 0;
+    if (dg != 'D' && dg != 'N')
+        inf = 3;
+    else if (symm) {
+        if (M != N)
+            inf = 5;
+        else if (MB != NB)
+            inf = 7;
+    } else if (M < 0)
+        inf = 4;
+    else if (N < 0)
+        inf = 5;
+    else if (MB < 1)
+        inf = 6;
+    else if (NB < 1)
+        inf = 7;
+    else if (LDA < 0)
+        inf = 9;
+    if (inf) return inf;
+    if (M == 0 || N == 0) return 0;
+    if (!A || LDA < M) return OOCDAG_EARG;
+    if (!device_ok()) return OOCDAG_ENODEV;
+    GenColArgs g;
+    for (int k = 0; k < 40; ++k) lcg_pow_host(1ull << k, g.pow2[k][0], g.pow2[k][1]);
+    g.x1 = (1103515245u * ((unsigned)iseed & 0x7fffffffu) + 12345u) & 0x7fffffffu;
+    g.M = M;
+    g.N = N;
+    g.mode = (af == 'F') ? 3 : symm ? 2 : tran ? 1 : 0;
+    // column chunks of ~128 MiB
+    int chunk = (int)std::max<long long>(1, (128ll << 20) / 8 / std::max(1, M));
+    chunk = std::min(chunk, N);
+    double *d = nullptr;
+    HIPTRY(hipMalloc(&d, (size_t)M * chunk * sizeof(double)));
+    struct F {
+        double *p;
+        ~F() { (void)hipFree(p); }
+    } fd{d};
+    for (int c0 = 0; c0 < N; c0 += chunk) {
+        g.col0 = c0;
+        g.ncols = std::min(chunk, N - c0);
+        g.D = d;
+        long long total = (long long)M * g.ncols;
+        int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_gen_cols, dim3(blocks), dim3(256), 0, 0, g);
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipMemcpy2D(A + (size_t)c0 * LDA, (size_t)LDA * sizeof(double), d,
+                           (size_t)M * sizeof(double), (size_t)M * sizeof(double), g.ncols,
+                           hipMemcpyDeviceToHost));
+    }
+    if (dg == 'D' && MB == NB) {  // LIN/pdmatgen.f:516-543
+        double maxmn = (double)std::max(M, N);
+        for (int i = 0; i < std::min(M, N); ++i) {
+            double *a = A + i + (size_t)i * LDA;
+            *a = std::fabs(*a) + maxmn;
+        }
+    }
+    return 0;
+}
+
+int oocdag_dgemm_nt(int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                    int ldb, double beta, double *C, int ldc, void *stream) {
+    g_err.clear();
+    if (m < 0 || n < 0 || k < 0 || lda < std::max(1, m) || ldb < std::max(1, n) ||
+        ldc < std::max(1, m))
+        return OOCDAG_EARG;
+    Gemm g;
+    g.A = plain((double *)A, lda);
+    g.B = plain((double *)B, ldb);
+    g.C = plain(C, ldc);
+    g.M = m;
+    g.N = n;
+    g.K = k;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.lower = 0;
+    hipError_t e = launch_gemm(g, GEMM_GENERAL, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) {
+        set_err("dgemm_nt: leading dimensions must be even and pointers 16-byte aligned");
+        return OOCDAG_EARG;
+    }
+    HIPTRY(e);
+    return OOCDAG_OK;
+}
+
+int oocdag_dsyrk_ln(int n, int k, double alpha, const double *A, int lda, double beta, double *C,
+                    int ldc, void *stream) {
+    g_err.clear();
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return OOCDAG_EARG;
+    Gemm g;
+    g.A = plain((double *)A, lda);
+    g.B = g.A;
+    g.C = plain(C, ldc);
+    g.M = n;
+    g.N = n;
+    g.K = k;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.lower = 1;
+    hipError_t e = launch_gemm(g, GEMM_UPDATE, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) {
+        set_err("dsyrk_ln: leading dimensions must be even and pointers 16-byte aligned");
+        return OOCDAG_EARG;
+    }
+    HIPTRY(e);
+    return OOCDAG_OK;
+}
+
+int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream) {
+    g_err.clear();
+    int dummy;
+    if (!info) info = &dummy;
+    *info = 0;
+    if (n < 0 || n % 64 || lda < std::max(1, n) || (lda & 1)) {
+        set_err("dpotrf_l: n must be a multiple of 64 and lda even");
+        return OOCDAG_EARG;
+    }
+    if (n == 0) return OOCDAG_OK;
+    hipStream_t s = (hipStream_t)stream;
+    Scratch sc;
+    // the left-looking sweep keeps one 64x64 inverse per block column
+    HIPTRY(hipMalloc(&sc.dinv, (size_t)(n / 64) * 64 * 64 * sizeof(double)));
+    HIPTRY(hipMalloc(&sc.dinfo, sizeof(int)));
+    int big = INT_MAX;
+    HIPTRY(hipMemcpyAsync(sc.dinfo, &big, sizeof(int), hipMemcpyHostToDevice, s));
+    HIPTRY(factor_column(plain(A, lda), 0, n, n, 0, sc.dinv, sc.dinfo, s));
+    return read_info(sc.dinfo, s, n, info);
+}
+
+int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb, void *stream) {
+    g_err.clear();
+    if (m < 0 || n < 0 || n % 64 || ldl < std::max(1, n) || ldb < std::max(1, m) || (ldl & 1) ||
+        (ldb & 1)) {
+        set_err("dtrsm_rltn: n must be a multiple of 64, leading dimensions even");
+        return OOCDAG_EARG;
+    }
+    if (m == 0 || n == 0) return OOCDAG_OK;
+    hipStream_t s = (hipStream_t)stream;
+    // Invert L's 64x64 diagonal blocks in LDS (L itself is not modified),
+    // then sweep block columns left to right:
+    //   B[:, c] -= B[:, :c] L[c, :c]^T ;  B[:, c] <- B[:, c] D_c^T
+    const int nblk = n / 64;
+    double *dinv = nullptr;
+    HIPTRY(hipMalloc(&dinv, (size_t)nblk * 64 * 64 * sizeof(double)));
+    struct F {
+        double *p;
+        ~F() { (void)hipFree(p); }
+    } fw{dinv};
+    for (int c = 0; c < nblk; ++c)
+        HIPTRY(launch_potrf64(plain((double *)L, ldl, c * 64, c * 64), 0, dinv + (size_t)c * 4096,
+                              nullptr, s));
+    for (int c = 0; c < nblk; ++c) {
+        if (c > 0) {
+            Gemm g;
+            g.A = plain(B, ldb, 0, 0);
+            g.B = plain((double *)L, ldl, c * 64, 0);
+            g.C = plain(B, ldb, 0, c * 64);
+            g.M = m;
+            g.N = 64;
+            g.K = c * 64;
+            g.alpha = -1.0;
+            g.beta = 1.0;
+            g.lower = 0;
+            HIPTRY(launch_gemm(g, GEMM_NARROW, s));
+        }
+        Gemm g;
+        g.A = plain(B, ldb, 0, c * 64);
+        g.C = g.A;
+        g.B = plain(dinv + (size_t)c * 4096, 64);
+        g.M = m;
+        g.N = 64;
+        g.K = 64;
+        g.alpha = 1.0;
+        g.beta = 0.0;
+        g.lower = 0;
+        HIPTRY(launch_gemm(g, GEMM_APPLY, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    return OOCDAG_OK;
+}
+
+void *oocdag_dev_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        set_err("hipMalloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+int oocdag_dev_free(void *p) {
+    HIPTRY(hipFree(p));
+    return OOCDAG_OK;
+}
+
+int oocdag_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    HIPTRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return OOCDAG_OK;
+}
+
+int oocdag_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    HIPTRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return OOCDAG_OK;
+}
+
+int oocdag_memset_d(void *dst, int value, size_t bytes) {
+    HIPTRY(hipMemset(dst, value, bytes));
+    return OOCDAG_OK;
+}
+
+int oocdag_device_sync(void) {
+    HIPTRY(hipDeviceSynchronize());
+    return OOCDAG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,291 @@
+// Diagonal-block factorization, triangular-solve apply, padding and the
+// on-device PDMATGEN for the out-of-core Cholesky.
+//
+// The reference factors each diagonal NB x NB tile on the host with MKL
+// dpotrf_ (oocquark.cpp:86-87,393-396) and then solves every sub-diagonal
+// tile with cublasDtrsm(R,L,T,N) (oocquark.cpp:130,405-408).  Here a whole
+// tile column (diagonal tile plus everything below it in the panel) is
+// factored on the device by a blocked left-looking sweep over 64-wide block
+// columns:
+//   1. P[c:, c] -= P[c:, :c] * P[c, :c]^T       (GEMM, 128x64 MFMA tiles)
+//   2. L_cc = chol(P[c, c]); D_c = L_cc^-1      (one workgroup, LDS-resident)
+//   3. P[c+1:, c] <- P[c+1:, c] * D_c^T          (GEMM apply, 128x64 tiles)
+// which is POTRF of the diagonal tile followed by the TRSM of the tiles
+// below it, with no host round trip.
+#include "ooc_internal.h"
+
+#include <cmath>
+
+namespace ooc {
+
+namespace {
+
+constexpr int PB = 64;        // diagonal block edge
+constexpr int PS = PB + 1;    // LDS column stride (odd: conflict-free column walks)
+
+__device__ __forceinline__ long long opnd_off(const Opnd &o, int r, int c) {
+    int gr = o.r0 + r, gc = o.c0 + c;
+    if (o.ts <= 0) return gr + (long long)gc * o.ld;
+    int ti = gr / o.ts, tk = gc / o.ts;
+    long long t = ti + (long long)tk * o.ym - (o.tri ? ((long long)tk * (tk + 1)) / 2 : 0);
+    return t * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
+}
+
+// One workgroup factors a 64 x 64 diagonal block held in LDS (right-looking,
+// one pivot column per step) and inverts the resulting triangle.
+// do_factor = 0: the block already holds a lower-triangular factor; only
+// its inverse is formed and the block is left untouched.
+__global__ __launch_bounds__(256) void k_potrf64(Opnd P, int gcol0, double *dinv, int *info,
+                                                 int do_factor) {
+    __shared__ double S[PB * PS];
+    __shared__ double X[PB * PS];
+    const int tid = threadIdx.x;
+    const long long base = opnd_off(P, 0, 0);  // block never straddles a tile
+    const long long ld = P.ld;
+    double *g = P.p + base;
+
+    for (int idx = tid; idx < PB * PB; idx += 256) {
+        int r = idx & (PB - 1), c = idx >> 6;
+        S[c * PS + r] = (r >= c) ? g[r + c * ld] : 0.0;
+    }
+    __syncthreads();
+
+    for (int k = 0; k < PB && do_factor; ++k) {
+        if (tid == 0) {
+            double d = S[k * PS + k];
+            if (!(d > 0.0)) atomicMin(info, gcol0 + k + 1);
+            S[k * PS + k] = sqrt(d);
+        }
+        __syncthreads();
+        const double dk = S[k * PS + k];
+        if (tid > k && tid < PB) S[k * PS + tid] /= dk;
+        __syncthreads();
+        // trailing lower update: S[i][j] -= S[i][k]*S[j][k], k < j <= i
+        for (int idx = tid; idx < PB * PB; idx += 256) {
+            int i = idx & (PB - 1), j = idx >> 6;
+            if (j > k && i >= j) S[j * PS + i] -= S[k * PS + i] * S[k * PS + j];
+        }
+        __syncthreads();
+    }
+
+    // write L back (lower triangle only)
+    if (do_factor)
+        for (int idx = tid; idx < PB * PB; idx += 256) {
+            int r = idx & (PB - 1), c = idx >> 6;
+            if (r >= c) g[r + c * ld] = S[c * PS + r];
+        }
+
+    // X = L^-1 by forward substitution, one column per thread.  Four threads
+    // share a column: each owns every fourth row, and the row-ordered sweep
+    // exchanges the finished entries through LDS.
+    const int col = tid >> 2, part = tid & 3;
+    for (int i = 0; i < PB; ++i) {
+        if (i >= col && (i & 3) == part) {
+            double s = (i == col) ? 1.0 : 0.0;
+            for (int d = col; d < i; ++d) s -= S[d * PS + i] * X[col * PS + d];
+            X[col * PS + i] = s / S[i * PS + i];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < PB * PB; idx += 256) {
+        int r = idx & (PB - 1), c = idx >> 6;
+        dinv[r + c * PB] = (r >= c) ? X[c * PS + r] : 0.0;
+    }
+}
+
+__global__ void k_pad_tile(double *t, int nb, int vr, int vc, int diag) {
+    long long n = (long long)nb * nb;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        int r = (int)(idx % nb), c = (int)(idx / nb);
+        if (r >= vr || c >= vc) t[idx] = (diag && r == c) ? 1.0 : 0.0;
+    }
+}
+
+// ---- PDMATGEN on the device ----------------------------------------------
+// Stream position of element (i, j) of the symmetric form (LIN/pdmatgen.f:
+// 248-379, 1x1 grid): min(i,j)*N + max(i,j) steps past x1 = a*ISEED + c.
+// The per-element value is 1 - 2*x/2^31 (:261); DIAG 'D' turns the diagonal
+// into |a_ii| + N (:516-543); AFORM 'F' is 10 on the diagonal and the REAL
+// literal 0.1 elsewhere (:546-626).
+struct LcgMap {
+    unsigned a, c;
+};
+struct GenArgs {
+    LcgMap pow2[40];   // 2^k-step maps
+    LcgMap stepN;      // N-step map
+    unsigned x1;
+    double *Y;
+    int N, nb, bb, jt0, jt1;
+    int symm_F;        // 1: AFORM 'F'
+    int diagD;
+    long long ntiles;  // tiles in the column range
+    int runs;          // column runs per tile
+};
+
+constexpr int GEN_RUN = 32;
+
+__device__ __forceinline__ unsigned lcg_apply(LcgMap m, unsigned x) {
+    return (m.a * x + m.c) & 0x7fffffffu;
+}
+
+__global__ __launch_bounds__(256) void k_gen_tiles(GenArgs g) {
+    // block -> (tile, 256-row chunk, run of GEN_RUN columns)
+    const int chunks = (g.nb + 255) / 256;
+    long long b = blockIdx.x;
+    int run = (int)(b % g.runs);
+    b /= g.runs;
+    int chunk = (int)(b % chunks);
+    long long t = b / chunks;
+    if (t >= g.ntiles) return;
+    // tile t (column-major over the lower tiles of columns jt0..) -> (ti, tj)
+    int tj = g.jt0;
+    long long rem = t;
+    while (rem >= (long long)(g.bb - tj)) {
+        rem -= (g.bb - tj);
+        ++tj;
+    }
+    int ti = tj + (int)rem;
+    int r = chunk * 256 + threadIdx.x;
+    if (r >= g.nb) return;
+    long long tidx = ti + (long long)tj * g.bb - ((long long)tj * (tj + 1)) / 2;
+    double *tile = g.Y + tidx * (long long)g.nb * g.nb;
+    const int gi = ti * g.nb + r;
+    const int cbeg = run * GEN_RUN;
+    const int cend = min(cbeg + GEN_RUN, g.nb);
+    const double c01 = (double)0.1f;
+    const double fN = (double)g.N;
+    // starting stream state for (gi, gj0)
+    int gj = tj * g.nb + cbeg;
+    unsigned x = 0;
+    bool have = false;
+    for (int c = cbeg; c < cend; ++c, ++gj) {
+        double v;
+        if (gi >= g.N || gj >= g.N) {
+            v = (gi == gj) ? 1.0 : 0.0;
+        } else if (g.symm_F) {
+            v = (gi == gj) ? 10.0 : c01;
+        } else {
+            if (!have) {
+                unsigned long long p = (gi >= gj) ? (unsigned long long)gj * g.N + gi
+                                                  : (unsigned long long)gi * g.N + gj;
+                x = g.x1;
+                for (int k = 0; p; ++k, p >>= 1)
+                    if (p & 1ull) x = lcg_apply(g.pow2[k], x);
+                have = true;
+            } else {
+                // moving one column right: position +N below the diagonal,
+                // +1 above it
+                x = (gi >= gj) ? lcg_apply(g.stepN, x) : lcg_apply(g.pow2[0], x);
+            }
+            v = 1.0 - 2.0 * ((double)x / 2147483648.0);
+            if (gi == gj && g.diagD) v = fabs(v) + fN;
+        }
+        tile[r + (long long)c * g.nb] = v;
+    }
+}
+
+LcgMap lcg_then(LcgMap f, LcgMap g) {  // g after f
+    LcgMap r;
+    r.a = (g.a * f.a) & 0x7fffffffu;
+    r.c = (g.a * f.c + g.c) & 0x7fffffffu;
+    return r;
+}
+
+LcgMap lcg_pow(unsigned long long n) {
+    LcgMap r{1u, 0u}, b{1103515245u, 12345u};
+    while (n) {
+        if (n & 1ull) r = lcg_then(r, b);
+        b = lcg_then(b, b);
+        n >>= 1;
+    }
+    return r;
+}
+
+}  // namespace
+
+hipError_t launch_potrf64(const Opnd &P, int gcol0, double *dinv, int *info, hipStream_t s) {
+    hipLaunchKernelGGL(k_potrf64, dim3(1), dim3(256), 0, s, P, gcol0, dinv, info,
+                       info != nullptr ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t factor_column(const Opnd &P, int row0, int mrows, int ncols, int gcol_base,
+                         double *dinv_ws, int *info, hipStream_t s) {
+    if (ncols % PB) return hipErrorInvalidValue;
+    const int nblk = ncols / PB;
+    for (int c = 0; c < nblk; ++c) {
+        const int r = row0 + c * PB;
+        hipError_t e;
+        if (c > 0) {
+            Gemm g;
+            g.A = P; g.A.r0 = r; g.A.c0 = row0;
+            g.B = g.A;
+            g.C = P; g.C.r0 = r; g.C.c0 = r;
+            g.M = mrows - c * PB;
+            g.N = PB;
+            g.K = c * PB;
+            g.alpha = -1.0;
+            g.beta = 1.0;
+            g.lower = 1;
+            if ((e = launch_gemm(g, GEMM_NARROW, s)) != hipSuccess) return e;
+        }
+        Opnd D = P;
+        D.r0 = r;
+        D.c0 = r;
+        double *dinv = dinv_ws + (size_t)c * PB * PB;
+        if ((e = launch_potrf64(D, gcol_base + c * PB, dinv, info, s)) != hipSuccess) return e;
+        const int below = mrows - (c + 1) * PB;
+        if (below > 0) {
+            Gemm g;
+            g.A = P; g.A.r0 = r + PB; g.A.c0 = r;
+            g.C = g.A;
+            g.B = plain(dinv, PB);
+            g.M = below;
+            g.N = PB;
+            g.K = PB;
+            g.alpha = 1.0;
+            g.beta = 0.0;
+            g.lower = 0;
+            if ((e = launch_gemm(g, GEMM_APPLY, s)) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_pad_tile(double *tile, int nb, int vr, int vc, int diag, hipStream_t s) {
+    if (vr >= nb && vc >= nb) return hipSuccess;
+    long long n = (long long)nb * nb;
+    int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_pad_tile, dim3(blocks), dim3(256), 0, s, tile, nb, vr, vc, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_tiles(double *Y, int N, int nb, int bb, int jt0, int jt1, char aform,
+                            char diag, int iseed, hipStream_t s) {
+    if (jt1 <= jt0) return hipSuccess;
+    GenArgs g;
+    for (int k = 0; k < 40; ++k) g.pow2[k] = lcg_pow(1ull << k);
+    g.stepN = lcg_pow((unsigned long long)N);
+    g.x1 = (1103515245u * ((unsigned)iseed & 0x7fffffffu) + 12345u) & 0x7fffffffu;
+    g.Y = Y;
+    g.N = N;
+    g.nb = nb;
+    g.bb = bb;
+    g.jt0 = jt0;
+    g.jt1 = jt1;
+    char af = (char)(aform & ~0x20), dg = (char)(diag & ~0x20);
+    g.symm_F = (af == 'F');
+    g.diagD = (dg == 'D');
+    long long nt = 0;
+    for (int j = jt0; j < jt1; ++j) nt += bb - j;
+    g.ntiles = nt;
+    g.runs = (nb + GEN_RUN - 1) / GEN_RUN;
+    const int chunks = (nb + 255) / 256;
+    long long blocks = nt * chunks * g.runs;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gen_tiles, dim3((unsigned)blocks), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+}  // namespace ooc

@@ -1,0 +1,335 @@
+// FP64 MFMA GEMM-NT for the Cholesky panel updates on CDNA4 (gfx950).
+//
+//   C = beta*C + alpha * A * B^T       (A: M x K, B: N x K, C: M x N)
+//
+// This one kernel family carries every dense update of the factorization:
+//  * the left-looking update of a Y panel from a staged X tile column
+//    (ooc_syrk, oocquark.cpp:321-356: SYRK on diagonal tiles, GEMM below),
+//  * the right-looking update inside a panel (oocquark.cpp:413-418),
+//  * the updates and triangular-solve applies of the blocked diagonal-tile
+//    factorization (oocquark.cpp:393-410).
+// The reference issued one cuBLAS call per NB x NB tile
+// (oocquark.cpp:130,176,222); here one launch covers the whole lower
+// trapezoid of a panel, and SYRK is the same kernel with blocks above the
+// diagonal skipped and the diagonal blocks stored lower-only.
+//
+// Work decomposition: 256 threads (4 wave64s as 2 x 2) per 128 x BN output
+// block; each wave owns a 64 x (BN/2) sub-block held as 16x16 FP64
+// accumulators of v_mfma_f64_16x16x4_f64.  K advances 16 at a time through
+// double-buffered LDS (one barrier per step); the next step's global loads
+// are issued before the current step's MFMAs.  Block order is remapped so
+// that each XCD (8 per MI355X, round-robin dispatch) receives a contiguous
+// range of blocks that share operand rows in its own L2.
+#include "ooc_internal.h"
+
+namespace ooc {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+namespace {
+
+constexpr int BM = 128;
+constexpr int BK = 16;
+constexpr int NT = 256;
+constexpr int LPAD = 16;  // doubles of padding per LDS row (row stride = 16 mod 32 doubles)
+
+struct KArgs {
+    Opnd A, B, C;
+    int M, N, K;
+    double alpha, beta;
+    int lower;
+    int mb, nb;       // block-grid extent
+    long long nblk;   // blocks launched
+};
+
+__device__ __forceinline__ long long tix(int ti, int tk, int ym, int tri) {
+    return ti + (long long)tk * ym - (tri ? ((long long)tk * (tk + 1)) / 2 : 0);
+}
+
+// Streams one operand's rows [rb + 2*rp, +2) along K for this thread.
+template <int BX>
+struct Stream {
+    static constexpr int PPC = BX / 2;       // row pairs per column
+    static constexpr int CPP = NT / PPC;     // columns per pass
+    static constexpr int NPASS = BK / CPP;   // passes per K step
+
+    const double *p;
+    long long ld, te, tbase, roff;
+    int ts, ym, tri, ti, tk, kc, col0;
+    bool ok0, ok1;
+
+    __device__ __forceinline__ void init(const Opnd &o, int rb, int rows, int tid) {
+        int rp = tid % PPC;
+        col0 = tid / PPC;
+        int r = rb + 2 * rp;
+        ok0 = r < rows;
+        ok1 = (r + 1) < rows;
+        p = o.p;
+        ld = o.ld;
+        te = o.te;
+        ym = o.ym;
+        tri = o.tri;
+        int gr = o.r0 + r;
+        int gk = o.c0;
+        if (o.ts > 0) {
+            ts = o.ts;
+            ti = gr / ts;
+            roff = gr - (long long)ti * ts;
+            tk = gk / ts;
+            kc = gk - tk * ts;
+            tbase = tix(ti, tk, ym, tri) * te + roff;
+        } else {
+            ts = 0x40000000;
+            ti = 0;
+            roff = gr;
+            tk = 0;
+            kc = gk;
+            tbase = roff;
+        }
+    }
+
+    __device__ __forceinline__ void load(d2 (&v)[NPASS], int k0, int K) const {
+#pragma unroll
+        for (int s = 0; s < NPASS; ++s) {
+            int c = col0 + s * CPP;
+            d2 x = {0.0, 0.0};
+            if (ok0 && (k0 + c) < K) {
+                const double *q = p + tbase + (long long)(kc + c) * ld;
+                if (ok1) {
+                    x = *reinterpret_cast<const d2 *>(q);
+                } else {
+                    x.x = q[0];
+                }
+            }
+            v[s] = x;
+        }
+    }
+
+    __device__ __forceinline__ void advance() {
+        kc += BK;
+        if (kc >= ts) {
+            kc -= ts;
+            ++tk;
+            tbase = tix(ti, tk, ym, tri) * te + roff;
+        }
+    }
+
+    __device__ __forceinline__ void store_lds(double *lds, int stride, const d2 (&v)[NPASS],
+                                              int tid) const {
+        int rp = tid % PPC;
+#pragma unroll
+        for (int s = 0; s < NPASS; ++s) {
+            int c = col0 + s * CPP;
+            *reinterpret_cast<d2 *>(lds + c * stride + 2 * rp) = v[s];
+        }
+    }
+};
+
+// Linear block id -> (mb, nb).  MODE 0: rectangular grid, 8-row groups,
+// column-major inside a group.  MODE 1: lower trapezoid (mb >= nb),
+// column-major.
+template <int MODE>
+__device__ __forceinline__ void decode_block(long long L, int Mb, int Nb, int &mb, int &nb) {
+    if (MODE == 0) {
+        const int GM = 8;
+        long long per_group = (long long)GM * Nb;
+        int g = (int)(L / per_group);
+        int first = g * GM;
+        int gsz = min(Mb - first, GM);
+        int w = (int)(L - (long long)g * per_group);
+        mb = first + w % gsz;
+        nb = w / gsz;
+    } else {
+        // column n holds Mb - n blocks; S(n) = n*Mb - n(n-1)/2 blocks precede it
+        double a = (double)Mb + 0.5;
+        double disc = a * a - 2.0 * (double)L;
+        int n = (int)floor(a - sqrt(fmax(disc, 0.0)));
+        n = max(0, min(n, Nb - 1));
+        auto S = [&](int c) { return (long long)c * Mb - ((long long)c * (c - 1)) / 2; };
+        while (n > 0 && S(n) > L) --n;
+        while (n + 1 < Nb && S(n + 1) <= L) ++n;
+        nb = n;
+        mb = n + (int)(L - S(n));
+    }
+}
+
+// Dispatch order -> logical order: XCD x (= blockIdx % 8) gets a contiguous
+// slice of logical ids.
+__device__ __forceinline__ long long xcd_remap(long long bid, long long G) {
+    const int NX = 8;
+    long long per = G / NX, rem = G % NX;
+    long long x = bid % NX, i = bid / NX;
+    return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+template <int BN, int MODE, int KIND>
+__global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
+    constexpr int WN = BN / 2;      // wave tile columns
+    constexpr int FM = 4;           // 64 / 16
+    constexpr int FN = WN / 16;
+    constexpr int SA = BM + LPAD;   // LDS row strides (doubles)
+    constexpr int SB = BN + LPAD;
+
+    __shared__ double As[2][BK * SA];
+    __shared__ double Bs[2][BK * SB];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    long long L = xcd_remap(blockIdx.x, a.nblk);
+    int mbi, nbi;
+    decode_block<MODE>(L, a.mb, a.nb, mbi, nbi);
+    const int m0 = mbi * BM, n0 = nbi * BN;
+    if (m0 >= a.M || n0 >= a.N) return;
+    if (a.lower && (a.C.r0 + m0 + BM - 1) < (a.C.c0 + n0)) return;
+
+    Stream<BM> sa;
+    Stream<BN> sb;
+    sa.init(a.A, m0, a.M, tid);
+    sb.init(a.B, n0, a.N, tid);
+
+    d4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    d2 ra[Stream<BM>::NPASS], rb[Stream<BN>::NPASS];
+    const int nk = (a.K + BK - 1) / BK;
+
+    if (nk > 0) {
+        sa.load(ra, 0, a.K);
+        sb.load(rb, 0, a.K);
+        sa.store_lds(As[0], SA, ra, tid);
+        sb.store_lds(Bs[0], SB, rb, tid);
+        sa.advance();
+        sb.advance();
+    }
+    __syncthreads();
+
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = (kt + 1) < nk;
+        if (more) {
+            sa.load(ra, (kt + 1) * BK, a.K);
+            sb.load(rb, (kt + 1) * BK, a.K);
+        }
+        const double *as = As[cur];
+        const double *bs = Bs[cur];
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+            double fa[FM], fb[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) fa[i] = as[kk * SA + wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) fb[j] = bs[kk * SB + wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            sa.store_lds(As[cur ^ 1], SA, ra, tid);
+            sb.store_lds(Bs[cur ^ 1], SB, rb, tid);
+            sa.advance();
+            sb.advance();
+        }
+        __syncthreads();
+    }
+
+    // Epilogue: lane holds rows 4*(lane/16)+r (r = 0..3), column lane%16 of
+    // each 16x16 accumulator.
+    const Opnd &Co = a.C;
+    const int cts = Co.ts > 0 ? Co.ts : 0x40000000;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int m = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int n = n0 + wn * WN + j * 16 + (lane & 15);
+            if (n >= a.N || m >= a.M) continue;
+            const int gr = Co.r0 + m, gc = Co.c0 + n;
+            long long off;
+            if (Co.ts > 0) {
+                int ti = gr / cts, tk = gc / cts;
+                off = tix(ti, tk, Co.ym, Co.tri) * Co.te + (gr - ti * cts) +
+                      (long long)(gc - tk * cts) * Co.ld;
+            } else {
+                off = gr + (long long)gc * Co.ld;
+            }
+            double *c = Co.p + off;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (m + r >= a.M) break;
+                if (a.lower && (gr + r) < gc) continue;
+                double v = a.alpha * acc[i][j][r];
+                if (a.beta != 0.0) v += a.beta * c[r];
+                c[r] = v;
+            }
+        }
+    }
+}
+
+template <int BN, int MODE, int KIND>
+hipError_t launch_one(const Gemm &g, hipStream_t s) {
+    KArgs a;
+    a.A = g.A;
+    a.B = g.B;
+    a.C = g.C;
+    a.M = g.M;
+    a.N = g.N;
+    a.K = g.K;
+    a.alpha = g.alpha;
+    a.beta = g.beta;
+    a.lower = g.lower;
+    a.mb = (g.M + BM - 1) / BM;
+    a.nb = (g.N + BN - 1) / BN;
+    if (MODE == 1) {
+        // lower trapezoid: column n has mb - n blocks
+        long long nbk = 0;
+        for (int n = 0; n < a.nb; ++n) nbk += (a.mb - n);
+        a.nblk = nbk;
+    } else {
+        a.nblk = (long long)a.mb * a.nb;
+    }
+    if (a.nblk <= 0) return hipSuccess;
+    if (a.nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_dgemm_nt<BN, MODE, KIND>), dim3((unsigned)a.nblk), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
+    if (g.M <= 0 || g.N <= 0) return hipSuccess;
+    // Operand constraints of the streaming loads (16-byte row-pair loads,
+    // 16-wide K steps that never straddle a tile).
+    const Opnd *ops[3] = {&g.A, &g.B, &g.C};
+    for (const Opnd *o : ops) {
+        if ((o->ld & 1) || (o->r0 & 1)) return hipErrorInvalidValue;
+        if (o->ts > 0 && ((o->ts % BK) != 0)) return hipErrorInvalidValue;
+        if (((uintptr_t)o->p) & 15) return hipErrorInvalidValue;
+    }
+    if ((g.A.ts > 0 && (g.A.c0 % BK)) || (g.B.ts > 0 && (g.B.c0 % BK))) return hipErrorInvalidValue;
+    switch (kind) {
+    case GEMM_UPDATE:
+        // the triangular block enumeration needs the diagonal on block
+        // boundaries; otherwise fall back to the rectangular grid + skipping
+        if (g.lower && g.C.r0 == g.C.c0 && g.M >= g.N) return launch_one<128, 1, 0>(g, s);
+        return launch_one<128, 0, 0>(g, s);
+    case GEMM_NARROW:
+        return launch_one<64, 0, 1>(g, s);
+    case GEMM_APPLY:
+        return launch_one<64, 0, 2>(g, s);
+    case GEMM_GENERAL:
+    default:
+        return launch_one<128, 0, 3>(g, s);
+    }
+}
+
+}  // namespace ooc

@@ -1,0 +1,98 @@
+// Internal interfaces of liboocdag: operand descriptors for the packed tile
+// storage and the kernel launchers used by the scheduler.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace ooc {
+
+// One matrix operand.  Element (r, c) of the operand lives at
+//   p + T(ti, tk) * te + (gr - ti*ts) + (gc - tk*ts) * ld
+// with gr = r0 + r, gc = c0 + c, ti = gr/ts, tk = gc/ts and
+//   T(ti, tk) = ti + tk*ym - tri*tk*(tk+1)/2.
+// tri = 1 is the packed lower-trapezoid panel of the reference's device
+// buffer Y (oocquark.cpp:256: tile (i,j) at (i + j*Ym - j(j+1)/2)*NB^2);
+// tri = 0 is a rectangular grid of tile columns, each ym tiles tall (the X
+// staging buffer, oocquark.cpp:304).  ts = 0 means plain column-major with
+// leading dimension ld.
+struct Opnd {
+    double *p;
+    long long ld;
+    long long te;
+    int ts;
+    int ym;
+    int tri;
+    int r0, c0;
+};
+
+inline Opnd plain(double *p, long long ld, int r0 = 0, int c0 = 0) {
+    Opnd o;
+    o.p = p; o.ld = ld; o.te = 0; o.ts = 0; o.ym = 0; o.tri = 0; o.r0 = r0; o.c0 = c0;
+    return o;
+}
+
+inline Opnd panel(double *p, int nb, int ym, int tri, int r0 = 0, int c0 = 0) {
+    Opnd o;
+    o.p = p; o.ld = nb; o.te = (long long)nb * nb; o.ts = nb; o.ym = ym; o.tri = tri;
+    o.r0 = r0; o.c0 = c0;
+    return o;
+}
+
+inline long long tile_index(int ti, int tk, int ym, int tri) {
+    return ti + (long long)tk * ym - (tri ? ((long long)tk * (tk + 1)) / 2 : 0);
+}
+
+// C(M x N) = beta*C + alpha * A(M x K) * B(N x K)^T.
+// lower != 0: only elements with (C.r0 + m) >= (C.c0 + n) are written, and
+// blocks strictly above that diagonal are not computed.
+struct Gemm {
+    Opnd A, B, C;
+    int M, N, K;
+    double alpha, beta;
+    int lower;
+};
+
+// Kernel families (distinct kernel symbols, so profiles separate them).
+enum GemmKind {
+    GEMM_UPDATE = 0,   // 128x128 tiles: trailing / left-looking panel updates
+    GEMM_NARROW = 1,   // 128x64 tiles: in-column blocked factorization updates
+    GEMM_APPLY = 2,    // 128x64 tiles: B <- B * Dinv^T (triangular solve apply)
+    GEMM_GENERAL = 3   // 128x128 tiles, rectangular: C-ABI dgemm entry point
+};
+
+hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s);
+
+// Largest block edge of the GEMM kernels: operands must provide this many
+// addressable rows past any block start they cover.
+constexpr int GEMM_BM = 128;
+
+// Factor the 64x64 diagonal block whose (0,0) is operand element (0,0) of P
+// (lower triangle only; the strict upper triangle is neither read nor
+// written) and write its inverse, zero-filled above the diagonal, to
+// dinv (64 x 64, ld 64).  A non-positive pivot at block column k stores
+// min(*info, gcol0 + k + 1) into *info.  info == nullptr: the block already
+// holds a factor; only its inverse is written.
+hipError_t launch_potrf64(const Opnd &P, int gcol0, double *dinv, int *info, hipStream_t s);
+
+// Blocked factorization of one tile column of a packed panel:
+// rows [row0, row0 + mrows) x columns [col0, col0 + ncols) of P (global
+// panel coordinates, row0 == col0 is the diagonal), ncols % 64 == 0.
+// Writes L in place (lower triangle of the diagonal part).
+hipError_t factor_column(const Opnd &P, int row0, int mrows, int ncols, int gcol_base,
+                         double *dinv_ws, int *info, hipStream_t s);
+
+// Make a staged tile whose valid part is vr x vc behave like an NB x NB
+// tile of the zero-padded problem: entries outside the valid part become 0,
+// and 1 on the diagonal when the tile is a diagonal tile (so the padded
+// matrix stays SPD and its factor's valid part is unchanged).
+hipError_t launch_pad_tile(double *tile, int nb, int vr, int vc, int diag, hipStream_t s);
+
+// PDMATGEN('S'/'H' or 'F', DIAG, ISEED) values written straight into the
+// packed lower tiles (tri = 1, ym = bb) of tile columns [jt0, jt1), padded
+// as launch_pad_tile does beyond N.
+hipError_t launch_gen_tiles(double *Y, int N, int nb, int bb, int jt0, int jt1, char aform,
+                            char diag, int iseed, hipStream_t s);
+
+}  // namespace ooc
