@@ -132,6 +132,19 @@ const char *oocdag_last_error(void);
 int oocdag_last_stats(double *seconds, double *h2d_bytes, double *d2h_bytes,
                       double *h2d_seconds, double *d2h_seconds);
 
+/* Instrumentation of the dominant kernel, the panel-update GEMM (replaces
+ * the per-task timestamps of oocquark.cpp:2-8 and the CUDA_PROFILE switch of
+ * run_gpu.sh:27): when enabled, every update launch is bracketed by HIP
+ * events on the stream it runs on.  profile_read returns the launches, the
+ * summed kernel seconds and the algorithmic flops (2*K per useful lower
+ * element) since enable.  Process-wide, not thread-safe. */
+int oocdag_profile_enable(int on);
+int oocdag_profile_read(long long *launches, double *seconds, double *flops);
+
+/* Progress lines on stderr every this many factored tile columns (0 = off),
+ * the successor of the reference's per-panel printf (oocdag.cpp:67). */
+int oocdag_set_progress(int every_columns);
+
 #ifdef __cplusplus
 }
 #endif

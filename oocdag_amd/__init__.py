@@ -56,6 +56,9 @@ SIGNATURES = {
     "oocdag_device_count": (C.c_int, []),
     "oocdag_last_error": (C.c_char_p, []),
     "oocdag_last_stats": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
+    "oocdag_profile_enable": (C.c_int, [C.c_int]),
+    "oocdag_profile_read": (C.c_int, [C.POINTER(C.c_longlong), _dp, _dp]),
+    "oocdag_set_progress": (C.c_int, [C.c_int]),
 }
 
 
@@ -141,6 +144,20 @@ def last_stats() -> dict:
     lib().oocdag_last_stats(*[C.byref(x) for x in v])
     keys = ["seconds", "h2d_bytes", "d2h_bytes", "h2d_seconds", "d2h_seconds"]
     return {k: x.value for k, x in zip(keys, v)}
+
+
+def profile_enable(on: bool = True) -> None:
+    lib().oocdag_profile_enable(1 if on else 0)
+
+
+def profile_read() -> dict:
+    n, t, f = C.c_longlong(0), C.c_double(0), C.c_double(0)
+    lib().oocdag_profile_read(C.byref(n), C.byref(t), C.byref(f))
+    return {"launches": n.value, "seconds": t.value, "flops": f.value}
+
+
+def set_progress(every_columns: int) -> None:
+    lib().oocdag_set_progress(int(every_columns))
 
 
 # ---------------------------------------------------------------------------

@@ -34,6 +34,56 @@ namespace {
 thread_local std::string g_err;
 thread_local double g_stat[5];
 
+// Optional instrumentation of the dominant kernel (the 128x128 panel-update
+// GEMM): an event pair around every launch on its own stream, summed after
+// the call.  Process-wide; meant for single-threaded benchmarking.
+struct UpdateProfile {
+    bool on = false;
+    std::vector<hipEvent_t> ev;   // start, end pairs
+    std::vector<double> flops;    // algorithmic flops of each launch
+    long long launches = 0;
+    double seconds = 0, total_flops = 0;
+} g_prof;
+
+int g_progress_every = 0;  // print a line every this many factored tile columns
+
+// Useful flops of a lower-trapezoid update: M x N output (diagonal of the
+// region on its top-left corner), lower part only, K-deep.
+double trap_update_flops(long long M, long long N, long long K) {
+    return 2.0 * (double)K * ((double)M * N - 0.5 * (double)N * (N - 1));
+}
+
+hipError_t launch_update(const Gemm &g, hipStream_t s) {
+    if (!g_prof.on) return launch_gemm(g, GEMM_UPDATE, s);
+    hipEvent_t a, b;
+    hipError_t e;
+    if ((e = hipEventCreate(&a)) != hipSuccess) return e;
+    if ((e = hipEventCreate(&b)) != hipSuccess) return e;
+    if ((e = hipEventRecord(a, s)) != hipSuccess) return e;
+    if ((e = launch_gemm(g, GEMM_UPDATE, s)) != hipSuccess) return e;
+    if ((e = hipEventRecord(b, s)) != hipSuccess) return e;
+    g_prof.ev.push_back(a);
+    g_prof.ev.push_back(b);
+    g_prof.flops.push_back(trap_update_flops(g.M, g.N, g.K));
+    return hipSuccess;
+}
+
+// Folds finished event pairs into the totals (call after a synchronize).
+void profile_collect() {
+    for (size_t i = 0; i + 1 < g_prof.ev.size(); i += 2) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, g_prof.ev[i], g_prof.ev[i + 1]) == hipSuccess) {
+            g_prof.seconds += ms * 1e-3;
+            g_prof.total_flops += g_prof.flops[i / 2];
+            g_prof.launches += 1;
+        }
+        (void)hipEventDestroy(g_prof.ev[i]);
+        (void)hipEventDestroy(g_prof.ev[i + 1]);
+    }
+    g_prof.ev.clear();
+    g_prof.flops.clear();
+}
+
 void set_err(const char *fmt, ...) {
     char buf[512];
     va_list ap;
@@ -190,10 +240,28 @@ template <class F>
 int factor_panel(double *dY, int NB, int Ym, int Yn, int gcol0, double *dinv, int *dinfo,
                  hipStream_t s, F on_column) {
     Opnd P = panel(dY, NB, Ym, 1);
+    // progress reporting keeps the host at most g_progress_every columns
+    // ahead of the device and prints when a block of columns is finished
+    std::vector<hipEvent_t> marks;
+    const double tstart = now();
     for (int j = 0; j < Yn; ++j) {
         HIPTRY(factor_column(P, j * NB, (Ym - j) * NB, NB, gcol0 + j * NB, dinv, dinfo, s));
         int rc = on_column(j);
         if (rc) return rc;
+        if (g_progress_every > 0 && (j + 1) % g_progress_every == 0) {
+            hipEvent_t e;
+            HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            HIPTRY(hipEventRecord(e, s));
+            marks.push_back(e);
+            if (marks.size() >= 2) {
+                HIPTRY(hipEventSynchronize(marks.front()));
+                (void)hipEventDestroy(marks.front());
+                marks.erase(marks.begin());
+                fprintf(stderr, "[oocdag] %d/%d tile columns factored, %.1f s\n",
+                        j + 1 - g_progress_every, Yn, now() - tstart);
+                fflush(stderr);
+            }
+        }
         if (j + 1 < Yn) {
             // right-looking update of the rest of the panel from column j
             // (ooc_syrk(&Y(jb+1,jb), &Y(jb+1,jb+1), ...), oocquark.cpp:414-418)
@@ -207,9 +275,10 @@ int factor_panel(double *dY, int NB, int Ym, int Yn, int gcol0, double *dinv, in
             g.alpha = -1.0;
             g.beta = 1.0;
             g.lower = 1;
-            HIPTRY(launch_gemm(g, GEMM_UPDATE, s));
+            HIPTRY(launch_update(g, s));
         }
     }
+    for (hipEvent_t e : marks) (void)hipEventDestroy(e);
     return OOCDAG_OK;
 }
 
@@ -347,7 +416,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             g.alpha = -1.0;
             g.beta = 1.0;
             g.lower = 1;
-            HIPTRY(launch_gemm(g, GEMM_UPDATE, st.comp));
+            HIPTRY(launch_update(g, st.comp));
             HIPTRY(hipEventRecord(ev_xfree[b], st.comp));
         }
 
@@ -393,6 +462,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     HIPTRY(hipStreamSynchronize(st.h2d));
     rc = read_info(sc.dinfo, st.comp, N, info);
     *secs = now() - t0;
+    profile_collect();
     g_stat[0] = *secs;
     g_stat[1] = h2d_bytes;
     g_stat[2] = d2h_bytes;
@@ -561,6 +631,7 @@ int oocdag_potrf_tiles(double *dY, int N, int NB, void *stream, int *info) {
     if (rc) return rc;
     rc = read_info(sc.dinfo, st.comp, N, info);
     g_stat[0] = now() - t0;
+    profile_collect();
     g_stat[1] = g_stat[2] = g_stat[3] = g_stat[4] = 0;
     return rc;
 }
@@ -670,7 +741,9 @@ int oocdag_pdmatgen(char aform, char diag, int M, int N, int MB, int NB, double 
                            (size_t)M * sizeof(double), (size_t)M * sizeof(double), g.ncols,
                            hipMemcpyDeviceToHost));
     }
-    if (dg == 'D' && MB == NB) {  // LIN/pdmatgen.f:516-543
+    // LIN/pdmatgen.f:516-543; AFORM 'F' overwrites the diagonal afterwards
+    // (:546-626), so DIAG has no effect on it
+    if (dg == 'D' && MB == NB && af != 'F') {
         double maxmn = (double)std::max(M, N);
         for (int i = 0; i < std::min(M, N); ++i) {
             double *a = A + i + (size_t)i * LDA;
@@ -827,6 +900,28 @@ int oocdag_memcpy_d2h(void *dst, const void *src, size_t bytes) {
 
 int oocdag_memset_d(void *dst, int value, size_t bytes) {
     HIPTRY(hipMemset(dst, value, bytes));
+    return OOCDAG_OK;
+}
+
+int oocdag_profile_enable(int on) {
+    profile_collect();
+    g_prof.on = on != 0;
+    g_prof.launches = 0;
+    g_prof.seconds = 0;
+    g_prof.total_flops = 0;
+    return OOCDAG_OK;
+}
+
+int oocdag_profile_read(long long *launches, double *seconds, double *flops) {
+    profile_collect();
+    if (launches) *launches = g_prof.launches;
+    if (seconds) *seconds = g_prof.seconds;
+    if (flops) *flops = g_prof.total_flops;
+    return OOCDAG_OK;
+}
+
+int oocdag_set_progress(int every_columns) {
+    g_progress_every = every_columns > 0 ? every_columns : 0;
     return OOCDAG_OK;
 }
 

@@ -242,18 +242,22 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
         __syncthreads();
     }
 
-    // Epilogue: lane holds rows 4*(lane/16)+r (r = 0..3), column lane%16 of
-    // each 16x16 accumulator.
+    // Epilogue.  v_mfma_f64_16x16x4_f64 leaves element r (0..3) of a lane's
+    // accumulator at row (lane/16) + 4*r, column lane%16 of its 16x16 block
+    // (checked on gfx950 against a CPU GEMM).
     const Opnd &Co = a.C;
     const int cts = Co.ts > 0 ? Co.ts : 0x40000000;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-        const int m = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+        const int mb16 = m0 + wm * 64 + i * 16;          // first row of the 16x16 block
+        const int m = mb16 + (lane >> 4);
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int n = n0 + wn * WN + j * 16 + (lane & 15);
             if (n >= a.N || m >= a.M) continue;
-            const int gr = Co.r0 + m, gc = Co.c0 + n;
+            // the 16 rows of the block lie in one tile (row offsets are
+            // multiples of 16 and tiles are multiples of 16 tall)
+            const int gr = Co.r0 + mb16, gc = Co.c0 + n;
             long long off;
             if (Co.ts > 0) {
                 int ti = gr / cts, tk = gc / cts;
@@ -262,14 +266,15 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
             } else {
                 off = gr + (long long)gc * Co.ld;
             }
-            double *c = Co.p + off;
+            double *c = Co.p + off + (lane >> 4);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                if (m + r >= a.M) break;
-                if (a.lower && (gr + r) < gc) continue;
+                const int row = m + 4 * r;
+                if (row >= a.M) break;
+                if (a.lower && (Co.r0 + row) < gc) continue;
                 double v = a.alpha * acc[i][j][r];
-                if (a.beta != 0.0) v += a.beta * c[r];
-                c[r] = v;
+                if (a.beta != 0.0) v += a.beta * c[4 * r];
+                c[4 * r] = v;
             }
         }
     }
@@ -316,6 +321,7 @@ hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
         if (((uintptr_t)o->p) & 15) return hipErrorInvalidValue;
     }
     if ((g.A.ts > 0 && (g.A.c0 % BK)) || (g.B.ts > 0 && (g.B.c0 % BK))) return hipErrorInvalidValue;
+    if (g.C.ts > 0 && (g.C.r0 % 16)) return hipErrorInvalidValue;
     switch (kind) {
     case GEMM_UPDATE:
         // the triangular block enumeration needs the diagonal on block

@@ -21,7 +21,7 @@ def rel(a, b):
 @pytest.mark.parametrize("n,nb,budget_tiles", [
     (4096, 256, 56),     # 28 MiB: the 4-panel OOC configuration
     (1000, 64, 96),      # ragged edge, many panels
-    (2048, 256, 10**6),  # whole lower triangle fits: one panel
+    (2048, 256, 60),     # whole lower triangle fits next to the X columns: one panel
     (1536, 128, 40),
 ])
 def test_cholesky_host_ooc_matches_oracle(gpu, ora, n, nb, budget_tiles):
