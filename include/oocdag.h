@@ -95,6 +95,32 @@ int oocdag_tiles_from_host(double *dY, const double *A, int N, int NB, int LDA);
 int oocdag_tiles_to_host(const double *dY, double *A, int N, int NB, int LDA);
 
 /* ---------------------------------------------------------------------------
+ * Solve with the factor: A X = B, A = L L^T.
+ * Replaces: dpotrs('L', N, nrhs, L, LDA, B, LDB, info) of the VERIFY block
+ *           oocdag.cpp:431-441 (host MKL in the reference)
+ * oocdag_potrs: host L (lower triangle, as left by oocdag_cholesky) and host
+ * B (N x nrhs, overwritten by X); the factor must fit in device memory.
+ * oocdag_potrs_tiles: device packed factor dY and device B (ldb >= N).
+ * ------------------------------------------------------------------------- */
+int oocdag_potrs(const double *L, int N, int LDL, double *B, int nrhs, int LDB);
+int oocdag_potrs_tiles(const double *dY, int N, int NB, double *dB, int nrhs, int ldb,
+                       void *stream);
+
+/* ---------------------------------------------------------------------------
+ * LIN scaled residual ||A - L L^T||_inf / (N * eps * ||A||_inf), eps = 2^-53.
+ * Replaces: PDPOTRRV('L') + PDLAFCHK(AFORM, DIAG, ..., IASEED, ANORM, FRESID)
+ *           LIN/pdpotrrv.f:240-314, LIN/pdlafchk.f:181-225
+ * A is regenerated on the device from (aform, diag, iseed) exactly as
+ * oocdag_pdmatgen produces it.  oocdag_llt_resid takes the host array left
+ * by oocdag_cholesky (L in the lower triangle, A's strict upper triangle
+ * kept; both count); oocdag_llt_resid_tiles the HBM-resident packed factor.
+ * ------------------------------------------------------------------------- */
+int oocdag_llt_resid(const double *A, int N, int LDA, char aform, char diag, int iseed,
+                     double *resid);
+int oocdag_llt_resid_tiles(const double *dY, int N, int NB, char aform, char diag, int iseed,
+                           double *resid);
+
+/* ---------------------------------------------------------------------------
  * Tile kernels (device pointers, column-major, explicit leading dimensions).
  * lda/ldb/ldc must be even and pointers 16-byte aligned.
  * Replaces: cublasDgemm_v2(N,T)   oocquark.cpp:222  (CORE_incore_dgemm)
@@ -119,6 +145,7 @@ void *oocdag_dev_alloc(size_t bytes);
 int oocdag_dev_free(void *p);
 int oocdag_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int oocdag_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int oocdag_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int oocdag_memset_d(void *dst, int value, size_t bytes);
 int oocdag_device_sync(void);
 /* Number of visible HIP devices (0 without a GPU); does not fail. */

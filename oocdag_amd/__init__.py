@@ -47,10 +47,17 @@ SIGNATURES = {
     "oocdag_dtrsm_rltn": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                     C.c_void_p]),
     "oocdag_dpotrf_l": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _ip, C.c_void_p]),
+    "oocdag_potrs": (C.c_int, [_dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int]),
+    "oocdag_potrs_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                     C.c_void_p]),
+    "oocdag_llt_resid": (C.c_int, [_dp, C.c_int, C.c_int, C.c_char, C.c_char, C.c_int, _dp]),
+    "oocdag_llt_resid_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char, C.c_char,
+                                         C.c_int, _dp]),
     "oocdag_dev_alloc": (C.c_void_p, [C.c_size_t]),
     "oocdag_dev_free": (C.c_int, [C.c_void_p]),
     "oocdag_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "oocdag_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "oocdag_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "oocdag_memset_d": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
     "oocdag_device_sync": (C.c_int, []),
     "oocdag_device_count": (C.c_int, []),
@@ -139,6 +146,21 @@ def pdmatgen(n: int, nb: int = 1, aform: str = "S", diag: str = "D", iseed: int 
     return A
 
 
+def potrs(L: np.ndarray, B: np.ndarray) -> None:
+    """dpotrs('L') with the factor left by :func:`cholesky` (oocdag.cpp:439);
+    B (N x nrhs, Fortran order) is overwritten with the solution."""
+    _check(lib().oocdag_potrs(_ptr(L), L.shape[1], L.shape[0], _ptr(B), B.shape[1], B.shape[0]),
+           "potrs")
+
+
+def llt_resid(A: np.ndarray, aform: str = "S", diag: str = "D", iseed: int = 100) -> float:
+    """PDPOTRRV + PDLAFCHK scaled residual of the factor left in A."""
+    r = C.c_double(0)
+    _check(lib().oocdag_llt_resid(_ptr(A), A.shape[1], A.shape[0], aform.encode(), diag.encode(),
+                                  iseed, C.byref(r)), "llt_resid")
+    return r.value
+
+
 def last_stats() -> dict:
     v = [C.c_double(0) for _ in range(5)]
     lib().oocdag_last_stats(*[C.byref(x) for x in v])
@@ -211,6 +233,17 @@ def potrf_tiles(buf: DeviceBuffer, n: int, nb: int) -> int:
     info = C.c_int(0)
     _check(lib().oocdag_potrf_tiles(buf.ptr, n, nb, None, C.byref(info)), "potrf_tiles")
     return info.value
+
+
+def potrs_tiles(buf: DeviceBuffer, n: int, nb: int, B: DeviceBuffer, nrhs: int, ldb: int):
+    _check(lib().oocdag_potrs_tiles(buf.ptr, n, nb, B.ptr, nrhs, ldb, None), "potrs_tiles")
+
+
+def llt_resid_tiles(buf: DeviceBuffer, n: int, nb: int, aform="S", diag="D", iseed=100) -> float:
+    r = C.c_double(0)
+    _check(lib().oocdag_llt_resid_tiles(buf.ptr, n, nb, aform.encode(), diag.encode(), iseed,
+                                        C.byref(r)), "llt_resid_tiles")
+    return r.value
 
 
 def tiles_from_host(buf: DeviceBuffer, A: np.ndarray, nb: int):

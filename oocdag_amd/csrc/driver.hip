@@ -290,6 +290,35 @@ int read_info(int *dinfo, hipStream_t s, int N, int *info) {
     return OOCDAG_OK;
 }
 
+// Busy time of a copy stream: an event pair around every copy, summed when
+// the call has finished.
+struct CopyTimer {
+    std::vector<hipEvent_t> ev;
+    hipError_t copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+        hipEvent_t a, b;
+        hipError_t e;
+        if ((e = hipEventCreate(&a)) != hipSuccess) return e;
+        if ((e = hipEventCreate(&b)) != hipSuccess) return e;
+        if ((e = hipEventRecord(a, s)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(dst, src, bytes, kind, s)) != hipSuccess) return e;
+        if ((e = hipEventRecord(b, s)) != hipSuccess) return e;
+        ev.push_back(a);
+        ev.push_back(b);
+        return hipSuccess;
+    }
+    double seconds() {
+        double t = 0;
+        for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) t += ms * 1e-3;
+        }
+        return t;
+    }
+    ~CopyTimer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 // Device scratch for the diagonal-block inverses and the info flag.
 struct Scratch {
     double *dinv = nullptr;
@@ -361,6 +390,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     HIPTRY(hipStreamSynchronize(st.comp));
 
     double h2d_bytes = 0, d2h_bytes = 0;
+    CopyTimer t_h2d, t_d2h;
     const double t0 = now();
     int hslot = 0;
     int nxb = pl.nx;
@@ -381,7 +411,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             pack_column(A, N, NB, LDA, tj, tj, slot[hslot]);
             size_t bytes = (size_t)(bb - tj) * nb2 * sizeof(double);
             double *dst = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
-            HIPTRY(hipMemcpyAsync(dst, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
+            HIPTRY(t_h2d.copy(dst, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
             HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
             h2d_bytes += bytes;
             hslot ^= 1;
@@ -399,7 +429,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             // A2X (oocquark.cpp:301-319): rows jb.. of tile column jjb
             pack_column(A, N, NB, LDA, jjb, (int)jb, slot[hslot]);
             size_t bytes = (size_t)Ym * nb2 * sizeof(double);
-            HIPTRY(hipMemcpyAsync(X, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
+            HIPTRY(t_h2d.copy(X, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
             HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
             HIPTRY(hipEventRecord(ev_xready[b], st.h2d));
             h2d_bytes += bytes;
@@ -444,8 +474,8 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
                               HIPTRY(hipStreamWaitEvent(st.d2h, ev_col, 0));
                               size_t bytes = (size_t)(Ym - jj) * nb2 * sizeof(double);
                               const double *src = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
-                              HIPTRY(hipMemcpyAsync(slot[s], src, bytes, hipMemcpyDeviceToHost,
-                                                    st.d2h));
+                              HIPTRY(t_d2h.copy(slot[s], src, bytes, hipMemcpyDeviceToHost,
+                                                st.d2h));
                               HIPTRY(hipEventRecord(ev_slot[s], st.d2h));
                               slot_used[s] = true;
                               pending.push_back(jj);
@@ -466,6 +496,8 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     g_stat[0] = *secs;
     g_stat[1] = h2d_bytes;
     g_stat[2] = d2h_bytes;
+    g_stat[3] = t_h2d.seconds();
+    g_stat[4] = t_d2h.seconds();
     for (auto &e : ev_slot) (void)hipEventDestroy(e);
     for (auto &e : ev_xready) (void)hipEventDestroy(e);
     for (auto &e : ev_xfree) (void)hipEventDestroy(e);
@@ -874,6 +906,155 @@ int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb
     return OOCDAG_OK;
 }
 
+int oocdag_potrs_tiles(const double *dY, int N, int NB, double *dB, int nrhs, int ldb,
+                       void *stream) {
+    g_err.clear();
+    if (!dY || !dB || N <= 0 || nrhs < 0 || ldb < N || check_nb(N, NB)) return OOCDAG_EARG;
+    if (nrhs == 0) return OOCDAG_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int bb = (N + NB - 1) / NB;
+    const long long Np = (long long)bb * NB;
+    const long long ldx = nrhs + (nrhs & 1);
+    double *dinv = nullptr, *Xt = nullptr;
+    struct F {
+        double **a, **b;
+        ~F() {
+            if (*a) (void)hipFree(*a);
+            if (*b) (void)hipFree(*b);
+        }
+    } f{&dinv, &Xt};
+    HIPTRY(hipMalloc(&dinv, (size_t)Np * 64 * sizeof(double)));
+    HIPTRY(hipMalloc(&Xt, (size_t)(ldx * Np) * sizeof(double)));
+    HIPTRY(hipMemsetAsync(Xt, 0, (size_t)(ldx * Np) * sizeof(double), s));
+    const Opnd Lp = panel((double *)dY, NB, bb, 1);
+    for (int k = 0; k < bb; ++k) {
+        Opnd Lk = Lp;
+        Lk.r0 = k * NB;
+        Lk.c0 = k * NB;
+        HIPTRY(diag_inverses(Lk, NB, dinv + (size_t)k * NB * 64, s));
+    }
+    HIPTRY(launch_transpose(dB, ldb, Xt, ldx, N, nrhs, s));
+    HIPTRY(potrs_tiles_t(dY, N, NB, dinv, Xt, nrhs, ldx, s));
+    HIPTRY(launch_transpose(Xt, ldx, dB, ldb, nrhs, N, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return OOCDAG_OK;
+}
+
+int oocdag_potrs(const double *L, int N, int LDL, double *B, int nrhs, int LDB) {
+    g_err.clear();
+    if (!L || !B || N <= 0 || nrhs < 0 || LDL < N || LDB < N) return OOCDAG_EARG;
+    if (!device_ok()) return OOCDAG_ENODEV;
+    const int NB = N >= 4096 ? 512 : 128;
+    double *dY = nullptr, *dB = nullptr;
+    struct F {
+        double **a, **b;
+        ~F() {
+            if (*a) (void)hipFree(*a);
+            if (*b) (void)hipFree(*b);
+        }
+    } f{&dY, &dB};
+    if (hipMalloc(&dY, oocdag_tiles_elems(N, NB) * sizeof(double)) != hipSuccess) {
+        set_err("factor does not fit in device memory");
+        return OOCDAG_ENOMEM;
+    }
+    int rc = oocdag_tiles_from_host(dY, L, N, NB, LDL);
+    if (rc) return rc;
+    const int ldb = N + (N & 1);
+    HIPTRY(hipMalloc(&dB, (size_t)ldb * nrhs * sizeof(double)));
+    HIPTRY(hipMemcpy2D(dB, (size_t)ldb * sizeof(double), B, (size_t)LDB * sizeof(double),
+                       (size_t)N * sizeof(double), nrhs, hipMemcpyHostToDevice));
+    rc = oocdag_potrs_tiles(dY, N, NB, dB, nrhs, ldb, nullptr);
+    if (rc) return rc;
+    HIPTRY(hipMemcpy2D(B, (size_t)LDB * sizeof(double), dB, (size_t)ldb * sizeof(double),
+                       (size_t)N * sizeof(double), nrhs, hipMemcpyDeviceToHost));
+    return OOCDAG_OK;
+}
+
+static int resid_finish(const double *rowsum, const double *anorm, int N, double *resid) {
+    std::vector<double> rs(N), an(N);
+    HIPTRY(hipMemcpy(rs.data(), rowsum, N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(an.data(), anorm, N * sizeof(double), hipMemcpyDeviceToHost));
+    double worst = 0, amax = 0;
+    for (int i = 0; i < N; ++i) {
+        worst = std::max(worst, rs[i]);
+        amax = std::max(amax, an[i]);
+    }
+    *resid = worst / ((double)N * std::ldexp(1.0, -53) * amax);
+    return OOCDAG_OK;
+}
+
+struct ResidScratch {
+    double *R = nullptr, *S = nullptr, *rowsum = nullptr, *anorm = nullptr;
+    ~ResidScratch() {
+        for (double *p : {R, S, rowsum, anorm})
+            if (p) (void)hipFree(p);
+    }
+    int init(int N, int NB) {
+        const int bb = (N + NB - 1) / NB;
+        const size_t col = (size_t)bb * NB * NB * sizeof(double);
+        if (hipMalloc(&R, col) != hipSuccess || hipMalloc(&S, col) != hipSuccess) {
+            set_err("residual scratch allocation failed");
+            return OOCDAG_ENOMEM;
+        }
+        HIPTRY(hipMalloc(&rowsum, (size_t)N * sizeof(double)));
+        HIPTRY(hipMalloc(&anorm, (size_t)N * sizeof(double)));
+        HIPTRY(hipMemset(rowsum, 0, (size_t)N * sizeof(double)));
+        HIPTRY(hipMemset(anorm, 0, (size_t)N * sizeof(double)));
+        return OOCDAG_OK;
+    }
+};
+
+int oocdag_llt_resid_tiles(const double *dY, int N, int NB, char aform, char diag, int iseed,
+                           double *resid) {
+    g_err.clear();
+    if (!dY || !resid || N <= 0 || check_nb(N, NB)) return OOCDAG_EARG;
+    if (!device_ok()) return OOCDAG_ENODEV;
+    ResidScratch sc;
+    int rc = sc.init(N, NB);
+    if (rc) return rc;
+    HIPTRY(llt_resid_tiles(dY, N, NB, aform, diag, iseed, sc.R, sc.S, sc.rowsum, sc.anorm, 0));
+    HIPTRY(hipDeviceSynchronize());
+    return resid_finish(sc.rowsum, sc.anorm, N, resid);
+}
+
+int oocdag_llt_resid(const double *A, int N, int LDA, char aform, char diag, int iseed,
+                     double *resid) {
+    g_err.clear();
+    if (!A || !resid || N <= 0 || LDA < N) return OOCDAG_EARG;
+    if (!device_ok()) return OOCDAG_ENODEV;
+    const int NB = N >= 4096 ? 512 : 128;
+    const int bb = (N + NB - 1) / NB;
+    double *dY = nullptr, *U = nullptr;
+    struct F {
+        double **a, **b;
+        ~F() {
+            if (*a) (void)hipFree(*a);
+            if (*b) (void)hipFree(*b);
+        }
+    } f{&dY, &U};
+    if (hipMalloc(&dY, oocdag_tiles_elems(N, NB) * sizeof(double)) != hipSuccess) {
+        set_err("factor does not fit in device memory");
+        return OOCDAG_ENOMEM;
+    }
+    int rc = oocdag_tiles_from_host(dY, A, N, NB, LDA);
+    if (rc) return rc;
+    ResidScratch sc;
+    if ((rc = sc.init(N, NB))) return rc;
+    HIPTRY(llt_resid_tiles(dY, N, NB, aform, diag, iseed, sc.R, sc.S, sc.rowsum, sc.anorm, 0));
+    // strict upper tiles: PDPOTRRV keeps A's upper triangle, so every change
+    // the factorization made there counts (LIN/pdlafchk.f:204,219)
+    HIPTRY(hipMalloc(&U, (size_t)N * NB * sizeof(double)));
+    for (int jt = 1; jt < bb; ++jt) {
+        int cols = std::min(NB, N - jt * NB);
+        HIPTRY(hipMemcpy2D(U, (size_t)N * sizeof(double), A + (size_t)jt * NB * LDA,
+                           (size_t)LDA * sizeof(double), (size_t)jt * NB * sizeof(double), cols,
+                           hipMemcpyHostToDevice));
+        HIPTRY(resid_upper_block(U, N, N, NB, jt, aform, diag, iseed, sc.rowsum, 0));
+    }
+    HIPTRY(hipDeviceSynchronize());
+    return resid_finish(sc.rowsum, sc.anorm, N, resid);
+}
+
 void *oocdag_dev_alloc(size_t bytes) {
     void *p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) {
@@ -895,6 +1076,11 @@ int oocdag_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 
 int oocdag_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     HIPTRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return OOCDAG_OK;
+}
+
+int oocdag_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    HIPTRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return OOCDAG_OK;
 }
 

@@ -126,6 +126,84 @@ struct Stream {
     }
 };
 
+// Streams a transposed operand: operand element (r, k) is element
+// (o.r0 + k, o.c0 + r) of the stored matrix o, so K runs down stored
+// columns.  Each thread loads a pair of consecutive k (16 bytes) for
+// BX/32 operand rows per step and scatters them into the k-major LDS tile.
+template <int BX>
+struct StreamT {
+    static constexpr int KP = BK / 2;       // k pairs per step
+    static constexpr int RPP = NT / KP;     // operand rows per pass
+    static constexpr int NPASS = BX / RPP;  // passes per K step
+
+    const double *p;
+    long long ld, te;
+    long long coloff[NPASS];
+    int tkc[NPASS];
+    bool ok[NPASS];
+    int ts, ym, tri, kp, rr, gk;
+
+    __device__ __forceinline__ void init(const Opnd &o, int rb, int rows, int tid) {
+        kp = tid % KP;
+        rr = tid / KP;
+        p = o.p;
+        ld = o.ld;
+        te = o.te;
+        ym = o.ym;
+        tri = o.tri;
+        ts = o.ts > 0 ? o.ts : 0x40000000;
+        gk = o.r0;
+#pragma unroll
+        for (int s = 0; s < NPASS; ++s) {
+            int r = rb + rr + s * RPP;
+            ok[s] = r < rows;
+            int gc = o.c0 + r;
+            tkc[s] = gc / ts;
+            coloff[s] = (long long)(gc - tkc[s] * ts) * ld;
+        }
+    }
+
+    __device__ __forceinline__ void load(d2 (&v)[NPASS], int k0, int K) const {
+        const int k = k0 + 2 * kp;
+        const int g = gk + 2 * kp;
+        const int ti = g / ts;
+        const long long roff = g - (long long)ti * ts;
+#pragma unroll
+        for (int s = 0; s < NPASS; ++s) {
+            d2 x = {0.0, 0.0};
+            if (ok[s] && k < K) {
+                const double *q = p + tix(ti, tkc[s], ym, tri) * te + roff + coloff[s];
+                if (k + 1 < K)
+                    x = *reinterpret_cast<const d2 *>(q);
+                else
+                    x.x = q[0];
+            }
+            v[s] = x;
+        }
+    }
+
+    __device__ __forceinline__ void advance() { gk += BK; }
+
+    __device__ __forceinline__ void store_lds(double *lds, int stride, const d2 (&v)[NPASS],
+                                              int) const {
+#pragma unroll
+        for (int s = 0; s < NPASS; ++s) {
+            int r = rr + s * RPP;
+            lds[(2 * kp) * stride + r] = v[s].x;
+            lds[(2 * kp + 1) * stride + r] = v[s].y;
+        }
+    }
+};
+
+template <int BX, int TR>
+struct StreamSel {
+    typedef Stream<BX> type;
+};
+template <int BX>
+struct StreamSel<BX, 1> {
+    typedef StreamT<BX> type;
+};
+
 // Linear block id -> (mb, nb).  MODE 0: rectangular grid, 8-row groups,
 // column-major inside a group.  MODE 1: lower trapezoid (mb >= nb),
 // column-major.
@@ -163,7 +241,7 @@ __device__ __forceinline__ long long xcd_remap(long long bid, long long G) {
     return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
-template <int BN, int MODE, int KIND>
+template <int BN, int MODE, int KIND, int TB>
 __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     constexpr int WN = BN / 2;      // wave tile columns
     constexpr int FM = 4;           // 64 / 16
@@ -186,8 +264,9 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     if (m0 >= a.M || n0 >= a.N) return;
     if (a.lower && (a.C.r0 + m0 + BM - 1) < (a.C.c0 + n0)) return;
 
+    typedef typename StreamSel<BN, TB>::type SB_t;
     Stream<BM> sa;
-    Stream<BN> sb;
+    SB_t sb;
     sa.init(a.A, m0, a.M, tid);
     sb.init(a.B, n0, a.N, tid);
 
@@ -197,7 +276,7 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    d2 ra[Stream<BM>::NPASS], rb[Stream<BN>::NPASS];
+    d2 ra[Stream<BM>::NPASS], rb[SB_t::NPASS];
     const int nk = (a.K + BK - 1) / BK;
 
     if (nk > 0) {
@@ -280,7 +359,7 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     }
 }
 
-template <int BN, int MODE, int KIND>
+template <int BN, int MODE, int KIND, int TB = 0>
 hipError_t launch_one(const Gemm &g, hipStream_t s) {
     KArgs a;
     a.A = g.A;
@@ -304,7 +383,7 @@ hipError_t launch_one(const Gemm &g, hipStream_t s) {
     }
     if (a.nblk <= 0) return hipSuccess;
     if (a.nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_dgemm_nt<BN, MODE, KIND>), dim3((unsigned)a.nblk), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((k_dgemm_nt<BN, MODE, KIND, TB>), dim3((unsigned)a.nblk), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
@@ -320,7 +399,10 @@ hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
         if (o->ts > 0 && ((o->ts % BK) != 0)) return hipErrorInvalidValue;
         if (((uintptr_t)o->p) & 15) return hipErrorInvalidValue;
     }
-    if ((g.A.ts > 0 && (g.A.c0 % BK)) || (g.B.ts > 0 && (g.B.c0 % BK))) return hipErrorInvalidValue;
+    const bool tb = kind == GEMM_GENERAL_TB || kind == GEMM_NARROW_TB || kind == GEMM_APPLY_TB;
+    if ((g.A.ts > 0 && (g.A.c0 % BK))) return hipErrorInvalidValue;
+    if (!tb && g.B.ts > 0 && (g.B.c0 % BK)) return hipErrorInvalidValue;
+    if (tb && g.B.ts > 0 && (g.B.r0 % BK)) return hipErrorInvalidValue;
     if (g.C.ts > 0 && (g.C.r0 % 16)) return hipErrorInvalidValue;
     switch (kind) {
     case GEMM_UPDATE:
@@ -332,6 +414,12 @@ hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
         return launch_one<64, 0, 1>(g, s);
     case GEMM_APPLY:
         return launch_one<64, 0, 2>(g, s);
+    case GEMM_GENERAL_TB:
+        return launch_one<128, 0, 4, 1>(g, s);
+    case GEMM_NARROW_TB:
+        return launch_one<64, 0, 5, 1>(g, s);
+    case GEMM_APPLY_TB:
+        return launch_one<64, 0, 6, 1>(g, s);
     case GEMM_GENERAL:
     default:
         return launch_one<128, 0, 3>(g, s);

@@ -59,7 +59,13 @@ enum GemmKind {
     GEMM_UPDATE = 0,   // 128x128 tiles: trailing / left-looking panel updates
     GEMM_NARROW = 1,   // 128x64 tiles: in-column blocked factorization updates
     GEMM_APPLY = 2,    // 128x64 tiles: B <- B * Dinv^T (triangular solve apply)
-    GEMM_GENERAL = 3   // 128x128 tiles, rectangular: C-ABI dgemm entry point
+    GEMM_GENERAL = 3,  // 128x128 tiles, rectangular: C-ABI dgemm entry point
+    // B given transposed: operand element (n, k) is element (B.r0 + k,
+    // B.c0 + n) of the stored matrix B (C = A * B_stored, "NN").  Used by
+    // the triangular solves with L (not L^T) on the right.
+    GEMM_GENERAL_TB = 4,
+    GEMM_NARROW_TB = 5,
+    GEMM_APPLY_TB = 6
 };
 
 hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s);
@@ -94,5 +100,30 @@ hipError_t launch_pad_tile(double *tile, int nb, int vr, int vc, int diag, hipSt
 // as launch_pad_tile does beyond N.
 hipError_t launch_gen_tiles(double *Y, int N, int nb, int bb, int jt0, int jt1, char aform,
                             char diag, int iseed, hipStream_t s);
+
+// ---- solves and residual (solve.hip) ----
+// 64x64 inverses of the diagonal blocks of the n x n lower triangle L
+// (n % 64 == 0) into dinv (n/64 blocks of 64*64, ld 64).
+hipError_t diag_inverses(const Opnd &L, int n, double *dinv, hipStream_t s);
+// B (m x n) <- B * L^-T and B <- B * L^-1 with L's block inverses in dinv.
+hipError_t trsm_right_lower_t(const Opnd &B, int m, int n, const Opnd &L, const double *dinv,
+                              hipStream_t s);
+hipError_t trsm_right_lower_n(const Opnd &B, int m, int n, const Opnd &L, const double *dinv,
+                              hipStream_t s);
+// X^T (nrhs x Npad, ld ldx) <- B^T A^-1 with A = L L^T in packed tiles dY;
+// dinv_all: the 64-block inverses of every diagonal tile, tile after tile.
+hipError_t potrs_tiles_t(const double *dY, int N, int NB, const double *dinv_all, double *Xt,
+                         int nrhs, long long ldx, hipStream_t s);
+hipError_t launch_transpose(const double *in, long long ld_in, double *out, long long ld_out,
+                            int rows, int cols, hipStream_t s);
+// Row sums of |A - L L^T| (rowsum) and of |A| (anorm) for the packed factor
+// dY of the PDMATGEN(aform, diag, iseed) matrix.  R and S: scratch of bb
+// tiles each.
+hipError_t llt_resid_tiles(const double *dY, int N, int NB, char aform, char diag, int iseed,
+                           double *R, double *S, double *rowsum, double *anorm, hipStream_t s);
+// Adds |A_host - A| row sums of the strict upper tiles above diagonal tile jt
+// (U: rows [0, jt*NB) of columns [jt*NB, jt*NB+NB), ld ldu).
+hipError_t resid_upper_block(const double *U, long long ldu, int N, int NB, int jt, char aform,
+                             char diag, int iseed, double *rowsum, hipStream_t s);
 
 }  // namespace ooc

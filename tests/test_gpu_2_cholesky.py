@@ -113,3 +113,95 @@ def test_potrf_tiles_repeatable(gpu):
         assert ooc.potrf_tiles(buf, n, nb) == 0
         outs.append(buf.to_array((ooc.tiles_elems(n, nb),)))
     assert np.array_equal(outs[0], outs[1])
+
+
+# ---------------------------------------------------------------- solve path
+
+@pytest.mark.parametrize("n,nrhs", [(1000, 5), (4096, 16), (300, 1)])
+def test_potrs_host_matches_numpy(gpu, ora, n, nrhs):
+    A = gpu.pdmatgen(n, 128)
+    A0 = A.copy(order="F")
+    _, info = gpu.cholesky(A, 128, 1 << 31)
+    assert info == 0
+    rng = np.random.default_rng(n)
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    X = B.copy(order="F")
+    gpu.potrs(A, X)
+    Xref = np.linalg.solve(A0, B)
+    assert np.max(np.abs(X - Xref)) / np.max(np.abs(Xref)) < 1e-12
+    assert ora.solve_resid(A0, X, B) <= 10.0
+
+
+def test_potrs_verify_identity(gpu):
+    """VERIFY block of oocdag.cpp:379-464: dpotrs(L, A) = I."""
+    n = 768
+    A = gpu.pdmatgen(n, 128)
+    A2 = A.copy(order="F")
+    gpu.cholesky(A, 128, 1 << 30)
+    gpu.potrs(A, A2)
+    assert np.max(np.abs(A2 - np.eye(n))) < 1e-12
+
+
+def test_potrs_tiles_device(gpu, ora):
+    ooc = gpu
+    n, nb, nrhs = 2000, 256, 7
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.pdmatgen_tiles(buf, n, nb)
+    assert ooc.potrf_tiles(buf, n, nb) == 0
+    rng = np.random.default_rng(11)
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    dB = ooc.DeviceBuffer.from_array(B)
+    ooc.potrs_tiles(buf, n, nb, dB, nrhs, n)
+    X = dB.to_array((n, nrhs))
+    A0 = ora.pdmatgen(n, nb)
+    assert np.max(np.abs(X - np.linalg.solve(A0, B))) < 1e-12
+
+
+# ---------------------------------------------------------------- LIN residual on the GPU
+
+def test_llt_resid_gpu_matches_oracle(gpu, ora):
+    n, nb = 1500, 128
+    A = gpu.pdmatgen(n, nb)
+    gpu.cholesky(A, nb, 1 << 30)
+    r_gpu, r_cpu = gpu.llt_resid(A), ora.llt_resid(A)
+    assert r_gpu <= 10 and r_cpu <= 10
+    assert r_gpu == pytest.approx(r_cpu, rel=0.5)  # both rounding-level
+    # a planted error dominates both and must agree closely
+    B = A.copy(order="F")
+    B[n - 1, 3] += 1e-9
+    assert gpu.llt_resid(B) == pytest.approx(ora.llt_resid(B), rel=1e-3)
+    # the strict upper triangle counts
+    C_ = A.copy(order="F")
+    C_[2, n - 5] += 1e-6
+    assert gpu.llt_resid(C_) == pytest.approx(ora.llt_resid(C_), rel=1e-3)
+
+
+def test_llt_resid_tiles_hbm(gpu, ora):
+    ooc = gpu
+    n, nb = 3000, 512
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.pdmatgen_tiles(buf, n, nb)
+    assert ooc.potrf_tiles(buf, n, nb) == 0
+    r = ooc.llt_resid_tiles(buf, n, nb)
+    A = ora.pdmatgen(n, nb)
+    ooc.tiles_to_host(buf, A, nb)
+    assert r <= 10
+    assert r == pytest.approx(ora.llt_resid(A), rel=0.5)
+
+
+def test_cli_reference_output(gpu):
+    """oocdag N NB memsize_MB prints the reference's summary line."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(gpu.LIB_PATH), "oocdag")
+    out = subprocess.run([exe, "2048", "256", "10", "--resid", "--verify"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    i = lines.index("N NB memsize(MB) quark_pthreads time Gflops")
+    f = lines[i + 1].split()
+    assert f[0] == "2048" and f[1] == "256" and float(f[2]) == 10.0
+    resid = [l for l in lines if l.startswith("scaled residual")][0]
+    assert float(resid.split()[-1]) <= 10
+    maxerr = [l for l in lines if l.startswith("max error")][0]
+    assert float(maxerr.split()[2]) < 1e-10
