@@ -85,6 +85,9 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=28672)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run device-to-device copies of BYTES (a plain copy kernel of "
+                         "known size that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -96,6 +99,17 @@ def main() -> int:
     if ooc.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration
+        src, dst = ooc.DeviceBuffer(nbytes), ooc.DeviceBuffer(nbytes)
+        ooc.lib().oocdag_memset_d(src.ptr, 1, nbytes)
+        for _ in range(3):
+            ooc.lib().oocdag_memcpy_d2d(dst.ptr, src.ptr, nbytes)
+        ooc.lib().oocdag_device_sync()
+        log(f"[bench] copy calibration: 3 device copies of {nbytes} bytes")
+        src.free()
+        dst.free()
 
     n, nb = args.n, args.nb
     flops = n ** 3 / 3.0
