@@ -121,6 +121,42 @@ int oocdag_llt_resid_tiles(const double *dY, int N, int NB, char aform, char dia
                            double *resid);
 
 /* ---------------------------------------------------------------------------
+ * Mixed precision + iterative refinement (no reference counterpart: the
+ * reference factors in FP64 only).  Factor a copy of the packed FP64 tiles
+ * dA with the trailing updates in FP64 (precision 0), FP32 (1) or BF16 with
+ * per-tile power-of-two scaling (2), FP32 accumulation; diagonal POTRF, TRSM,
+ * residual r = b - A x and the update x += d stay in FP64.  Refine until the
+ * normwise backward error max_k ||b_k - A x_k|| / (||A|| ||x_k|| + ||b_k||)
+ * (infinity norms) is <= tol; a low-precision factor that is not positive
+ * definite, or refinement that stalls (error not halving) or exceeds
+ * max_iter, falls back to the FP64 factorization.  dA must hold the full
+ * diagonal tiles (as oocdag_gram_tiles / oocdag_pdmatgen_tiles write them).
+ * NB % 128 == 0.  dB, dX: device, column-major N x nrhs.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int precision_used;    /* update precision of the accepted factor */
+    int fell_back;         /* 1: the low-precision attempt was abandoned */
+    int iterations;        /* refinement steps over all attempts */
+    int info;              /* > 0: A itself is not positive definite */
+    double backward_error; /* of the returned solution */
+    double factor_seconds, solve_seconds, total_seconds;
+} oocdag_mpir_stats;
+
+int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const double *dB, int nrhs, int ldb,
+                            double *dX, int ldx, int precision, double tol, int max_iter,
+                            oocdag_mpir_stats *stats);
+
+/* Gram test matrices of the mixed-precision workloads: A = (1/N) G G^T + mu I
+ * with G an N x N matrix of N(0,1) variates from a counter-based generator
+ * (splitmix64 + Box-Muller, keyed by seed), written as packed lower tiles
+ * with full diagonal tiles.  Needs a temporary N x N buffer on the device. */
+int oocdag_gram_tiles(double *dY, int N, int NB, double mu, unsigned long long seed, void *stream);
+/* N(0,1) variates (same generator, stream_id selects an independent stream)
+ * into a device block of rows x cols, leading dimension ld. */
+int oocdag_normal_fill(double *dB, int rows, int cols, int ld, unsigned long long seed,
+                       unsigned long long stream_id);
+
+/* ---------------------------------------------------------------------------
  * Tile kernels (device pointers, column-major, explicit leading dimensions).
  * lda/ldb/ldc must be even and pointers 16-byte aligned.
  * Replaces: cublasDgemm_v2(N,T)   oocquark.cpp:222  (CORE_incore_dgemm)

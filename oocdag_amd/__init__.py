@@ -69,6 +69,28 @@ SIGNATURES = {
 }
 
 
+class MpirStats(C.Structure):
+    _fields_ = [("precision_used", C.c_int), ("fell_back", C.c_int), ("iterations", C.c_int),
+                ("info", C.c_int), ("backward_error", C.c_double), ("factor_seconds", C.c_double),
+                ("solve_seconds", C.c_double), ("total_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SIGNATURES.update({
+    "oocdag_mpir_solve_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                          C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                          C.POINTER(MpirStats)]),
+    "oocdag_gram_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_ulonglong,
+                                    C.c_void_p]),
+    "oocdag_normal_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
+                                     C.c_ulonglong]),
+})
+
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
+
+
 class OocdagError(RuntimeError):
     pass
 
@@ -244,6 +266,26 @@ def llt_resid_tiles(buf: DeviceBuffer, n: int, nb: int, aform="S", diag="D", ise
     _check(lib().oocdag_llt_resid_tiles(buf.ptr, n, nb, aform.encode(), diag.encode(), iseed,
                                         C.byref(r)), "llt_resid_tiles")
     return r.value
+
+
+def gram_tiles(buf: DeviceBuffer, n: int, nb: int, mu: float, seed: int = 1):
+    _check(lib().oocdag_gram_tiles(buf.ptr, n, nb, mu, seed, None), "gram_tiles")
+
+
+def normal_fill(buf: DeviceBuffer, rows: int, cols: int, ld: int, seed: int = 1, stream_id: int = 2):
+    _check(lib().oocdag_normal_fill(buf.ptr, rows, cols, ld, seed, stream_id), "normal_fill")
+
+
+def mpir_solve_tiles(A: DeviceBuffer, n: int, nb: int, B: DeviceBuffer, nrhs: int,
+                     X: DeviceBuffer, precision="fp32", tol: float = None, max_iter: int = 30):
+    """Mixed-precision factor + FP64 iterative refinement; returns the stats dict."""
+    st = MpirStats()
+    if tol is None:
+        tol = 2.0 ** -53 * 8
+    _check(lib().oocdag_mpir_solve_tiles(A.ptr, n, nb, B.ptr, nrhs, n, X.ptr, n,
+                                         PRECISIONS.get(precision, precision), tol, max_iter,
+                                         C.byref(st)), "mpir_solve_tiles")
+    return st.as_dict()
 
 
 def tiles_from_host(buf: DeviceBuffer, A: np.ndarray, nb: int):

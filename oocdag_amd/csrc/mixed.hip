@@ -1,0 +1,399 @@
+// Mixed-precision Cholesky with FP64 iterative refinement, and the Gram-
+// matrix generator of the mixed-precision workloads.  No reference
+// counterpart (the reference factors in FP64 only); every result is checked
+// against the FP64 path.
+//
+// Factor: the working copy W of A (packed FP64 tiles) is factored tile
+// column by tile column; the diagonal POTRF and the TRSM of each column run
+// in FP64 (factor_column), the trailing update in FP64, FP32 or BF16 (per-
+// tile scaled) on MFMA with FP32 accumulation into the FP64 tiles.
+// Refinement, all in FP64 with right-hand sides kept transposed (nrhs x N):
+//   x = W^-1 b;  repeat  r = b - A x,  eta = max_k ||r_k|| / (||A|| ||x_k|| + ||b_k||),
+//                        stop if eta <= tol,  x += W^-1 r
+// A low-precision factor that is not positive definite, or refinement that
+// stalls (eta not halving) or runs out of iterations, falls back to the
+// plain FP64 factorization.
+#include "ooc_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+namespace ooc {
+
+int mfma_rowmap(int kind, hipStream_t s);
+hipError_t convert_column(const double *Y, int nb, int ym, int jt, int ntiles, int mode,
+                          float *scale, void *out, hipStream_t s);
+hipError_t launch_update_lp(int mode, double *C, int c_ym, int r0, const void *X,
+                            const float *scale, int nb, int M, int N, int rowmap,
+                            hipStream_t s);
+
+namespace {
+
+// counter-based normal variates: splitmix64 (Steele, Lea, Flood 2014) of
+// (seed, stream, index) -> two uniforms -> Box-Muller
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double normal_at(unsigned long long seed, unsigned long long stream,
+                                            unsigned long long idx) {
+    unsigned long long h = splitmix64(seed ^ splitmix64(stream * 0x632be59bd9b4e019ull + idx));
+    unsigned long long h2 = splitmix64(h);
+    double u1 = ((h >> 11) + 1.0) * (1.0 / 9007199254740993.0);  // (0, 1]
+    double u2 = (h2 >> 11) * (1.0 / 9007199254740992.0);         // [0, 1)
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__global__ void k_fill_normal(double *G, long long rows, long long cols, long long ld,
+                              unsigned long long seed, unsigned long long stream, long long nvalid_r,
+                              long long nvalid_c) {
+    const long long total = rows * cols;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        long long r = idx % rows, c = idx / rows;
+        G[r + c * ld] = (r < nvalid_r && c < nvalid_c)
+                            ? normal_at(seed, stream, (unsigned long long)(c * nvalid_r + r))
+                            : 0.0;
+    }
+}
+
+// Diagonal tiles: copy the lower triangle into the upper one and add mu on
+// the diagonal (1 on the padded diagonal past N).
+__global__ void k_finish_diag(double *Y, int N, int nb, int bb, double mu) {
+    const int t = blockIdx.y;
+    const long long base = ((long long)t * bb - ((long long)t * (t - 1)) / 2) * nb * nb;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nb * nb;
+         idx += gridDim.x * blockDim.x) {
+        int r = idx % nb, c = idx / nb;
+        int g = t * nb + r;
+        if (r == c)
+            Y[base + idx] = (g < N) ? Y[base + idx] + mu : 1.0;
+        else if (r < c)
+            Y[base + idx] = Y[base + c + (long long)r * nb];
+    }
+}
+
+// Row sums of |A| over the full symmetric matrix from its packed lower tiles
+// (diagonal tiles stored full).
+__global__ void k_tile_rowabs(const double *Y, int nb, int bb, double *rowsum) {
+    const int ti = blockIdx.y, tj = blockIdx.z;
+    if (tj > ti) return;
+    const long long base = (ti + (long long)tj * bb - ((long long)tj * (tj + 1)) / 2) * nb * nb;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nb) return;
+    double rs = 0.0, cs = 0.0;
+    for (int c = 0; c < nb; ++c) rs += fabs(Y[base + r + (long long)c * nb]);
+    atomicAdd(&rowsum[ti * nb + r], rs);
+    if (ti != tj) {
+        for (int rr = 0; rr < nb; ++rr) cs += fabs(Y[base + rr + (long long)r * nb]);
+        atomicAdd(&rowsum[tj * nb + r], cs);
+    }
+}
+
+// Per right-hand side (row k of the transposed block) max |.| over N columns.
+__global__ void k_rows_absmax(const double *Xt, long long ldx, int nrhs, int N, double *out) {
+    const int k = blockIdx.x;
+    double m = 0.0;
+    for (int c = threadIdx.x; c < N; c += blockDim.x) m = fmax(m, fabs(Xt[k + (long long)c * ldx]));
+    __shared__ double red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+}
+
+__global__ void k_axpy(double *y, const double *x, long long n, double a) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+hipError_t gram_tiles(double *dY, int N, int NB, double mu, unsigned long long seed,
+                      hipStream_t s) {
+    const int bb = (N + NB - 1) / NB;
+    const long long Np = (long long)bb * NB;
+    double *G = nullptr;
+    hipError_t e = hipMalloc(&G, (size_t)(Np * Np) * sizeof(double));
+    if (e != hipSuccess) return e;
+    int blocks = 65536;
+    hipLaunchKernelGGL(k_fill_normal, dim3(blocks), dim3(256), 0, s, G, Np, Np, Np, seed, 1ull,
+                       (long long)N, (long long)N);
+    if ((e = hipGetLastError()) == hipSuccess) {
+        Gemm g;
+        g.A = plain(G, Np);
+        g.B = g.A;
+        g.C = panel(dY, NB, bb, 1);
+        g.M = (int)Np;
+        g.N = (int)Np;
+        g.K = (int)Np;
+        g.alpha = 1.0 / (double)N;
+        g.beta = 0.0;
+        g.lower = 1;
+        e = launch_gemm(g, GEMM_UPDATE, s);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_finish_diag, dim3(64, bb), dim3(256), 0, s, dY, N, NB, bb, mu);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(G);
+    return e != hipSuccess ? e : e2;
+}
+
+hipError_t normal_block(double *dB, int rows, int cols, long long ld, unsigned long long seed,
+                        unsigned long long stream, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_normal, dim3(4096), dim3(256), 0, s, dB, (long long)rows,
+                       (long long)cols, ld, seed, stream, (long long)rows, (long long)cols);
+    return hipGetLastError();
+}
+
+// Y^T (nrhs x Np) = alpha * X^T A + beta * Y^T for symmetric A in packed lower
+// tiles with full diagonal tiles: per tile column j,
+//   Y^T[:, j]      += X^T[:, rows >= j] A[rows >= j, j]      (stored, read transposed)
+//   Y^T[:, rows>j] += X^T[:, j] A[rows > j, j]^T
+hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt, long long ld,
+                  int nrhs, double alpha, double beta, hipStream_t s) {
+    const int bb = (N + NB - 1) / NB;
+    const Opnd Ap = panel((double *)dA, NB, bb, 1);
+    hipError_t e;
+    if (beta != 1.0) {
+        // scale Y^T by beta via a zero-K GEMM
+        Gemm g;
+        g.A = plain((double *)Xt, ld);
+        g.B = Ap;
+        g.C = plain(Yt, ld);
+        g.M = nrhs; g.N = bb * NB; g.K = 0;
+        g.alpha = 0.0; g.beta = beta; g.lower = 0;
+        if ((e = launch_gemm(g, GEMM_GENERAL, s)) != hipSuccess) return e;
+    }
+    for (int j = 0; j < bb; ++j) {
+        Gemm g;
+        g.A = plain((double *)Xt, ld, 0, j * NB);
+        g.B = Ap; g.B.r0 = j * NB; g.B.c0 = j * NB;
+        g.C = plain(Yt, ld, 0, j * NB);
+        g.M = nrhs; g.N = NB; g.K = (bb - j) * NB;
+        g.alpha = alpha; g.beta = 1.0; g.lower = 0;
+        if ((e = launch_gemm(g, GEMM_GENERAL_TB, s)) != hipSuccess) return e;
+        if (j + 1 < bb) {
+            Gemm h;
+            h.A = plain((double *)Xt, ld, 0, j * NB);
+            h.B = Ap; h.B.r0 = (j + 1) * NB; h.B.c0 = j * NB;
+            h.C = plain(Yt, ld, 0, (j + 1) * NB);
+            h.M = nrhs; h.N = (bb - j - 1) * NB; h.K = NB;
+            h.alpha = alpha; h.beta = 1.0; h.lower = 0;
+            if ((e = launch_gemm(h, GEMM_GENERAL, s)) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// Factor W (packed tiles) with the given update precision; 64-block inverses
+// of every diagonal tile land in dinv_all for the solves.
+int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int *dinfo,
+                 void *lpbuf, float *scale, hipStream_t s) {
+    const int bb = (N + NB - 1) / NB;
+    const Opnd P = panel(W, NB, bb, 1);
+    const size_t per = (size_t)NB * NB;
+    const size_t inv_per_tile = (size_t)(NB / 64) * 64 * 64;
+    int rowmap = 0;
+    if (precision > 0) {
+        rowmap = mfma_rowmap(precision - 1, s);
+        if (rowmap < 0) return -1;
+    }
+    for (int j = 0; j < bb; ++j) {
+        if (factor_column(P, j * NB, (bb - j) * NB, NB, j * NB, dinv_all + j * inv_per_tile, dinfo,
+                          s) != hipSuccess)
+            return -2;
+        if (j + 1 == bb) break;
+        const int M = (bb - j - 1) * NB;
+        if (precision == 0) {
+            Gemm g;
+            g.A = P; g.A.r0 = (j + 1) * NB; g.A.c0 = j * NB;
+            g.B = g.A;
+            g.C = P; g.C.r0 = (j + 1) * NB; g.C.c0 = (j + 1) * NB;
+            g.M = M; g.N = M; g.K = NB;
+            g.alpha = -1.0; g.beta = 1.0; g.lower = 1;
+            if (launch_gemm(g, GEMM_UPDATE, s) != hipSuccess) return -2;
+        } else {
+            const int mode = precision - 1;
+            const size_t esz = mode == 0 ? 4 : 2;
+            if (convert_column(W, NB, bb, j, bb - j, mode, scale, lpbuf, s) != hipSuccess) return -2;
+            const void *X = (const char *)lpbuf + per * esz;  // skip the diagonal tile
+            if (launch_update_lp(mode, W, bb, (j + 1) * NB, X, scale + 1, NB, M, M, rowmap, s) !=
+                hipSuccess)
+                return -2;
+        }
+    }
+    return 0;
+}
+
+}  // namespace ooc
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+#include "../../include/oocdag.h"
+
+using namespace ooc;
+
+namespace {
+struct Dev {
+    void *p = nullptr;
+    ~Dev() {
+        if (p) (void)hipFree(p);
+    }
+    bool alloc(size_t n) { return hipMalloc(&p, n) == hipSuccess; }
+    double *d() { return (double *)p; }
+};
+}  // namespace
+
+extern "C" int oocdag_gram_tiles(double *dY, int N, int NB, double mu, unsigned long long seed,
+                                 void *stream) {
+    if (!dY || N <= 0 || NB < 64 || NB % 64) return OOCDAG_EARG;
+    return gram_tiles(dY, N, NB, mu, seed, (hipStream_t)stream) == hipSuccess ? OOCDAG_OK
+                                                                              : OOCDAG_EHIP;
+}
+
+extern "C" int oocdag_normal_fill(double *dB, int rows, int cols, int ld, unsigned long long seed,
+                                  unsigned long long stream_id) {
+    if (!dB || rows < 0 || cols < 0 || ld < rows) return OOCDAG_EARG;
+    if (normal_block(dB, rows, cols, ld, seed, stream_id, 0) != hipSuccess) return OOCDAG_EHIP;
+    return hipDeviceSynchronize() == hipSuccess ? OOCDAG_OK : OOCDAG_EHIP;
+}
+
+extern "C" int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const double *dB, int nrhs,
+                                       int ldb, double *dX, int ldx, int precision, double tol,
+                                       int max_iter, oocdag_mpir_stats *st) {
+    oocdag_mpir_stats dummy;
+    if (!st) st = &dummy;
+    *st = oocdag_mpir_stats{};
+    if (!dA || !dB || !dX || N <= 0 || nrhs <= 0 || ldb < N || ldx < N || NB < 128 || NB % 128 ||
+        precision < 0 || precision > 2)
+        return OOCDAG_EARG;
+    if (precision == 2 && NB % 64) return OOCDAG_EARG;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return OOCDAG_EHIP;
+    struct SD {
+        hipStream_t s;
+        ~SD() { (void)hipStreamDestroy(s); }
+    } sd{s};
+    const double t0 = now();
+    const int bb = (N + NB - 1) / NB;
+    const long long Np = (long long)bb * NB;
+    const long long ldt = nrhs + (nrhs & 1);
+    const size_t tiles = (size_t)bb * (bb + 1) / 2 * NB * NB;
+    Dev W, dinv, info, lp, scale, Bt, Xt, Rt, anorm, nrm;
+    if (!W.alloc(tiles * 8) || !dinv.alloc((size_t)Np * 64 * 8) || !info.alloc(sizeof(int)) ||
+        !lp.alloc((size_t)bb * NB * NB * 4) || !scale.alloc((size_t)bb * 4) ||
+        !Bt.alloc((size_t)(ldt * Np) * 8) || !Xt.alloc((size_t)(ldt * Np) * 8) ||
+        !Rt.alloc((size_t)(ldt * Np) * 8) || !anorm.alloc((size_t)Np * 8) ||
+        !nrm.alloc((size_t)2 * ldt * 8))
+        return OOCDAG_ENOMEM;
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    // ||A||_inf
+    if (!ok(hipMemsetAsync(anorm.p, 0, (size_t)Np * 8, s))) return OOCDAG_EHIP;
+    hipLaunchKernelGGL(k_tile_rowabs, dim3((NB + 255) / 256, bb, bb), dim3(256), 0, s, dA, NB, bb,
+                       anorm.d());
+    std::vector<double> hrow(Np);
+    if (!ok(hipMemcpyAsync(hrow.data(), anorm.p, Np * 8, hipMemcpyDeviceToHost, s)) ||
+        !ok(hipStreamSynchronize(s)))
+        return OOCDAG_EHIP;
+    const double Anorm = *std::max_element(hrow.begin(), hrow.end());
+    // B^T, padded with zeros past N
+    if (!ok(hipMemsetAsync(Bt.p, 0, (size_t)(ldt * Np) * 8, s)) ||
+        !ok(launch_transpose(dB, ldb, Bt.d(), ldt, N, nrhs, s)))
+        return OOCDAG_EHIP;
+    std::vector<double> bnorm(nrhs), hx(nrhs), hr(nrhs);
+    hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Bt.d(), ldt, nrhs, N, nrm.d());
+    if (!ok(hipMemcpyAsync(bnorm.data(), nrm.p, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+        !ok(hipStreamSynchronize(s)))
+        return OOCDAG_EHIP;
+
+    int prec = precision;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const double tf = now();
+        int big = INT_MAX;
+        if (!ok(hipMemcpyAsync(W.p, dA, tiles * 8, hipMemcpyDeviceToDevice, s)) ||
+            !ok(hipMemcpyAsync(info.p, &big, sizeof(int), hipMemcpyHostToDevice, s)))
+            return OOCDAG_EHIP;
+        int rc = factor_mixed(W.d(), N, NB, prec, dinv.d(), (int *)info.p, lp.p, (float *)scale.p, s);
+        if (rc == -1) return OOCDAG_EHIP;  // MFMA layout probe failed
+        if (rc) return OOCDAG_EHIP;
+        int hinfo = INT_MAX;
+        if (!ok(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s)) ||
+            !ok(hipStreamSynchronize(s)))
+            return OOCDAG_EHIP;
+        st->factor_seconds += now() - tf;
+        const double ts = now();
+        bool good = hinfo == INT_MAX;
+        if (!good && prec == 0) {
+            st->info = std::min(hinfo, N);
+            return OOCDAG_OK;
+        }
+        double eta = INFINITY, prev = INFINITY;
+        int it = 0;
+        if (good) {
+            // x = W^-1 b
+            if (!ok(hipMemcpyAsync(Xt.p, Bt.p, (size_t)(ldt * Np) * 8, hipMemcpyDeviceToDevice, s)) ||
+                !ok(potrs_tiles_t(W.d(), N, NB, dinv.d(), Xt.d(), nrhs, ldt, s)))
+                return OOCDAG_EHIP;
+            for (;;) {
+                // r = b - A x
+                if (!ok(hipMemcpyAsync(Rt.p, Bt.p, (size_t)(ldt * Np) * 8, hipMemcpyDeviceToDevice, s)) ||
+                    !ok(symm_t(dA, N, NB, Xt.d(), Rt.d(), ldt, nrhs, -1.0, 1.0, s)))
+                    return OOCDAG_EHIP;
+                hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Xt.d(), ldt, nrhs, N,
+                                   nrm.d());
+                hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Rt.d(), ldt, nrhs, N,
+                                   nrm.d() + ldt);
+                if (!ok(hipMemcpyAsync(hx.data(), nrm.p, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+                    !ok(hipMemcpyAsync(hr.data(), nrm.d() + ldt, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+                    !ok(hipStreamSynchronize(s)))
+                    return OOCDAG_EHIP;
+                prev = eta;
+                eta = 0;
+                for (int k = 0; k < nrhs; ++k)
+                    eta = std::max(eta, hr[k] / (Anorm * hx[k] + bnorm[k]));
+                if (!(eta == eta)) eta = INFINITY;
+                if (eta <= tol) break;
+                if (it >= max_iter || !(eta < 0.5 * prev)) {
+                    good = false;
+                    break;
+                }
+                // x += W^-1 r
+                if (!ok(potrs_tiles_t(W.d(), N, NB, dinv.d(), Rt.d(), nrhs, ldt, s))) return OOCDAG_EHIP;
+                hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(256), 0, s, Xt.d(), Rt.d(), ldt * Np, 1.0);
+                ++it;
+            }
+        }
+        st->solve_seconds += now() - ts;
+        st->iterations += it;
+        st->backward_error = eta;
+        if (good || prec == 0) {
+            st->precision_used = prec;
+            break;
+        }
+        st->fell_back = 1;
+        prec = 0;  // plain FP64 factorization
+    }
+    if (!ok(launch_transpose(Xt.d(), ldt, dX, ldx, nrhs, N, s)) || !ok(hipStreamSynchronize(s)))
+        return OOCDAG_EHIP;
+    st->total_seconds = now() - t0;
+    return OOCDAG_OK;
+}

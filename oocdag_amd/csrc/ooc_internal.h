@@ -126,4 +126,8 @@ hipError_t llt_resid_tiles(const double *dY, int N, int NB, char aform, char dia
 hipError_t resid_upper_block(const double *U, long long ldu, int N, int NB, int jt, char aform,
                              char diag, int iseed, double *rowsum, hipStream_t s);
 
+// ---- mixed precision (mixed.hip) ----
+hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt, long long ld,
+                  int nrhs, double alpha, double beta, hipStream_t s);
+
 }  // namespace ooc

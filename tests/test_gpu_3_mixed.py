@@ -1,0 +1,82 @@
+"""Mixed-precision factorization + FP64 iterative refinement (north-star
+addition; no reference code).  Checked against an FP64 oracle on the same
+inputs: numpy/LAPACK solve of the same matrix copied back from the device.
+Bars: normwise backward error <= tol (the FP64 level asked for), forward
+error within kappa * eps * 1e2 of the FP64 solution, and a correct answer
+whether or not the low-precision attempt fell back to FP64."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS = 2.0 ** -53
+
+
+def gram_host(ooc, buf, n, nb):
+    A = np.zeros((n, n), order="F")
+    ooc.tiles_to_host(buf, A, nb)
+    return np.tril(A) + np.tril(A, -1).T
+
+
+def test_gram_generator_properties(gpu):
+    ooc = gpu
+    n, nb, mu = 1024, 256, 1e-2
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, mu, seed=7)
+    A = gram_host(ooc, buf, n, nb)
+    # E[(G G^T / n)_ii] = 1, off-diagonal ~ N(0, 1/n)
+    assert abs(np.mean(np.diag(A)) - (1 + mu)) < 0.02
+    off = A[np.tril_indices(n, -1)]
+    assert abs(np.std(off) * np.sqrt(n) - 1.0) < 0.05
+    w = np.linalg.eigvalsh(A)
+    assert w.min() > mu * 0.99 and w.max() < 4.5
+    # diagonal tiles are stored full and symmetric
+    tiles = buf.to_array((ooc.tiles_elems(n, nb),))
+    d0 = tiles[: nb * nb].reshape(nb, nb, order="F")
+    assert np.array_equal(d0, d0.T)
+    # same seed -> same matrix
+    buf2 = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf2, n, nb, mu, seed=7)
+    assert np.array_equal(buf2.to_array((ooc.tiles_elems(n, nb),)), tiles)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("mu", [1e-2, 1e-6])
+def test_mpir_matches_fp64_oracle(gpu, prec, mu):
+    ooc = gpu
+    n, nb, nrhs = 2048, 256, 4
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, mu, seed=3)
+    A = gram_host(ooc, buf, n, nb)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=3, stream_id=2)
+    B = dB.to_array((n, nrhs))
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    tol = 8 * EPS
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, prec, tol=tol, max_iter=30)
+    X = dX.to_array((n, nrhs))
+    print(prec, mu, st)
+    # backward error recomputed independently on the host
+    R = B - A @ X
+    eta = max(np.abs(R[:, k]).max() / (np.abs(A).sum(1).max() * np.abs(X[:, k]).max() +
+                                        np.abs(B[:, k]).max()) for k in range(nrhs))
+    assert st["backward_error"] <= tol
+    assert eta <= 2 * tol
+    Xref = np.linalg.solve(A, B)
+    kappa = np.linalg.cond(A, 1)
+    ferr = np.abs(X - Xref).max() / np.abs(Xref).max()
+    assert ferr <= max(1e2 * kappa * EPS, 1e-13)
+    if prec == "fp32" and mu == 1e-2:
+        assert st["fell_back"] == 0 and st["precision_used"] == 1
+
+
+def test_mpir_non_spd_reports_info(gpu):
+    ooc = gpu
+    n, nb = 512, 128
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, -10.0, seed=1)   # shifted far below zero: indefinite
+    dB = ooc.DeviceBuffer(n * 8)
+    ooc.normal_fill(dB, n, 1, n)
+    dX = ooc.DeviceBuffer(n * 8)
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, 1, dX, "fp32")
+    assert st["info"] > 0 and st["fell_back"] == 1
