@@ -223,35 +223,54 @@ void unpack_column(double *A, int N, int NB, int LDA, int tj, int ti0, const dou
 }
 
 struct Streams {
-    hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr;
+    hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr, bulk = nullptr;
     bool own_comp = true;
     ~Streams() {
+        if (bulk) (void)hipStreamDestroy(bulk);
         if (h2d) (void)hipStreamDestroy(h2d);
         if (d2h) (void)hipStreamDestroy(d2h);
         if (comp && own_comp) (void)hipStreamDestroy(comp);
     }
 };
 
+// FP64 trailing update of a packed panel (ooc_syrk, oocquark.cpp:321-356).
+struct Fp64Updater : PanelUpdater {
+    Opnd P;
+    int NB, Ym;
+    hipError_t update(int j, int c0, int nc, hipStream_t s) override {
+        Gemm g;
+        g.A = P; g.A.r0 = c0 * NB; g.A.c0 = j * NB;
+        g.B = g.A;
+        g.C = P; g.C.r0 = c0 * NB; g.C.c0 = c0 * NB;
+        g.M = (Ym - c0) * NB;
+        g.N = nc * NB;
+        g.K = NB;
+        g.alpha = -1.0;
+        g.beta = 1.0;
+        g.lower = 1;
+        return launch_update(g, s);
+    }
+};
+
 // Factors tile columns [0, Yn) of a packed panel already resident in HBM
-// (ooc_incore, oocquark.cpp:358-421).  gcol0 = global column of panel
-// column 0.  on_column(j) is called after column j's factor kernels were
-// queued (so the caller can record an event and start its copy-out).
-template <class F>
+// (ooc_incore, oocquark.cpp:358-421) with look-ahead on two streams.
+// gcol0 = global column of panel column 0.  on_column(j) is called after
+// column j's factor kernels were queued on s0.
 int factor_panel(double *dY, int NB, int Ym, int Yn, int gcol0, double *dinv, int *dinfo,
-                 hipStream_t s, F on_column) {
-    Opnd P = panel(dY, NB, Ym, 1);
-    // progress reporting keeps the host at most g_progress_every columns
-    // ahead of the device and prints when a block of columns is finished
-    std::vector<hipEvent_t> marks;
+                 hipStream_t s0, hipStream_t s1, const std::function<int(int)> &on_column) {
+    Fp64Updater up;
+    up.P = panel(dY, NB, Ym, 1);
+    up.NB = NB;
+    up.Ym = Ym;
     const double tstart = now();
-    for (int j = 0; j < Yn; ++j) {
-        HIPTRY(factor_column(P, j * NB, (Ym - j) * NB, NB, gcol0 + j * NB, dinv, dinfo, s));
+    std::vector<hipEvent_t> marks;
+    auto progress = [&](int j) -> int {
         int rc = on_column(j);
         if (rc) return rc;
         if (g_progress_every > 0 && (j + 1) % g_progress_every == 0) {
             hipEvent_t e;
             HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            HIPTRY(hipEventRecord(e, s));
+            HIPTRY(hipEventRecord(e, s0));
             marks.push_back(e);
             if (marks.size() >= 2) {
                 HIPTRY(hipEventSynchronize(marks.front()));
@@ -262,24 +281,15 @@ int factor_panel(double *dY, int NB, int Ym, int Yn, int gcol0, double *dinv, in
                 fflush(stderr);
             }
         }
-        if (j + 1 < Yn) {
-            // right-looking update of the rest of the panel from column j
-            // (ooc_syrk(&Y(jb+1,jb), &Y(jb+1,jb+1), ...), oocquark.cpp:414-418)
-            Gemm g;
-            g.A = P; g.A.r0 = (j + 1) * NB; g.A.c0 = j * NB;
-            g.B = g.A;
-            g.C = P; g.C.r0 = (j + 1) * NB; g.C.c0 = (j + 1) * NB;
-            g.M = (Ym - j - 1) * NB;
-            g.N = (Yn - j - 1) * NB;
-            g.K = NB;
-            g.alpha = -1.0;
-            g.beta = 1.0;
-            g.lower = 1;
-            HIPTRY(launch_update(g, s));
-        }
-    }
+        return OOCDAG_OK;
+    };
+    int rc = factor_panel_la(up.P, NB, Ym, Yn, gcol0, dinv, 0, dinfo, s0, s1, up, progress);
     for (hipEvent_t e : marks) (void)hipEventDestroy(e);
-    return OOCDAG_OK;
+    if (rc == -4) {
+        set_err("HIP error in the panel factorization: %s", hipGetErrorString(hipGetLastError()));
+        return OOCDAG_EHIP;
+    }
+    return rc;
 }
 
 int read_info(int *dinfo, hipStream_t s, int N, int *info) {
@@ -350,6 +360,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     HIPTRY(hipStreamCreateWithFlags(&st.h2d, hipStreamNonBlocking));
     HIPTRY(hipStreamCreateWithFlags(&st.comp, hipStreamNonBlocking));
     HIPTRY(hipStreamCreateWithFlags(&st.d2h, hipStreamNonBlocking));
+    HIPTRY(hipStreamCreateWithFlags(&st.bulk, hipStreamNonBlocking));
 
     double *dY = nullptr;
     if (hipMalloc(&dY, (size_t)pl.tiles_needed * nb2 * sizeof(double)) != hipSuccess) {
@@ -463,7 +474,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             slot_used[s] = false;
             return OOCDAG_OK;
         };
-        rc = factor_panel(dY, NB, Ym, Yn, (int)jb * NB, sc.dinv, sc.dinfo, st.comp,
+        rc = factor_panel(dY, NB, Ym, Yn, (int)jb * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
                           [&](int jj) -> int {
                               int s = 2 + (jj & 1);
                               if (slot_used[s]) {
@@ -659,7 +670,9 @@ int oocdag_potrf_tiles(double *dY, int N, int NB, void *stream, int *info) {
     int rc = sc.init(NB, st.comp);
     if (rc) return rc;
     const double t0 = now();
-    rc = factor_panel(dY, NB, bb, bb, 0, sc.dinv, sc.dinfo, st.comp, [](int) { return 0; });
+    HIPTRY(hipStreamCreateWithFlags(&st.bulk, hipStreamNonBlocking));
+    rc = factor_panel(dY, NB, bb, bb, 0, sc.dinv, sc.dinfo, st.comp, st.bulk,
+                      [](int) { return 0; });
     if (rc) return rc;
     rc = read_info(sc.dinfo, st.comp, N, info);
     g_stat[0] = now() - t0;

@@ -15,6 +15,7 @@
 #include "ooc_internal.h"
 
 #include <cmath>
+#include <functional>
 
 namespace ooc {
 
@@ -31,66 +32,55 @@ __device__ __forceinline__ long long opnd_off(const Opnd &o, int r, int c) {
     return t * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
 }
 
-// One workgroup factors a 64 x 64 diagonal block held in LDS (right-looking,
-// one pivot column per step) and inverts the resulting triangle.
-// do_factor = 0: the block already holds a lower-triangular factor; only
-// its inverse is formed and the block is left untouched.
-__global__ __launch_bounds__(256) void k_potrf64(Opnd P, int gcol0, double *dinv, int *info,
-                                                 int do_factor) {
+// One wave64 factors a 64 x 64 diagonal block held in LDS and inverts the
+// resulting triangle; lane i owns row i during the factorization (left-
+// looking: column j is a dot product of row i with row j, then one square
+// root and a scaling) and column i during the inversion (independent forward
+// substitutions, no barriers).  do_factor = 0: the block already holds a
+// lower-triangular factor; only its inverse is formed and the block is left
+// untouched.
+__global__ __launch_bounds__(64) void k_potrf64(Opnd P, int gcol0, double *dinv, int *info,
+                                                int do_factor) {
     __shared__ double S[PB * PS];
     __shared__ double X[PB * PS];
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x;
     const long long base = opnd_off(P, 0, 0);  // block never straddles a tile
     const long long ld = P.ld;
     double *g = P.p + base;
 
-    for (int idx = tid; idx < PB * PB; idx += 256) {
-        int r = idx & (PB - 1), c = idx >> 6;
-        S[c * PS + r] = (r >= c) ? g[r + c * ld] : 0.0;
-    }
+    for (int c = 0; c < PB; ++c) S[c * PS + lane] = (lane >= c) ? g[lane + c * ld] : 0.0;
     __syncthreads();
 
-    for (int k = 0; k < PB && do_factor; ++k) {
-        if (tid == 0) {
-            double d = S[k * PS + k];
-            if (!(d > 0.0)) atomicMin(info, gcol0 + k + 1);
-            S[k * PS + k] = sqrt(d);
+    if (do_factor) {
+        for (int j = 0; j < PB; ++j) {
+            double v = 0.0;
+            if (lane >= j) {
+                v = S[j * PS + lane];
+                for (int k = 0; k < j; ++k) v -= S[k * PS + lane] * S[k * PS + j];
+            }
+            if (lane == j) {
+                if (!(v > 0.0)) atomicMin(info, gcol0 + j + 1);
+                S[j * PS + j] = sqrt(v);
+            }
+            __syncthreads();
+            if (lane > j) S[j * PS + lane] = v / S[j * PS + j];
+            __syncthreads();
         }
-        __syncthreads();
-        const double dk = S[k * PS + k];
-        if (tid > k && tid < PB) S[k * PS + tid] /= dk;
-        __syncthreads();
-        // trailing lower update: S[i][j] -= S[i][k]*S[j][k], k < j <= i
-        for (int idx = tid; idx < PB * PB; idx += 256) {
-            int i = idx & (PB - 1), j = idx >> 6;
-            if (j > k && i >= j) S[j * PS + i] -= S[k * PS + i] * S[k * PS + j];
-        }
-        __syncthreads();
+        for (int c = 0; c < PB; ++c)
+            if (lane >= c) g[lane + c * ld] = S[c * PS + lane];
     }
 
-    // write L back (lower triangle only)
-    if (do_factor)
-        for (int idx = tid; idx < PB * PB; idx += 256) {
-            int r = idx & (PB - 1), c = idx >> 6;
-            if (r >= c) g[r + c * ld] = S[c * PS + r];
-        }
-
-    // X = L^-1 by forward substitution, one column per thread.  Four threads
-    // share a column: each owns every fourth row, and the row-ordered sweep
-    // exchanges the finished entries through LDS.
-    const int col = tid >> 2, part = tid & 3;
-    for (int i = 0; i < PB; ++i) {
-        if (i >= col && (i & 3) == part) {
-            double s = (i == col) ? 1.0 : 0.0;
-            for (int d = col; d < i; ++d) s -= S[d * PS + i] * X[col * PS + d];
-            X[col * PS + i] = s / S[i * PS + i];
-        }
-        __syncthreads();
+    // X = L^-1, lane = column
+    const int col = lane;
+    for (int i = 0; i < col; ++i) X[col * PS + i] = 0.0;
+    X[col * PS + col] = 1.0 / S[col * PS + col];
+    for (int i = col + 1; i < PB; ++i) {
+        double sum = 0.0;
+        for (int d = col; d < i; ++d) sum += S[d * PS + i] * X[col * PS + d];
+        X[col * PS + i] = -sum / S[i * PS + i];
     }
-    for (int idx = tid; idx < PB * PB; idx += 256) {
-        int r = idx & (PB - 1), c = idx >> 6;
-        dinv[r + c * PB] = (r >= c) ? X[c * PS + r] : 0.0;
-    }
+    __syncthreads();
+    for (int c = 0; c < PB; ++c) dinv[lane + c * PB] = (lane >= c) ? X[c * PS + lane] : 0.0;
 }
 
 __global__ void k_pad_tile(double *t, int nb, int vr, int vc, int diag) {
@@ -205,7 +195,7 @@ LcgMap lcg_pow(unsigned long long n) {
 }  // namespace
 
 hipError_t launch_potrf64(const Opnd &P, int gcol0, double *dinv, int *info, hipStream_t s) {
-    hipLaunchKernelGGL(k_potrf64, dim3(1), dim3(256), 0, s, P, gcol0, dinv, info,
+    hipLaunchKernelGGL(k_potrf64, dim3(1), dim3(64), 0, s, P, gcol0, dinv, info,
                        info != nullptr ? 1 : 0);
     return hipGetLastError();
 }
@@ -251,6 +241,56 @@ hipError_t factor_column(const Opnd &P, int row0, int mrows, int ncols, int gcol
         }
     }
     return hipSuccess;
+}
+
+int factor_panel_la(const Opnd &P, int NB, int Ym, int Yn, int gcol0, double *dinv,
+                    size_t dinv_stride, int *dinfo, hipStream_t s0, hipStream_t s1,
+                    PanelUpdater &up, const std::function<int(int)> &on_column) {
+    hipEvent_t ev_fact, ev_bulk[2];
+    if (hipEventCreateWithFlags(&ev_fact, hipEventDisableTiming) != hipSuccess) return -4;
+    if (hipEventCreateWithFlags(&ev_bulk[0], hipEventDisableTiming) != hipSuccess) return -4;
+    if (hipEventCreateWithFlags(&ev_bulk[1], hipEventDisableTiming) != hipSuccess) return -4;
+    struct Cleanup {
+        hipEvent_t *e;
+        ~Cleanup() {
+            (void)hipEventDestroy(e[0]);
+            (void)hipEventDestroy(e[1]);
+            (void)hipEventDestroy(e[2]);
+        }
+    };
+    hipEvent_t all[3] = {ev_fact, ev_bulk[0], ev_bulk[1]};
+    Cleanup cl{all};
+    int rc = 0;
+    bool any_bulk = false;
+#define LA_TRY(x)                       \
+    do {                                \
+        if ((x) != hipSuccess) return -4; \
+    } while (0)
+    for (int j = 0; j < Yn; ++j) {
+        // column j received bulk(j-2) on s1
+        if (j >= 2) LA_TRY(hipStreamWaitEvent(s0, ev_bulk[j & 1], 0));
+        LA_TRY(factor_column(P, j * NB, (Ym - j) * NB, NB, gcol0 + j * NB,
+                             dinv + (size_t)j * dinv_stride, dinfo, s0));
+        if ((rc = on_column(j)) != 0) return rc;
+        if (j + 1 >= Yn) break;
+        LA_TRY(up.prepare(j, s0));
+        // column j+1 also received bulk(j-1)
+        if (j >= 1) LA_TRY(hipStreamWaitEvent(s0, ev_bulk[(j - 1) & 1], 0));
+        LA_TRY(up.update(j, j + 1, 1, s0));
+        if (j + 2 < Yn) {
+            LA_TRY(hipEventRecord(ev_fact, s0));
+            LA_TRY(hipStreamWaitEvent(s1, ev_fact, 0));
+            LA_TRY(up.update(j, j + 2, Yn - j - 2, s1));
+            LA_TRY(hipEventRecord(ev_bulk[j & 1], s1));
+            any_bulk = true;
+        }
+    }
+    if (any_bulk) {
+        LA_TRY(hipEventRecord(ev_fact, s1));
+        LA_TRY(hipStreamWaitEvent(s0, ev_fact, 0));
+    }
+#undef LA_TRY
+    return 0;
 }
 
 hipError_t launch_pad_tile(double *tile, int nb, int vr, int vc, int diag, hipStream_t s) {

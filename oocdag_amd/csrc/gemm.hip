@@ -241,10 +241,12 @@ __device__ __forceinline__ long long xcd_remap(long long bid, long long G) {
     return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
-template <int BN, int MODE, int KIND, int TB>
+template <int BMT, int BN, int MODE, int KIND, int TB>
 __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
+    constexpr int BM = BMT;         // block rows (128, or 32 for few right-hand sides)
+    constexpr int WM = BM / 2;      // wave tile rows
     constexpr int WN = BN / 2;      // wave tile columns
-    constexpr int FM = 4;           // 64 / 16
+    constexpr int FM = WM / 16;
     constexpr int FN = WN / 16;
     constexpr int SA = BM + LPAD;   // LDS row strides (doubles)
     constexpr int SB = BN + LPAD;
@@ -303,7 +305,7 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
             const int kk = ks * 4 + (lane >> 4);
             double fa[FM], fb[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) fa[i] = as[kk * SA + wm * 64 + i * 16 + (lane & 15)];
+            for (int i = 0; i < FM; ++i) fa[i] = as[kk * SA + wm * WM + i * 16 + (lane & 15)];
 #pragma unroll
             for (int j = 0; j < FN; ++j) fb[j] = bs[kk * SB + wn * WN + j * 16 + (lane & 15)];
 #pragma unroll
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     const int cts = Co.ts > 0 ? Co.ts : 0x40000000;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-        const int mb16 = m0 + wm * 64 + i * 16;          // first row of the 16x16 block
+        const int mb16 = m0 + wm * WM + i * 16;          // first row of the 16x16 block
         const int m = mb16 + (lane >> 4);
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     }
 }
 
-template <int BN, int MODE, int KIND, int TB = 0>
+template <int BN, int MODE, int KIND, int TB = 0, int BM = 128>
 hipError_t launch_one(const Gemm &g, hipStream_t s) {
     KArgs a;
     a.A = g.A;
@@ -383,7 +385,8 @@ hipError_t launch_one(const Gemm &g, hipStream_t s) {
     }
     if (a.nblk <= 0) return hipSuccess;
     if (a.nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_dgemm_nt<BN, MODE, KIND, TB>), dim3((unsigned)a.nblk), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((k_dgemm_nt<BM, BN, MODE, KIND, TB>), dim3((unsigned)a.nblk), dim3(NT), 0, s,
+                       a);
     return hipGetLastError();
 }
 
@@ -399,7 +402,8 @@ hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
         if (o->ts > 0 && ((o->ts % BK) != 0)) return hipErrorInvalidValue;
         if (((uintptr_t)o->p) & 15) return hipErrorInvalidValue;
     }
-    const bool tb = kind == GEMM_GENERAL_TB || kind == GEMM_NARROW_TB || kind == GEMM_APPLY_TB;
+    const bool tb = kind == GEMM_GENERAL_TB || kind == GEMM_NARROW_TB || kind == GEMM_APPLY_TB ||
+                    kind == GEMM_SKINNY_TB;
     if ((g.A.ts > 0 && (g.A.c0 % BK))) return hipErrorInvalidValue;
     if (!tb && g.B.ts > 0 && (g.B.c0 % BK)) return hipErrorInvalidValue;
     if (tb && g.B.ts > 0 && (g.B.r0 % BK)) return hipErrorInvalidValue;
@@ -420,6 +424,10 @@ hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
         return launch_one<64, 0, 5, 1>(g, s);
     case GEMM_APPLY_TB:
         return launch_one<64, 0, 6, 1>(g, s);
+    case GEMM_SKINNY:
+        return launch_one<128, 0, 7, 0, 32>(g, s);
+    case GEMM_SKINNY_TB:
+        return launch_one<128, 0, 8, 1, 32>(g, s);
     case GEMM_GENERAL:
     default:
         return launch_one<128, 0, 3>(g, s);

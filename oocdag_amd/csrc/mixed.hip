@@ -163,10 +163,10 @@ hipError_t normal_block(double *dB, int rows, int cols, long long ld, unsigned l
     return hipGetLastError();
 }
 
-// Y^T (nrhs x Np) = alpha * X^T A + beta * Y^T for symmetric A in packed lower
-// tiles with full diagonal tiles: per tile column j,
-//   Y^T[:, j]      += X^T[:, rows >= j] A[rows >= j, j]      (stored, read transposed)
-//   Y^T[:, rows>j] += X^T[:, j] A[rows > j, j]^T
+// Y^T (nrhs x Np) += alpha * X^T A for symmetric A in packed lower tiles
+// with full diagonal tiles, as 2*bb launches with K = NB each:
+//   tile row i   : Y^T[:, :(i+1)NB] += X^T[:, i] A[i, :(i+1)NB]   (stored, read transposed)
+//   tile column j: Y^T[:, rows > j]  += X^T[:, j] A[rows > j, j]^T
 hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt, long long ld,
                   int nrhs, double alpha, double beta, hipStream_t s) {
     const int bb = (N + NB - 1) / NB;
@@ -182,65 +182,160 @@ hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt,
         g.alpha = 0.0; g.beta = beta; g.lower = 0;
         if ((e = launch_gemm(g, GEMM_GENERAL, s)) != hipSuccess) return e;
     }
-    for (int j = 0; j < bb; ++j) {
+    for (int i = 0; i < bb; ++i) {
         Gemm g;
-        g.A = plain((double *)Xt, ld, 0, j * NB);
-        g.B = Ap; g.B.r0 = j * NB; g.B.c0 = j * NB;
-        g.C = plain(Yt, ld, 0, j * NB);
-        g.M = nrhs; g.N = NB; g.K = (bb - j) * NB;
+        g.A = plain((double *)Xt, ld, 0, i * NB);
+        g.B = Ap; g.B.r0 = i * NB; g.B.c0 = 0;  // operand (n, k) = A(i*NB + k, n)
+        g.C = plain(Yt, ld, 0, 0);
+        g.M = nrhs; g.N = (i + 1) * NB; g.K = NB;
         g.alpha = alpha; g.beta = 1.0; g.lower = 0;
-        if ((e = launch_gemm(g, GEMM_GENERAL_TB, s)) != hipSuccess) return e;
-        if (j + 1 < bb) {
+        if ((e = launch_gemm(g, nrhs <= 32 ? GEMM_SKINNY_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
+        if (i + 1 < bb) {
             Gemm h;
-            h.A = plain((double *)Xt, ld, 0, j * NB);
-            h.B = Ap; h.B.r0 = (j + 1) * NB; h.B.c0 = j * NB;
-            h.C = plain(Yt, ld, 0, (j + 1) * NB);
-            h.M = nrhs; h.N = (bb - j - 1) * NB; h.K = NB;
+            h.A = plain((double *)Xt, ld, 0, i * NB);
+            h.B = Ap; h.B.r0 = (i + 1) * NB; h.B.c0 = i * NB;
+            h.C = plain(Yt, ld, 0, (i + 1) * NB);
+            h.M = nrhs; h.N = (bb - i - 1) * NB; h.K = NB;
             h.alpha = alpha; h.beta = 1.0; h.lower = 0;
-            if ((e = launch_gemm(h, GEMM_GENERAL, s)) != hipSuccess) return e;
+            if ((e = launch_gemm(h, nrhs <= 32 ? GEMM_SKINNY : GEMM_GENERAL, s)) != hipSuccess) return e;
         }
     }
     return hipSuccess;
 }
 
-// Factor W (packed tiles) with the given update precision; 64-block inverses
-// of every diagonal tile land in dinv_all for the solves.
-int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int *dinfo,
-                 void *lpbuf, float *scale, hipStream_t s) {
-    const int bb = (N + NB - 1) / NB;
-    const Opnd P = panel(W, NB, bb, 1);
-    const size_t per = (size_t)NB * NB;
-    const size_t inv_per_tile = (size_t)(NB / 64) * 64 * 64;
-    int rowmap = 0;
-    if (precision > 0) {
-        rowmap = mfma_rowmap(precision - 1, s);
-        if (rowmap < 0) return -1;
+__global__ void k_identity(double *Z, int nb, int count) {
+    const long long total = (long long)nb * nb * count;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        long long w = idx % ((long long)nb * nb);
+        Z[idx] = ((w % nb) == (w / nb)) ? 1.0 : 0.0;
     }
-    for (int j = 0; j < bb; ++j) {
-        if (factor_column(P, j * NB, (bb - j) * NB, NB, j * NB, dinv_all + j * inv_per_tile, dinfo,
-                          s) != hipSuccess)
-            return -2;
-        if (j + 1 == bb) break;
-        const int M = (bb - j - 1) * NB;
-        if (precision == 0) {
+}
+
+// Z_k = L_kk^-T for every diagonal tile (NB x NB each, ld NB), from the
+// 64-block inverses the factorization left in dinv_all.
+hipError_t tile_inverses(const double *dY, int N, int NB, const double *dinv_all, double *Z,
+                         hipStream_t s) {
+    const int bb = (N + NB - 1) / NB;
+    const Opnd Lp = panel((double *)dY, NB, bb, 1);
+    const size_t per = (size_t)NB * NB, inv_per_tile = (size_t)(NB / 64) * 4096;
+    hipLaunchKernelGGL(k_identity, dim3(4096), dim3(256), 0, s, Z, NB, bb);
+    hipError_t e = hipGetLastError();
+    for (int k = 0; k < bb && e == hipSuccess; ++k) {
+        Opnd Lk = Lp;
+        Lk.r0 = k * NB;
+        Lk.c0 = k * NB;
+        e = trsm_right_lower_t(plain(Z + k * per, NB), NB, NB, Lk, dinv_all + k * inv_per_tile, s);
+    }
+    return e;
+}
+
+// X^T <- B^T A^-1 with the diagonal tiles' inverses Z (one GEMM per tile per
+// sweep; T: nrhs x NB scratch, ld ldx).
+hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double *Xt, int nrhs,
+                       long long ldx, double *T, hipStream_t s) {
+    const int bb = (N + NB - 1) / NB;
+    const Opnd Lp = panel((double *)dY, NB, bb, 1);
+    const size_t per = (size_t)NB * NB;
+    hipError_t e;
+    auto diag = [&](int k, GemmKind kind) -> hipError_t {
+        Gemm g;
+        g.A = plain(Xt, ldx, 0, k * NB);
+        g.B = plain((double *)Z + k * per, NB);
+        g.C = plain(T, ldx);
+        g.M = nrhs; g.N = NB; g.K = NB;
+        g.alpha = 1.0; g.beta = 0.0; g.lower = 0;
+        hipError_t r = launch_gemm(g, kind, s);
+        if (r != hipSuccess) return r;
+        return hipMemcpy2DAsync(Xt + (size_t)k * NB * ldx, ldx * sizeof(double), T,
+                                ldx * sizeof(double), nrhs * sizeof(double), NB,
+                                hipMemcpyDeviceToDevice, s);
+    };
+    for (int k = 0; k < bb; ++k) {  // Y^T_k = B^T_k L_kk^-T = B^T_k Z_k
+        if ((e = diag(k, nrhs <= 32 ? GEMM_SKINNY_TB : GEMM_GENERAL_TB)) != hipSuccess) return e;
+        if (k + 1 < bb) {
             Gemm g;
-            g.A = P; g.A.r0 = (j + 1) * NB; g.A.c0 = j * NB;
-            g.B = g.A;
-            g.C = P; g.C.r0 = (j + 1) * NB; g.C.c0 = (j + 1) * NB;
-            g.M = M; g.N = M; g.K = NB;
-            g.alpha = -1.0; g.beta = 1.0; g.lower = 1;
-            if (launch_gemm(g, GEMM_UPDATE, s) != hipSuccess) return -2;
-        } else {
-            const int mode = precision - 1;
-            const size_t esz = mode == 0 ? 4 : 2;
-            if (convert_column(W, NB, bb, j, bb - j, mode, scale, lpbuf, s) != hipSuccess) return -2;
-            const void *X = (const char *)lpbuf + per * esz;  // skip the diagonal tile
-            if (launch_update_lp(mode, W, bb, (j + 1) * NB, X, scale + 1, NB, M, M, rowmap, s) !=
-                hipSuccess)
-                return -2;
+            g.A = plain(Xt, ldx, 0, k * NB);
+            g.B = Lp; g.B.r0 = (k + 1) * NB; g.B.c0 = k * NB;
+            g.C = plain(Xt, ldx, 0, (k + 1) * NB);
+            g.M = nrhs; g.N = (bb - k - 1) * NB; g.K = NB;
+            g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
+            if ((e = launch_gemm(g, nrhs <= 32 ? GEMM_SKINNY : GEMM_GENERAL, s)) != hipSuccess) return e;
         }
     }
-    return 0;
+    for (int k = bb - 1; k >= 0; --k) {  // X^T_k = Y^T_k L_kk^-1 = Y^T_k Z_k^T
+        if ((e = diag(k, nrhs <= 32 ? GEMM_SKINNY : GEMM_GENERAL)) != hipSuccess) return e;
+        if (k > 0) {
+            Gemm g;
+            g.A = plain(Xt, ldx, 0, k * NB);
+            g.B = Lp; g.B.r0 = k * NB; g.B.c0 = 0;
+            g.C = plain(Xt, ldx, 0, 0);
+            g.M = nrhs; g.N = k * NB; g.K = NB;
+            g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
+            if ((e = launch_gemm(g, nrhs <= 32 ? GEMM_SKINNY_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// Trailing updates of the mixed-precision factorization.  precision 0:
+// FP64 GEMM; 1/2: convert the final tile column to FP32/BF16 (double-buffered
+// by column parity, so column j+1's conversion never overwrites what the
+// bulk update of column j is still reading) and run the low-precision MFMA
+// update into the FP64 tiles.
+struct MixedUpdater : PanelUpdater {
+    double *W;
+    int NB, bb, precision, rowmap;
+    void *lp[2];
+    float *scale[2];
+    hipError_t prepare(int j, hipStream_t s) override {
+        if (precision == 0) return hipSuccess;
+        return convert_column(W, NB, bb, j, bb - j, precision - 1, scale[j & 1], lp[j & 1], s);
+    }
+    hipError_t update(int j, int c0, int nc, hipStream_t s) override {
+        if (precision == 0) {
+            const Opnd P = panel(W, NB, bb, 1);
+            Gemm g;
+            g.A = P; g.A.r0 = c0 * NB; g.A.c0 = j * NB;
+            g.B = g.A;
+            g.C = P; g.C.r0 = c0 * NB; g.C.c0 = c0 * NB;
+            g.M = (bb - c0) * NB; g.N = nc * NB; g.K = NB;
+            g.alpha = -1.0; g.beta = 1.0; g.lower = 1;
+            return launch_gemm(g, GEMM_UPDATE, s);
+        }
+        const int mode = precision - 1;
+        const size_t esz = mode == 0 ? 4 : 2;
+        const size_t skip = (size_t)(c0 - j) * NB * NB * esz;  // tiles j .. c0-1
+        return launch_update_lp(mode, W, bb, c0 * NB, (const char *)lp[j & 1] + skip,
+                                scale[j & 1] + (c0 - j), NB, (bb - c0) * NB, nc * NB, rowmap, s);
+    }
+};
+
+// Factor W (packed tiles) with the given update precision; 64-block inverses
+// of every diagonal tile land in dinv_all for the solves.  lpbuf: two
+// low-precision tile columns (bb tiles of 4 bytes each, each), scale: 2*bb.
+int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int *dinfo,
+                 void *lpbuf, float *scale, hipStream_t s, hipStream_t s1) {
+    const int bb = (N + NB - 1) / NB;
+    MixedUpdater up;
+    up.W = W;
+    up.NB = NB;
+    up.bb = bb;
+    up.precision = precision;
+    up.rowmap = 0;
+    const size_t col_bytes = (size_t)bb * NB * NB * 4;
+    up.lp[0] = lpbuf;
+    up.lp[1] = (char *)lpbuf + col_bytes;
+    up.scale[0] = scale;
+    up.scale[1] = scale + bb;
+    if (precision > 0) {
+        up.rowmap = mfma_rowmap(precision - 1, s);
+        if (up.rowmap < 0) return -1;
+    }
+    const size_t inv_per_tile = (size_t)(NB / 64) * 64 * 64;
+    int rc = factor_panel_la(panel(W, NB, bb, 1), NB, bb, bb, 0, dinv_all, inv_per_tile, dinfo, s,
+                             s1, up, [](int) { return 0; });
+    return rc ? -2 : 0;
 }
 
 }  // namespace ooc
@@ -287,20 +382,28 @@ extern "C" int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const do
         precision < 0 || precision > 2)
         return OOCDAG_EARG;
     if (precision == 2 && NB % 64) return OOCDAG_EARG;
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr, s1 = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return OOCDAG_EHIP;
+    if (hipStreamCreateWithFlags(&s1, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return OOCDAG_EHIP;
+    }
     struct SD {
-        hipStream_t s;
-        ~SD() { (void)hipStreamDestroy(s); }
-    } sd{s};
+        hipStream_t s, s1;
+        ~SD() {
+            (void)hipStreamDestroy(s1);
+            (void)hipStreamDestroy(s);
+        }
+    } sd{s, s1};
     const double t0 = now();
     const int bb = (N + NB - 1) / NB;
     const long long Np = (long long)bb * NB;
     const long long ldt = nrhs + (nrhs & 1);
     const size_t tiles = (size_t)bb * (bb + 1) / 2 * NB * NB;
-    Dev W, dinv, info, lp, scale, Bt, Xt, Rt, anorm, nrm;
+    Dev W, dinv, info, lp, scale, Bt, Xt, Rt, anorm, nrm, Z, T;
     if (!W.alloc(tiles * 8) || !dinv.alloc((size_t)Np * 64 * 8) || !info.alloc(sizeof(int)) ||
-        !lp.alloc((size_t)bb * NB * NB * 4) || !scale.alloc((size_t)bb * 4) ||
+        !Z.alloc((size_t)bb * NB * NB * 8) || !T.alloc((size_t)ldt * NB * 8) ||
+        !lp.alloc((size_t)2 * bb * NB * NB * 4) || !scale.alloc((size_t)2 * bb * 4) ||
         !Bt.alloc((size_t)(ldt * Np) * 8) || !Xt.alloc((size_t)(ldt * Np) * 8) ||
         !Rt.alloc((size_t)(ldt * Np) * 8) || !anorm.alloc((size_t)Np * 8) ||
         !nrm.alloc((size_t)2 * ldt * 8))
@@ -332,9 +435,11 @@ extern "C" int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const do
         if (!ok(hipMemcpyAsync(W.p, dA, tiles * 8, hipMemcpyDeviceToDevice, s)) ||
             !ok(hipMemcpyAsync(info.p, &big, sizeof(int), hipMemcpyHostToDevice, s)))
             return OOCDAG_EHIP;
-        int rc = factor_mixed(W.d(), N, NB, prec, dinv.d(), (int *)info.p, lp.p, (float *)scale.p, s);
+        int rc = factor_mixed(W.d(), N, NB, prec, dinv.d(), (int *)info.p, lp.p, (float *)scale.p, s,
+                              s1);
         if (rc == -1) return OOCDAG_EHIP;  // MFMA layout probe failed
         if (rc) return OOCDAG_EHIP;
+        if (!ok(tile_inverses(W.d(), N, NB, dinv.d(), Z.d(), s))) return OOCDAG_EHIP;
         int hinfo = INT_MAX;
         if (!ok(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s)) ||
             !ok(hipStreamSynchronize(s)))
@@ -351,7 +456,7 @@ extern "C" int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const do
         if (good) {
             // x = W^-1 b
             if (!ok(hipMemcpyAsync(Xt.p, Bt.p, (size_t)(ldt * Np) * 8, hipMemcpyDeviceToDevice, s)) ||
-                !ok(potrs_tiles_t(W.d(), N, NB, dinv.d(), Xt.d(), nrhs, ldt, s)))
+                !ok(potrs_inv_t(W.d(), N, NB, Z.d(), Xt.d(), nrhs, ldt, T.d(), s)))
                 return OOCDAG_EHIP;
             for (;;) {
                 // r = b - A x
@@ -377,7 +482,7 @@ extern "C" int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const do
                     break;
                 }
                 // x += W^-1 r
-                if (!ok(potrs_tiles_t(W.d(), N, NB, dinv.d(), Rt.d(), nrhs, ldt, s))) return OOCDAG_EHIP;
+                if (!ok(potrs_inv_t(W.d(), N, NB, Z.d(), Rt.d(), nrhs, ldt, T.d(), s))) return OOCDAG_EHIP;
                 hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(256), 0, s, Xt.d(), Rt.d(), ldt * Np, 1.0);
                 ++it;
             }

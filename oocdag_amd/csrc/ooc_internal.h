@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 
 namespace ooc {
 
@@ -65,7 +66,10 @@ enum GemmKind {
     // the triangular solves with L (not L^T) on the right.
     GEMM_GENERAL_TB = 4,
     GEMM_NARROW_TB = 5,
-    GEMM_APPLY_TB = 6
+    GEMM_APPLY_TB = 6,
+    // 32 x 128 blocks for M <= 32 (a few right-hand sides in the solves)
+    GEMM_SKINNY = 7,
+    GEMM_SKINNY_TB = 8
 };
 
 hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s);
@@ -88,6 +92,28 @@ hipError_t launch_potrf64(const Opnd &P, int gcol0, double *dinv, int *info, hip
 // Writes L in place (lower triangle of the diagonal part).
 hipError_t factor_column(const Opnd &P, int row0, int mrows, int ncols, int gcol_base,
                          double *dinv_ws, int *info, hipStream_t s);
+
+// Trailing-update policy of a panel factorization.
+struct PanelUpdater {
+    virtual ~PanelUpdater() {}
+    // called on the critical-path stream once tile column j is final
+    virtual hipError_t prepare(int j, hipStream_t s) { (void)j; (void)s; return hipSuccess; }
+    // apply the update from tile column j to panel tile columns [c0, c0 + nc)
+    // (rows c0*NB and below)
+    virtual hipError_t update(int j, int c0, int nc, hipStream_t s) = 0;
+};
+
+// Factors tile columns [0, Yn) of the packed panel P (Ym tile rows) with
+// look-ahead: column j's factorization and the update of column j+1 run on
+// s0 while the bulk update of columns j+2.. from column j runs on s1.
+// gcol0: global column of panel column 0 (for info); dinv + j*dinv_stride
+// receives column j's 64-block inverses (stride 0: reuse one buffer).
+// on_column(j) runs right after column j's kernels were queued on s0 (the
+// column is final once s0 reaches that point); a nonzero return aborts.
+// Returns an OOCDAG-style code (0 ok, -4 HIP error, other: on_column's).
+int factor_panel_la(const Opnd &P, int NB, int Ym, int Yn, int gcol0, double *dinv,
+                    size_t dinv_stride, int *dinfo, hipStream_t s0, hipStream_t s1,
+                    PanelUpdater &up, const std::function<int(int)> &on_column);
 
 // Make a staged tile whose valid part is vr x vc behave like an NB x NB
 // tile of the zero-padded problem: entries outside the valid part become 0,
