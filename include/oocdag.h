@@ -157,6 +157,38 @@ int oocdag_normal_fill(double *dB, int rows, int cols, int ld, unsigned long lon
                        unsigned long long stream_id);
 
 /* ---------------------------------------------------------------------------
+ * Batched small SPD factor + solve (no reference counterpart).  nsys
+ * independent systems A_s x_s = b_s, n_s in [1, 1024], nrhs_s in [1, 8], in
+ * device arenas laid out by oocdag_batch_layout (A_s n x n and B_s/X_s
+ * n x nrhs, column-major with leading dimension n rounded up to even).  One
+ * persistent kernel (work queue over the systems, largest first; each
+ * workgroup factors and solves one system with LDS-staged panels) replayed
+ * through a captured hipGraph `reps` times; its block size and register
+ * tiling are picked by timing the compiled variants on a sample of the
+ * batch.  precision 0: FP64 factor; 1: FP32 factor + FP64 refinement to the
+ * backward error tol.  status_s: 0 ok, k > 0 not positive definite at
+ * column k, -2 refinement did not reach tol (rerun it with precision 0).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    double seconds;             /* one batch, averaged over reps */
+    double systems_per_second;
+    double max_backward_error;  /* over systems with status 0 */
+    int failed;                 /* systems with status != 0 */
+    int variant_T, variant_RB;  /* chosen block size and register tile */
+    double tune_seconds;
+} oocdag_batch_stats;
+
+long long oocdag_batch_layout(int nsys, const int *n, const int *nrhs, long long *a_off,
+                              long long *b_off);
+int oocdag_batch_generate(int nsys, const int *n, const int *nrhs, const long long *a_off,
+                          const long long *b_off, double *dA, double *dB, double mu,
+                          unsigned long long seed);
+int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long long *a_off,
+                         const long long *b_off, const double *dA, const double *dB, double *dX,
+                         int precision, double tol, int max_iter, int reps, double *berr,
+                         int *iters, int *status, oocdag_batch_stats *stats);
+
+/* ---------------------------------------------------------------------------
  * Tile kernels (device pointers, column-major, explicit leading dimensions).
  * lda/ldb/ldc must be even and pointers 16-byte aligned.
  * Replaces: cublasDgemm_v2(N,T)   oocquark.cpp:222  (CORE_incore_dgemm)

@@ -88,6 +88,25 @@ SIGNATURES.update({
                                      C.c_ulonglong]),
 })
 
+class BatchStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("systems_per_second", C.c_double),
+                ("max_backward_error", C.c_double), ("failed", C.c_int), ("variant_T", C.c_int),
+                ("variant_RB", C.c_int), ("tune_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_llp = C.POINTER(C.c_longlong)
+SIGNATURES.update({
+    "oocdag_batch_layout": (C.c_longlong, [C.c_int, _ip, _ip, _llp, _llp]),
+    "oocdag_batch_generate": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
+                                        C.c_double, C.c_ulonglong]),
+    "oocdag_batched_solve": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _ip,
+                                       _ip, C.POINTER(BatchStats)]),
+})
+
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
 
 
@@ -286,6 +305,63 @@ def mpir_solve_tiles(A: DeviceBuffer, n: int, nb: int, B: DeviceBuffer, nrhs: in
                                          PRECISIONS.get(precision, precision), tol, max_iter,
                                          C.byref(st)), "mpir_solve_tiles")
     return st.as_dict()
+
+
+class Batch:
+    """A batch of small SPD systems in device arenas (oocdag_batch_layout)."""
+
+    def __init__(self, n, nrhs):
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+        k = len(self.n)
+        self.a_off = np.zeros(k, dtype=np.int64)
+        self.b_off = np.zeros(k + 1, dtype=np.int64)
+        total_a = lib().oocdag_batch_layout(k, self._i(self.n), self._i(self.nrhs),
+                                            self._l(self.a_off), self._l(self.b_off))
+        self.A = DeviceBuffer(max(total_a, 1) * 8)
+        self.B = DeviceBuffer(max(int(self.b_off[k]), 1) * 8)
+        self.X = DeviceBuffer(max(int(self.b_off[k]), 1) * 8)
+
+    @staticmethod
+    def _i(a):
+        return a.ctypes.data_as(_ip)
+
+    @staticmethod
+    def _l(a):
+        return a.ctypes.data_as(_llp)
+
+    def ld(self, s):
+        return int(self.n[s] + (self.n[s] & 1))
+
+    def generate(self, mu=0.1, seed=1):
+        _check(lib().oocdag_batch_generate(len(self.n), self._i(self.n), self._i(self.nrhs),
+                                           self._l(self.a_off), self._l(self.b_off), self.A.ptr,
+                                           self.B.ptr, mu, seed), "batch_generate")
+
+    def solve(self, precision="fp64", tol=None, max_iter=10, reps=1):
+        k = len(self.n)
+        berr = np.zeros(k)
+        iters = np.zeros(k, dtype=np.int32)
+        status = np.zeros(k, dtype=np.int32)
+        st = BatchStats()
+        if tol is None:
+            tol = 8 * 2.0 ** -53
+        _check(lib().oocdag_batched_solve(k, self._i(self.n), self._i(self.nrhs),
+                                          self._l(self.a_off), self._l(self.b_off), self.A.ptr,
+                                          self.B.ptr, self.X.ptr, PRECISIONS[precision], tol,
+                                          max_iter, reps, berr.ctypes.data_as(_dp),
+                                          self._i(iters), self._i(status), C.byref(st)),
+               "batched_solve")
+        return st.as_dict(), berr, iters, status
+
+    def host_system(self, s):
+        """(A, B, X) of system s copied to the host."""
+        n, r, ld = int(self.n[s]), int(self.nrhs[s]), self.ld(s)
+        def get(buf, off, cols):
+            out = np.empty(ld * cols)
+            _check(lib().oocdag_memcpy_d2h(out.ctypes.data, buf.ptr + int(off) * 8, out.nbytes), "d2h")
+            return out.reshape(ld, cols, order="F")[:n]
+        return get(self.A, self.a_off[s], n), get(self.B, self.b_off[s], r), get(self.X, self.b_off[s], r)
 
 
 def tiles_from_host(buf: DeviceBuffer, A: np.ndarray, nb: int):

@@ -1,0 +1,656 @@
+// Batched small SPD factor + solve in one persistent kernel (no reference
+// counterpart: the reference factors one large matrix).
+//
+// Each workgroup (256 threads) pulls system ids from an atomic work queue
+// (systems pre-sorted largest first) and runs the whole system inside the
+// workgroup:
+//   1. W = A (FT = double or float working copy, column-major, ld n),
+//   2. right-looking blocked Cholesky with block T: the T x T diagonal block
+//      is factored and inverted by one wave in LDS; the panel below is
+//      solved row by row against the inverse and staged in LDS; the trailing
+//      lower triangle is updated with RB x RB register blocks reading the
+//      staged panel,
+//   3. x = W^-1 b by blocked forward/backward substitution with the stored
+//      diagonal-block inverses, right-hand sides resident in LDS,
+//   4. FP64 residual r = b - A x, normwise backward error
+//      max_k ||r_k|| / (||A|| ||x_k|| + ||b_k||), and (always for FT = float,
+//      else if above tol) refinement x += W^-1 r until the error is <= tol.
+// Systems whose factor is not positive definite or whose refinement stalls
+// report status != 0 so the caller can rerun them in FP64.
+//
+// Variants (T, RB, FT) are compiled ahead; the host autotuner times them on
+// a sample of the batch and launches the fastest through a captured hipGraph.
+#include "ooc_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "../../include/oocdag.h"
+
+namespace ooc {
+namespace {
+
+constexpr int NTB = 256;
+constexpr int PMAX = 1024;        // max staged panel rows
+constexpr int NMAX = PMAX + 64;   // max system order
+constexpr int RMAX = 8;           // max right-hand sides per system
+
+struct SysDesc {
+    int n, nrhs, ld;       // ld: even leading dimension >= n of A, B, X and W
+    long long a, b, w, d;  // offsets: A/B/X (doubles), W and inverses (FT elements)
+};
+
+struct BatchArgs {
+    const SysDesc *sys;
+    const int *order;
+    int nsys;
+    const double *A, *B;
+    double *X;
+    void *W, *Dinv;
+    double *berr;
+    int *iters, *status;
+    int *counter;
+    double tol;
+    int max_iter;
+};
+
+template <typename FT, int T>
+struct Smem {
+    // panel (factorization) and solve vectors share storage
+    union {
+        FT P[PMAX * (T + 1)];
+        struct {
+            double y[RMAX * NMAX];
+            double x[RMAX * NMAX];
+        } v;
+    } u;
+    FT D[T * (T + 1)];
+    FT Di[T * (T + 1)];
+    double red[NTB];
+    double tmp[RMAX * 64];
+    int bad;
+    int sid;
+};
+
+// Orders LDS traffic between the lanes of one wave (the wave runs in lock
+// step; this only stops the compiler from moving LDS accesses across it).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void tri_lower_decode(int idx, int &bi, int &bj) {
+    int r = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= idx) ++r;
+    while (r * (r + 1) / 2 > idx) --r;
+    bi = r;
+    bj = idx - r * (r + 1) / 2;
+}
+
+// x <- W^-1 y for all right-hand sides held in S.u.v.y (overwritten with x)
+template <typename FT, int T>
+__device__ void tri_solve(Smem<FT, T> &S, const FT *W, const FT *Dinv, int n, int ld, int nrhs) {
+    const int tid = threadIdx.x;
+    double *y = S.u.v.y;
+    const int nbk = (n + T - 1) / T;
+    // forward: y_blk = Di y_blk ; y[rest] -= L[rest, blk] y_blk
+    for (int kb = 0; kb < nbk; ++kb) {
+        const int k0 = kb * T, w = min(T, n - k0);
+        const FT *Di = Dinv + (long long)kb * T * T;
+        if (tid < w * nrhs) {
+            int c = tid % w, k = tid / w;
+            double s = 0.0;
+            for (int d = 0; d <= c; ++d) s += (double)Di[c + d * T] * y[k * NMAX + k0 + d];
+            S.tmp[k * 64 + c] = s;
+        }
+        __syncthreads();
+        if (tid < w * nrhs) {
+            int c = tid % w, k = tid / w;
+            y[k * NMAX + k0 + c] = S.tmp[k * 64 + c];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < (n - k0 - w) * nrhs; idx += NTB) {
+            int i = k0 + w + idx % (n - k0 - w), k = idx / (n - k0 - w);
+            double s = 0.0;
+            for (int c = 0; c < w; ++c) s += (double)W[i + (long long)(k0 + c) * ld] * y[k * NMAX + k0 + c];
+            y[k * NMAX + i] -= s;
+        }
+        __syncthreads();
+    }
+    // backward: x_blk = Di^T y_blk ; y[above] -= L[blk, above]^T x_blk
+    for (int kb = nbk - 1; kb >= 0; --kb) {
+        const int k0 = kb * T, w = min(T, n - k0);
+        const FT *Di = Dinv + (long long)kb * T * T;
+        if (tid < w * nrhs) {
+            int c = tid % w, k = tid / w;
+            double s = 0.0;
+            for (int d = c; d < w; ++d) s += (double)Di[d + c * T] * y[k * NMAX + k0 + d];
+            S.tmp[k * 64 + c] = s;
+        }
+        __syncthreads();
+        if (tid < w * nrhs) {
+            int c = tid % w, k = tid / w;
+            y[k * NMAX + k0 + c] = S.tmp[k * 64 + c];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < k0 * nrhs; idx += NTB) {
+            int i = idx % k0, k = idx / k0;
+            double s = 0.0;
+            for (int r = 0; r < w; ++r) s += (double)W[(k0 + r) + (long long)i * ld] * y[k * NMAX + k0 + r];
+            y[k * NMAX + i] -= s;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename FT, int T, int RB>
+__global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Smem<FT, T> &S = *reinterpret_cast<Smem<FT, T> *>(smem_raw);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (;;) {
+        if (tid == 0) S.sid = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int q = S.sid;
+        __syncthreads();
+        if (q >= a.nsys) break;
+        const int sidx = a.order[q];
+        const SysDesc sd = a.sys[sidx];
+        const int n = sd.n, nrhs = sd.nrhs, ld = sd.ld;
+        const double *A = a.A + sd.a;
+        const double *B = a.B + sd.b;
+        double *X = a.X + sd.b;
+        FT *W = (FT *)a.W + sd.w;
+        FT *Dinv = (FT *)a.Dinv + sd.d;
+        if (n > PMAX || nrhs > RMAX) {
+            if (tid == 0) a.status[sidx] = -1;
+            continue;
+        }
+        if (tid == 0) S.bad = 0;
+        // 1. working copy (lower triangle)
+        for (long long idx = tid; idx < (long long)n * n; idx += NTB) {
+            int r = (int)(idx % n), c = (int)(idx / n);
+            if (r >= c) W[r + (long long)c * ld] = (FT)A[r + (long long)c * ld];
+        }
+        __syncthreads();
+        // 2. blocked right-looking Cholesky
+        for (int k0 = 0; k0 < n; k0 += T) {
+            const int w = min(T, n - k0), m = n - k0 - w;
+            for (int idx = tid; idx < T * T; idx += NTB) {
+                int r = idx % T, c = idx / T;
+                S.D[c * (T + 1) + r] = (r < w && c < w && r >= c) ? W[(k0 + r) + (long long)(k0 + c) * ld]
+                                                                   : (FT)(r == c ? 1 : 0);
+            }
+            __syncthreads();
+            if (wave == 0) {
+                // factor (lane = row) then invert (lane = column), lanes >= T idle
+                for (int j = 0; j < T; ++j) {
+                    FT v = 0;
+                    if (lane >= j && lane < T) {
+                        v = S.D[j * (T + 1) + lane];
+                        for (int k = 0; k < j; ++k) v -= S.D[k * (T + 1) + lane] * S.D[k * (T + 1) + j];
+                    }
+                    if (lane == j) {
+                        if (!(v > (FT)0)) S.bad = k0 + j + 1;
+                        S.D[j * (T + 1) + j] = sqrt(v);
+                    }
+                    wave_sync();
+                    if (lane > j && lane < T) S.D[j * (T + 1) + lane] = v / S.D[j * (T + 1) + j];
+                    wave_sync();
+                }
+                if (lane < T) {
+                    const int col = lane;
+                    for (int i = 0; i < col; ++i) S.Di[col * (T + 1) + i] = 0;
+                    S.Di[col * (T + 1) + col] = (FT)1 / S.D[col * (T + 1) + col];
+                    for (int i = col + 1; i < T; ++i) {
+                        FT sum = 0;
+                        for (int d = col; d < i; ++d) sum += S.D[d * (T + 1) + i] * S.Di[col * (T + 1) + d];
+                        S.Di[col * (T + 1) + i] = -sum / S.D[i * (T + 1) + i];
+                    }
+                }
+            }
+            __syncthreads();
+            if (S.bad) break;
+            for (int idx = tid; idx < T * T; idx += NTB) {
+                int r = idx % T, c = idx / T;
+                if (r < w && c < w && r >= c) W[(k0 + r) + (long long)(k0 + c) * ld] = S.D[c * (T + 1) + r];
+                Dinv[(long long)(k0 / T) * T * T + idx] = S.Di[c * (T + 1) + r];
+            }
+            // panel: P[r] = W[k0+w+r, k0:k0+w] * Di^T, staged in LDS
+            for (int r = tid; r < m; r += NTB) {
+                FT row[T];
+#pragma unroll
+                for (int c = 0; c < T; ++c) row[c] = (c < w) ? W[(k0 + w + r) + (long long)(k0 + c) * ld] : (FT)0;
+#pragma unroll
+                for (int c = 0; c < T; ++c) {
+                    FT s = 0;
+#pragma unroll
+                    for (int d = 0; d <= c; ++d) s += row[d] * S.Di[d * (T + 1) + c];  // Di(c, d)
+                    S.u.P[r * (T + 1) + c] = s;
+                    if (c < w) W[(k0 + w + r) + (long long)(k0 + c) * ld] = s;
+                }
+            }
+            __syncthreads();
+            // trailing lower update with RB x RB register blocks
+            const int nbr = (m + RB - 1) / RB;
+            const int total = nbr * (nbr + 1) / 2;
+            for (int idx = tid; idx < total; idx += NTB) {
+                int bi, bj;
+                tri_lower_decode(idx, bi, bj);
+                FT acc[RB][RB];
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+#pragma unroll
+                    for (int j = 0; j < RB; ++j) acc[i][j] = 0;
+                for (int k = 0; k < w; ++k) {
+                    FT pa[RB], pb[RB];
+#pragma unroll
+                    for (int i = 0; i < RB; ++i) {
+                        int r = bi * RB + i;
+                        pa[i] = r < m ? S.u.P[r * (T + 1) + k] : (FT)0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < RB; ++j) {
+                        int c = bj * RB + j;
+                        pb[j] = c < m ? S.u.P[c * (T + 1) + k] : (FT)0;
+                    }
+#pragma unroll
+                    for (int i = 0; i < RB; ++i)
+#pragma unroll
+                        for (int j = 0; j < RB; ++j) acc[i][j] += pa[i] * pb[j];
+                }
+#pragma unroll
+                for (int j = 0; j < RB; ++j) {
+                    int c = bj * RB + j;
+                    if (c >= m) continue;
+                    FT *col = W + (long long)(k0 + w + c) * ld + (k0 + w);
+#pragma unroll
+                    for (int i = 0; i < RB; ++i) {
+                        int r = bi * RB + i;
+                        if (r < m && r >= c) col[r] -= acc[i][j];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (S.bad) {
+            if (tid == 0) {
+                a.status[sidx] = S.bad;
+                a.berr[sidx] = INFINITY;
+                a.iters[sidx] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        // 3./4. solve and refine (x in S.u.v.x, work vector in S.u.v.y)
+        double *y = S.u.v.y, *x = S.u.v.x;
+        for (int idx = tid; idx < n * nrhs; idx += NTB) {
+            int i = idx % n, k = idx / n;
+            y[k * NMAX + i] = B[i + (long long)k * ld];
+            x[k * NMAX + i] = 0.0;
+        }
+        __syncthreads();
+        int it = 0;
+        double eta = INFINITY, prev = INFINITY;
+        for (;;) {
+            tri_solve<FT, T>(S, W, Dinv, n, ld, nrhs);
+            for (int idx = tid; idx < n * nrhs; idx += NTB) {
+                int i = idx % n, k = idx / n;
+                x[k * NMAX + i] += y[k * NMAX + i];
+            }
+            __syncthreads();
+            // r = b - A x (FP64), row sums of |A|, norms
+            double rmax[RMAX], xmax[RMAX], bmax[RMAX];
+#pragma unroll
+            for (int k = 0; k < RMAX; ++k) rmax[k] = xmax[k] = bmax[k] = 0.0;
+            double amax = 0.0;
+            for (int i = tid; i < n; i += NTB) {
+                double rs[RMAX], as = 0.0;
+#pragma unroll
+                for (int k = 0; k < RMAX; ++k) rs[k] = (k < nrhs) ? B[i + (long long)k * ld] : 0.0;
+                for (int j = 0; j < n; ++j) {
+                    const double aij = A[i + (long long)j * ld];
+                    as += fabs(aij);
+#pragma unroll
+                    for (int k = 0; k < RMAX; ++k)
+                        if (k < nrhs) rs[k] -= aij * x[k * NMAX + j];
+                }
+                amax = fmax(amax, as);
+#pragma unroll
+                for (int k = 0; k < RMAX; ++k) {
+                    if (k >= nrhs) break;
+                    y[k * NMAX + i] = rs[k];
+                    rmax[k] = fmax(rmax[k], fabs(rs[k]));
+                    xmax[k] = fmax(xmax[k], fabs(x[k * NMAX + i]));
+                    bmax[k] = fmax(bmax[k], fabs(B[i + (long long)k * ld]));
+                }
+            }
+            // block max-reductions: amax, then per rhs the three norms
+            double e = 0.0;
+            S.red[tid] = amax;
+            __syncthreads();
+            for (int s2 = NTB / 2; s2 > 0; s2 >>= 1) {
+                if (tid < s2) S.red[tid] = fmax(S.red[tid], S.red[tid + s2]);
+                __syncthreads();
+            }
+            const double An = S.red[0];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < RMAX; ++k) {
+                if (k >= nrhs) break;
+                double vals[3] = {rmax[k], xmax[k], bmax[k]};
+                double out[3];
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    S.red[tid] = vals[v];
+                    __syncthreads();
+                    for (int s2 = NTB / 2; s2 > 0; s2 >>= 1) {
+                        if (tid < s2) S.red[tid] = fmax(S.red[tid], S.red[tid + s2]);
+                        __syncthreads();
+                    }
+                    out[v] = S.red[0];
+                    __syncthreads();
+                }
+                e = fmax(e, out[0] / (An * out[1] + out[2]));
+            }
+            prev = eta;
+            eta = e;
+            if (eta <= a.tol) break;
+            if (it >= a.max_iter || !(eta < 0.5 * prev)) break;
+            ++it;  // y holds r: next pass solves W d = r and adds d to x
+        }
+        for (int idx = tid; idx < n * nrhs; idx += NTB) {
+            int i = idx % n, k = idx / n;
+            X[i + (long long)k * ld] = x[k * NMAX + i];
+        }
+        if (tid == 0) {
+            a.berr[sidx] = eta;
+            a.iters[sidx] = it;
+            a.status[sidx] = (eta <= a.tol) ? 0 : -2;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+__global__ void k_add_diag(double *A, const SysDesc *sys, int nsys, double mu) {
+    const int s = blockIdx.x;
+    if (s >= nsys) return;
+    const SysDesc sd = sys[s];
+    for (int i = threadIdx.x; i < sd.n; i += blockDim.x) A[sd.a + i + (long long)i * sd.ld] += mu;
+}
+
+template <typename FT, int T, int RB>
+size_t smem_bytes() {
+    return sizeof(Smem<FT, T>);
+}
+
+typedef void (*BatchKernel)(BatchArgs);
+
+struct Variant {
+    BatchKernel k;
+    size_t smem;
+    int T, RB, fp32;
+};
+
+Variant variants_fp64[] = {
+    {k_batched<double, 16, 4>, smem_bytes<double, 16, 4>(), 16, 4, 0},
+    {k_batched<double, 16, 8>, smem_bytes<double, 16, 8>(), 16, 8, 0},
+    {k_batched<double, 32, 4>, 0, 32, 4, 0},  // needs PMAX*33*8 > LDS: disabled (smem 0)
+};
+Variant variants_fp32[] = {
+    {k_batched<float, 16, 4>, smem_bytes<float, 16, 4>(), 16, 4, 1},
+    {k_batched<float, 16, 8>, smem_bytes<float, 16, 8>(), 16, 8, 1},
+    {k_batched<float, 32, 4>, smem_bytes<float, 32, 4>(), 32, 4, 1},
+    {k_batched<float, 32, 8>, smem_bytes<float, 32, 8>(), 32, 8, 1},
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+}  // namespace ooc
+
+using namespace ooc;
+
+extern "C" long long oocdag_batch_layout(int nsys, const int *n, const int *nrhs, long long *a_off,
+                                         long long *b_off) {
+    // A blocks n x n and B/X blocks n x nrhs with leading dimension
+    // ld = n rounded up to even, each 32-double aligned;
+    // returns the total doubles of the A arena, B arena size in b_off[nsys]
+    long long a = 0, b = 0;
+    for (int s = 0; s < nsys; ++s) {
+        const long long ld = n[s] + (n[s] & 1);
+        a_off[s] = a;
+        b_off[s] = b;
+        a += (ld * n[s] + 31) / 32 * 32;
+        b += (ld * nrhs[s] + 31) / 32 * 32;
+    }
+    b_off[nsys] = b;
+    return a;
+}
+
+extern "C" int oocdag_batch_generate(int nsys, const int *n, const int *nrhs, const long long *a_off,
+                                     const long long *b_off, double *dA, double *dB, double mu,
+                                     unsigned long long seed) {
+    // A_s = (1/n) G_s G_s^T + mu I (full symmetric), B_s ~ N(0,1)
+    int nmax = 0;
+    for (int s = 0; s < nsys; ++s) nmax = std::max(nmax, n[s]);
+    double *G = nullptr;
+    if (hipMalloc(&G, (size_t)nmax * nmax * sizeof(double)) != hipSuccess) return OOCDAG_ENOMEM;
+    std::vector<SysDesc> sd(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        sd[s].n = n[s];
+        sd[s].nrhs = nrhs[s];
+        sd[s].ld = n[s] + (n[s] & 1);
+        sd[s].a = a_off[s];
+        sd[s].b = b_off[s];
+    }
+    int rc = OOCDAG_OK;
+    for (int s = 0; s < nsys && rc == OOCDAG_OK; ++s) {
+        const int m = n[s], ld = m + (m & 1);
+        sd[s].ld = ld;
+        if (oocdag_normal_fill(G, ld, m, ld, seed, 1000000ull + s) != OOCDAG_OK) rc = OOCDAG_EHIP;
+        // zero the padding row of G so G G^T has zeros there
+        if (rc == OOCDAG_OK && (m & 1))
+            (void)hipMemset2D(G + m, ld * sizeof(double), 0, sizeof(double), m);
+        Gemm g;
+        g.A = plain(G, ld);
+        g.B = g.A;
+        g.C = plain(dA + a_off[s], ld);
+        g.M = m; g.N = m; g.K = m;
+        g.alpha = 1.0 / m; g.beta = 0.0; g.lower = 0;
+        if (rc == OOCDAG_OK && launch_gemm(g, GEMM_GENERAL, 0) != hipSuccess) rc = OOCDAG_EHIP;
+        if (rc == OOCDAG_OK &&
+            oocdag_normal_fill(dB + b_off[s], m, nrhs[s], ld, seed, 2000000ull + s) != OOCDAG_OK)
+            rc = OOCDAG_EHIP;
+    }
+    SysDesc *dsd = nullptr;
+    if (rc == OOCDAG_OK && hipMalloc(&dsd, nsys * sizeof(SysDesc)) == hipSuccess) {
+        (void)hipMemcpy(dsd, sd.data(), nsys * sizeof(SysDesc), hipMemcpyHostToDevice);
+        hipLaunchKernelGGL(k_add_diag, dim3(nsys), dim3(256), 0, 0, dA, dsd, nsys, mu);
+        if (hipDeviceSynchronize() != hipSuccess) rc = OOCDAG_EHIP;
+        (void)hipFree(dsd);
+    }
+    (void)hipFree(G);
+    return rc;
+}
+
+extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long long *a_off,
+                                    const long long *b_off, const double *dA, const double *dB,
+                                    double *dX, int precision, double tol, int max_iter, int reps,
+                                    double *berr, int *iters, int *status, oocdag_batch_stats *st) {
+    oocdag_batch_stats dummy;
+    if (!st) st = &dummy;
+    *st = oocdag_batch_stats{};
+    if (nsys <= 0 || !n || !nrhs || !dA || !dB || !dX || (precision != 0 && precision != 1))
+        return OOCDAG_EARG;
+    for (int s = 0; s < nsys; ++s)
+        if (n[s] < 1 || n[s] > PMAX || nrhs[s] < 1 || nrhs[s] > RMAX) return OOCDAG_EARG;
+    // descriptors and workspace offsets (FT elements), largest systems first
+    std::vector<SysDesc> sd(nsys);
+    const int Tmax = 32;
+    long long w = 0, d = 0;
+    for (int s = 0; s < nsys; ++s) {
+        sd[s].n = n[s];
+        sd[s].nrhs = nrhs[s];
+        sd[s].ld = n[s] + (n[s] & 1);
+        sd[s].a = a_off[s];
+        sd[s].b = b_off[s];
+        sd[s].w = w;
+        sd[s].d = d;
+        w += ((long long)sd[s].ld * n[s] + 31) / 32 * 32;
+        d += (long long)((n[s] + 15) / 16) * Tmax * Tmax;
+    }
+    std::vector<int> order(nsys);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+    const size_t fts = precision == 0 ? 8 : 4;
+    SysDesc *dsd = nullptr;
+    int *dorder = nullptr, *dint = nullptr, *dcounter = nullptr;
+    double *dberr = nullptr;
+    void *dW = nullptr, *dD = nullptr;
+    struct F {
+        void **p[7];
+        ~F() {
+            for (auto q : p)
+                if (*q) (void)hipFree(*q);
+        }
+    } fr{{(void **)&dsd, (void **)&dorder, (void **)&dint, (void **)&dcounter, (void **)&dberr, &dW,
+          &dD}};
+    if (hipMalloc(&dsd, nsys * sizeof(SysDesc)) != hipSuccess ||
+        hipMalloc(&dorder, nsys * sizeof(int)) != hipSuccess ||
+        hipMalloc(&dint, 2 * (size_t)nsys * sizeof(int)) != hipSuccess ||
+        hipMalloc(&dcounter, sizeof(int)) != hipSuccess ||
+        hipMalloc(&dberr, nsys * sizeof(double)) != hipSuccess || hipMalloc(&dW, w * fts) != hipSuccess ||
+        hipMalloc(&dD, d * fts) != hipSuccess)
+        return OOCDAG_ENOMEM;
+    if (hipMemcpy(dsd, sd.data(), nsys * sizeof(SysDesc), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dorder, order.data(), nsys * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return OOCDAG_EHIP;
+    int ncu = 256;
+    {
+        int dev = 0;
+        hipDeviceProp_t pr;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess)
+            ncu = pr.multiProcessorCount;
+    }
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return OOCDAG_EHIP;
+    struct SS {
+        hipStream_t s;
+        ~SS() { (void)hipStreamDestroy(s); }
+    } ss{s};
+
+    auto make_args = [&](const int *ord, int cnt) {
+        BatchArgs a;
+        a.sys = dsd;
+        a.order = ord;
+        a.nsys = cnt;
+        a.A = dA;
+        a.B = dB;
+        a.X = dX;
+        a.W = dW;
+        a.Dinv = dD;
+        a.berr = dberr;
+        a.iters = dint;
+        a.status = dint + nsys;
+        a.counter = dcounter;
+        a.tol = tol;
+        a.max_iter = max_iter;
+        return a;
+    };
+    auto run = [&](const Variant &v, const int *ord, int cnt) -> hipError_t {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, dcounter, 0);
+        BatchArgs a = make_args(ord, cnt);
+        hipLaunchKernelGGL(v.k, dim3(ncu), dim3(NTB), v.smem, s, a);
+        return hipGetLastError();
+    };
+    // --- autotune on a sample: every 16th system of the sorted order
+    const double tt0 = now_s();
+    std::vector<int> sample;
+    for (int i = 0; i < nsys; i += std::max(1, nsys / 256)) sample.push_back(order[i]);
+    int *dsample = nullptr;
+    if (hipMalloc(&dsample, sample.size() * sizeof(int)) != hipSuccess) return OOCDAG_ENOMEM;
+    (void)hipMemcpy(dsample, sample.data(), sample.size() * sizeof(int), hipMemcpyHostToDevice);
+    Variant *vs = precision == 0 ? variants_fp64 : variants_fp32;
+    const int nv = precision == 0 ? 3 : 4;
+    int best = -1;
+    double best_t = 1e30;
+    for (int i = 0; i < nv; ++i) {
+        if (vs[i].smem == 0 || vs[i].smem > 160 * 1024) continue;
+        if (hipFuncSetAttribute((const void *)vs[i].k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vs[i].smem) != hipSuccess)
+            continue;
+        double tbest = 1e30;
+        for (int rep = 0; rep < 2; ++rep) {
+            double t0 = now_s();
+            if (run(vs[i], dsample, (int)sample.size()) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                tbest = 1e30;
+                break;
+            }
+            tbest = std::min(tbest, now_s() - t0);
+        }
+        if (tbest < best_t) {
+            best_t = tbest;
+            best = i;
+        }
+    }
+    (void)hipFree(dsample);
+    if (best < 0) return OOCDAG_EHIP;
+    st->tune_seconds = now_s() - tt0;
+    st->variant_T = vs[best].T;
+    st->variant_RB = vs[best].RB;
+    // --- capture the full batch (queue reset + persistent kernel) in a graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return OOCDAG_EHIP;
+    hipError_t ce = run(vs[best], dorder, nsys);
+    if (hipStreamEndCapture(s, &graph) != hipSuccess || ce != hipSuccess) return OOCDAG_EHIP;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return OOCDAG_EHIP;
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    double tsum = 0;
+    int rc = OOCDAG_OK;
+    for (int r = 0; r < std::max(1, reps); ++r) {
+        (void)hipEventRecord(e0, s);
+        if (hipGraphLaunch(exec, s) != hipSuccess) rc = OOCDAG_EHIP;
+        (void)hipEventRecord(e1, s);
+        if (hipStreamSynchronize(s) != hipSuccess) rc = OOCDAG_EHIP;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        tsum += ms * 1e-3;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    st->seconds = tsum / std::max(1, reps);
+    st->systems_per_second = nsys / st->seconds;
+    std::vector<double> hb(nsys);
+    std::vector<int> hi(2 * (size_t)nsys);
+    if (hipMemcpy(hb.data(), dberr, nsys * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hi.data(), dint, 2 * (size_t)nsys * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return OOCDAG_EHIP;
+    for (int i = 0; i < nsys; ++i) {
+        if (berr) berr[i] = hb[i];
+        if (iters) iters[i] = hi[i];
+        if (status) status[i] = hi[nsys + i];
+        if (hi[nsys + i] != 0) st->failed += 1;
+        else st->max_backward_error = std::max(st->max_backward_error, hb[i]);
+    }
+    return OOCDAG_OK;
+}
