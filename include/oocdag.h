@@ -189,6 +189,47 @@ int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long lon
                          int *iters, int *status, oocdag_batch_stats *stats);
 
 /* ---------------------------------------------------------------------------
+ * Continuous-batching solve server (no reference counterpart).  Requests are
+ * served by `slots` worker threads with their own HIP streams, admitted
+ * against an HBM byte budget; small systems (n <= 1024, nrhs <= 8) that are
+ * pending together run in one launch of the batched kernel; larger ones are
+ * factored in mixed precision + FP64 refinement and their factor is cached by
+ * matrix_id (least recently used evicted under the budget); a request whose
+ * working set exceeds the whole budget takes the out-of-core FP64 route.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    long long matrix_id;         /* same id = same matrix (its factor may be cached) */
+    int n, nrhs;                 /* nrhs in [1, 64] */
+    double mu;                   /* A = (1/n) G G^T + mu I, G from `seed` */
+    unsigned long long seed;
+    unsigned long long rhs_seed; /* B ~ N(0,1) from this seed */
+    int precision;               /* factor precision: 0 FP64, 1 FP32, 2 BF16 (+ refinement) */
+    double tol;                  /* target normwise backward error */
+} oocdag_request;
+
+typedef struct {
+    int status;                  /* 0 ok, > 0 not positive definite, < 0 OOCDAG_E* */
+    int route;                   /* 0 batched, 1 HBM mixed precision, 2 out-of-core */
+    int cache_hit;
+    int batch_size;              /* requests in the same launch (route 0) */
+    int precision_used, fell_back, iterations;
+    double backward_error;
+    double queue_seconds, service_seconds, latency_seconds;
+    double x_checksum;           /* sum of the solution's entries */
+} oocdag_result;
+
+typedef struct oocdag_server oocdag_server;
+oocdag_server *oocdag_server_create(size_t hbm_budget_bytes, int slots);
+long long oocdag_server_submit(oocdag_server *srv, const oocdag_request *rq); /* ticket > 0 */
+/* Blocks until the ticket is done; x (optional) receives the n x nrhs
+ * solution, column-major, up to x_len doubles. */
+int oocdag_server_wait(oocdag_server *srv, long long ticket, oocdag_result *res, double *x,
+                       size_t x_len);
+int oocdag_server_stats(oocdag_server *srv, size_t *used_bytes, int *cached, int *pending,
+                        int *running);
+int oocdag_server_destroy(oocdag_server *srv);
+
+/* ---------------------------------------------------------------------------
  * Tile kernels (device pointers, column-major, explicit leading dimensions).
  * lda/ldb/ldc must be even and pointers 16-byte aligned.
  * Replaces: cublasDgemm_v2(N,T)   oocquark.cpp:222  (CORE_incore_dgemm)

@@ -107,6 +107,31 @@ SIGNATURES.update({
                                        _ip, C.POINTER(BatchStats)]),
 })
 
+class Request(C.Structure):
+    _fields_ = [("matrix_id", C.c_longlong), ("n", C.c_int), ("nrhs", C.c_int), ("mu", C.c_double),
+                ("seed", C.c_ulonglong), ("rhs_seed", C.c_ulonglong), ("precision", C.c_int),
+                ("tol", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("route", C.c_int), ("cache_hit", C.c_int),
+                ("batch_size", C.c_int), ("precision_used", C.c_int), ("fell_back", C.c_int),
+                ("iterations", C.c_int), ("backward_error", C.c_double),
+                ("queue_seconds", C.c_double), ("service_seconds", C.c_double),
+                ("latency_seconds", C.c_double), ("x_checksum", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SIGNATURES.update({
+    "oocdag_server_create": (C.c_void_p, [C.c_size_t, C.c_int]),
+    "oocdag_server_submit": (C.c_longlong, [C.c_void_p, C.POINTER(Request)]),
+    "oocdag_server_wait": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(Result), _dp, C.c_size_t]),
+    "oocdag_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), _ip, _ip, _ip]),
+    "oocdag_server_destroy": (C.c_int, [C.c_void_p]),
+})
+
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
 
 
@@ -362,6 +387,50 @@ class Batch:
             _check(lib().oocdag_memcpy_d2h(out.ctypes.data, buf.ptr + int(off) * 8, out.nbytes), "d2h")
             return out.reshape(ld, cols, order="F")[:n]
         return get(self.A, self.a_off[s], n), get(self.B, self.b_off[s], r), get(self.X, self.b_off[s], r)
+
+
+class Server:
+    """Continuous-batching SPD solve server (oocdag_server_*)."""
+
+    def __init__(self, hbm_budget_bytes: int, slots: int = 3):
+        self.h = lib().oocdag_server_create(int(hbm_budget_bytes), int(slots))
+        if not self.h:
+            raise OocdagError("oocdag_server_create failed (no GPU or bad arguments)")
+        self.shapes = {}
+
+    def submit(self, matrix_id, n, nrhs, mu, seed, rhs_seed, precision="fp32",
+               tol=8 * 2.0 ** -53) -> int:
+        rq = Request(matrix_id, n, nrhs, mu, seed, rhs_seed, PRECISIONS[precision], tol)
+        t = lib().oocdag_server_submit(self.h, C.byref(rq))
+        if t <= 0:
+            raise OocdagError(f"submit failed ({t})")
+        self.shapes[t] = (n, nrhs)
+        return t
+
+    def wait(self, ticket, want_x=False):
+        res = Result()
+        n, r = self.shapes.pop(ticket)
+        x = np.zeros((n, r), order="F") if want_x else None
+        _check(lib().oocdag_server_wait(self.h, ticket, C.byref(res),
+                                        x.ctypes.data_as(_dp) if want_x else None,
+                                        n * r if want_x else 0), "server_wait")
+        return res.as_dict(), x
+
+    def stats(self):
+        u, c, p, r = C.c_size_t(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        lib().oocdag_server_stats(self.h, C.byref(u), C.byref(c), C.byref(p), C.byref(r))
+        return {"used_bytes": u.value, "cached": c.value, "pending": p.value, "running": r.value}
+
+    def close(self):
+        if self.h:
+            lib().oocdag_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def tiles_from_host(buf: DeviceBuffer, A: np.ndarray, nb: int):

@@ -19,7 +19,7 @@ CLI = os.path.join(HERE, "oocdag")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ROCM_LLVM_LIB = "/opt/rocm/lib/llvm/lib"
 
-SOURCES = ["gemm.hip", "gemm_lp.hip", "factor.hip", "solve.hip", "mixed.hip", "batched.hip", "driver.hip"]
+SOURCES = ["gemm.hip", "gemm_lp.hip", "factor.hip", "solve.hip", "mixed.hip", "batched.hip", "serve.hip", "driver.hip"]
 
 HIPFLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",

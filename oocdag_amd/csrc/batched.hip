@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -378,6 +379,10 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
+__global__ void k_add_diag_one(double *A, int n, int ld, double mu) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) A[i + (long long)i * ld] += mu;
+}
+
 __global__ void k_add_diag(double *A, const SysDesc *sys, int nsys, double mu) {
     const int s = blockIdx.x;
     if (s >= nsys) return;
@@ -483,6 +488,39 @@ extern "C" int oocdag_batch_generate(int nsys, const int *n, const int *nrhs, co
     return rc;
 }
 
+namespace {
+std::mutex g_tune_mu;
+int g_tuned[2] = {-1, -1};
+}  // namespace
+
+namespace ooc {
+// Generates one A = (1/n) G G^T + mu I (full, ld even) with G from the
+// counter-based normal generator keyed by (seed, stream_id); Gtmp holds at
+// least ld * n doubles.
+hipError_t gen_small_gram(double *A, int n, int ld, double mu, unsigned long long seed,
+                          unsigned long long stream_id, double *Gtmp, hipStream_t s) {
+    hipError_t e = normal_block(Gtmp, n, n, ld, seed, stream_id, s);
+    if (e != hipSuccess) return e;
+    if (ld != n && (e = hipMemset2DAsync(Gtmp + n, ld * sizeof(double), 0, sizeof(double), n, s)) != hipSuccess)
+        return e;
+    Gemm g;
+    g.A = plain(Gtmp, ld);
+    g.B = g.A;
+    g.C = plain(A, ld);
+    g.M = n; g.N = n; g.K = n;
+    g.alpha = 1.0 / n; g.beta = 0.0; g.lower = 0;
+    if ((e = launch_gemm(g, GEMM_GENERAL, s)) != hipSuccess) return e;
+    SysDesc sd;
+    sd.n = n;
+    sd.ld = ld;
+    sd.a = 0;
+    // add mu on the diagonal through a one-system descriptor passed by value
+    hipLaunchKernelGGL(k_add_diag_one, dim3(1), dim3(256), 0, s, A, n, ld, mu);
+    (void)sd;
+    return hipGetLastError();
+}
+}  // namespace ooc
+
 extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long long *a_off,
                                     const long long *b_off, const double *dA, const double *dB,
                                     double *dX, int precision, double tol, int max_iter, int reps,
@@ -513,27 +551,38 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
     const size_t fts = precision == 0 ? 8 : 4;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return OOCDAG_EHIP;
+    struct SS {
+        hipStream_t s;
+        ~SS() {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    } ss{s};
     SysDesc *dsd = nullptr;
     int *dorder = nullptr, *dint = nullptr, *dcounter = nullptr;
     double *dberr = nullptr;
     void *dW = nullptr, *dD = nullptr;
     struct F {
+        hipStream_t s;
         void **p[7];
         ~F() {
             for (auto q : p)
-                if (*q) (void)hipFree(*q);
+                if (*q) (void)hipFreeAsync(*q, s);
         }
-    } fr{{(void **)&dsd, (void **)&dorder, (void **)&dint, (void **)&dcounter, (void **)&dberr, &dW,
+    } fr{s, {(void **)&dsd, (void **)&dorder, (void **)&dint, (void **)&dcounter, (void **)&dberr, &dW,
           &dD}};
-    if (hipMalloc(&dsd, nsys * sizeof(SysDesc)) != hipSuccess ||
-        hipMalloc(&dorder, nsys * sizeof(int)) != hipSuccess ||
-        hipMalloc(&dint, 2 * (size_t)nsys * sizeof(int)) != hipSuccess ||
-        hipMalloc(&dcounter, sizeof(int)) != hipSuccess ||
-        hipMalloc(&dberr, nsys * sizeof(double)) != hipSuccess || hipMalloc(&dW, w * fts) != hipSuccess ||
-        hipMalloc(&dD, d * fts) != hipSuccess)
+    if (hipMallocAsync((void **)&dsd, nsys * sizeof(SysDesc), s) != hipSuccess ||
+        hipMallocAsync((void **)&dorder, nsys * sizeof(int), s) != hipSuccess ||
+        hipMallocAsync((void **)&dint, 2 * (size_t)nsys * sizeof(int), s) != hipSuccess ||
+        hipMallocAsync((void **)&dcounter, sizeof(int), s) != hipSuccess ||
+        hipMallocAsync((void **)&dberr, nsys * sizeof(double), s) != hipSuccess ||
+        hipMallocAsync(&dW, w * fts, s) != hipSuccess || hipMallocAsync(&dD, d * fts, s) != hipSuccess)
         return OOCDAG_ENOMEM;
-    if (hipMemcpy(dsd, sd.data(), nsys * sizeof(SysDesc), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dorder, order.data(), nsys * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(dsd, sd.data(), nsys * sizeof(SysDesc), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dorder, order.data(), nsys * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
         return OOCDAG_EHIP;
     int ncu = 256;
     {
@@ -542,13 +591,6 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess)
             ncu = pr.multiProcessorCount;
     }
-    hipStream_t s = nullptr;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return OOCDAG_EHIP;
-    struct SS {
-        hipStream_t s;
-        ~SS() { (void)hipStreamDestroy(s); }
-    } ss{s};
-
     auto make_args = [&](const int *ord, int cnt) {
         BatchArgs a;
         a.sys = dsd;
@@ -573,22 +615,29 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         hipLaunchKernelGGL(v.k, dim3(ncu), dim3(NTB), v.smem, s, a);
         return hipGetLastError();
     };
-    // --- autotune on a sample: every 16th system of the sorted order
+    Variant *vs = precision == 0 ? variants_fp64 : variants_fp32;
+    const int nv = precision == 0 ? 3 : 4;
+    for (int i = 0; i < nv; ++i)
+        if (vs[i].smem > 0 && vs[i].smem <= 160 * 1024)
+            (void)hipFuncSetAttribute((const void *)vs[i].k,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)vs[i].smem);
+    int best = -1;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (reps <= 0) best = g_tuned[precision];
+    }
+    // --- autotune on a sample of the sorted order (skipped when a tuned
+    // variant is cached and the caller asked for a plain run)
     const double tt0 = now_s();
+    if (best < 0) {
     std::vector<int> sample;
     for (int i = 0; i < nsys; i += std::max(1, nsys / 256)) sample.push_back(order[i]);
     int *dsample = nullptr;
-    if (hipMalloc(&dsample, sample.size() * sizeof(int)) != hipSuccess) return OOCDAG_ENOMEM;
-    (void)hipMemcpy(dsample, sample.data(), sample.size() * sizeof(int), hipMemcpyHostToDevice);
-    Variant *vs = precision == 0 ? variants_fp64 : variants_fp32;
-    const int nv = precision == 0 ? 3 : 4;
-    int best = -1;
+    if (hipMallocAsync((void **)&dsample, sample.size() * sizeof(int), s) != hipSuccess) return OOCDAG_ENOMEM;
+    (void)hipMemcpyAsync(dsample, sample.data(), sample.size() * sizeof(int), hipMemcpyHostToDevice, s);
     double best_t = 1e30;
     for (int i = 0; i < nv; ++i) {
         if (vs[i].smem == 0 || vs[i].smem > 160 * 1024) continue;
-        if (hipFuncSetAttribute((const void *)vs[i].k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)vs[i].smem) != hipSuccess)
-            continue;
         double tbest = 1e30;
         for (int rep = 0; rep < 2; ++rep) {
             double t0 = now_s();
@@ -604,11 +653,24 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
             best = i;
         }
     }
-    (void)hipFree(dsample);
+    (void)hipFreeAsync(dsample, s);
     if (best < 0) return OOCDAG_EHIP;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[precision] = best;
+    }
+    }
     st->tune_seconds = now_s() - tt0;
     st->variant_T = vs[best].T;
     st->variant_RB = vs[best].RB;
+    int rc = OOCDAG_OK;
+    if (reps <= 0) {
+        // one direct launch on this call's stream (used by the server)
+        const double t0 = now_s();
+        if (run(vs[best], dorder, nsys) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            return OOCDAG_EHIP;
+        st->seconds = now_s() - t0;
+    } else {
     // --- capture the full batch (queue reset + persistent kernel) in a graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -623,7 +685,6 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     double tsum = 0;
-    int rc = OOCDAG_OK;
     for (int r = 0; r < std::max(1, reps); ++r) {
         (void)hipEventRecord(e0, s);
         if (hipGraphLaunch(exec, s) != hipSuccess) rc = OOCDAG_EHIP;
@@ -639,11 +700,13 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     (void)hipGraphDestroy(graph);
     if (rc) return rc;
     st->seconds = tsum / std::max(1, reps);
+    }
     st->systems_per_second = nsys / st->seconds;
     std::vector<double> hb(nsys);
     std::vector<int> hi(2 * (size_t)nsys);
-    if (hipMemcpy(hb.data(), dberr, nsys * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(hi.data(), dint, 2 * (size_t)nsys * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpyAsync(hb.data(), dberr, nsys * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(hi.data(), dint, 2 * (size_t)nsys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
         return OOCDAG_EHIP;
     for (int i = 0; i < nsys; ++i) {
         if (berr) berr[i] = hb[i];

@@ -14,6 +14,7 @@
 // stalls (eta not halving) or runs out of iterations, falls back to the
 // plain FP64 factorization.
 #include "ooc_internal.h"
+#include "../../include/oocdag.h"
 
 #include <algorithm>
 #include <chrono>
@@ -129,7 +130,7 @@ hipError_t gram_tiles(double *dY, int N, int NB, double mu, unsigned long long s
     const int bb = (N + NB - 1) / NB;
     const long long Np = (long long)bb * NB;
     double *G = nullptr;
-    hipError_t e = hipMalloc(&G, (size_t)(Np * Np) * sizeof(double));
+    hipError_t e = hipMallocAsync((void **)&G, (size_t)(Np * Np) * sizeof(double), s);
     if (e != hipSuccess) return e;
     int blocks = 65536;
     hipLaunchKernelGGL(k_fill_normal, dim3(blocks), dim3(256), 0, s, G, Np, Np, Np, seed, 1ull,
@@ -151,8 +152,8 @@ hipError_t gram_tiles(double *dY, int N, int NB, double mu, unsigned long long s
         hipLaunchKernelGGL(k_finish_diag, dim3(64, bb), dim3(256), 0, s, dY, N, NB, bb, mu);
         e = hipGetLastError();
     }
+    (void)hipFreeAsync(G, s);
     hipError_t e2 = hipStreamSynchronize(s);
-    (void)hipFree(G);
     return e != hipSuccess ? e : e2;
 }
 
@@ -340,23 +341,207 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
 
 }  // namespace ooc
 
+namespace ooc {
+
+// ---------------------------------------------------------------------------
+// Reusable pieces: a resident factorization (kept by the server's factor
+// cache) and the refinement solve against it.
+// ---------------------------------------------------------------------------
+
+namespace {
+template <class T>
+hipError_t salloc(T **p, size_t bytes, hipStream_t s) {
+    return hipMallocAsync((void **)p, bytes, s);
+}
+}  // namespace
+
+size_t mp_factor_bytes(int N, int NB) {
+    const size_t bb = (size_t)((N + NB - 1) / NB);
+    const size_t tiles = bb * (bb + 1) / 2 * NB * NB;
+    return (tiles + bb * NB * NB + bb * NB * 64) * 8 + 2 * bb * NB * NB * 4;
+}
+
+void mp_release(MpFactor &f, hipStream_t s) {
+    if (f.W) (void)hipFreeAsync(f.W, s);
+    if (f.Z) (void)hipFreeAsync(f.Z, s);
+    if (f.dinv) (void)hipFreeAsync(f.dinv, s);
+    f.W = f.Z = f.dinv = nullptr;
+}
+
+int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipStream_t s0,
+              hipStream_t s1) {
+    const double t0 = now();
+    const int bb = (N + NB - 1) / NB;
+    const long long Np = (long long)bb * NB;
+    const size_t tiles = (size_t)bb * (bb + 1) / 2 * NB * NB;
+    f.A = dA;
+    f.N = N;
+    f.NB = NB;
+    f.precision = precision;
+    f.info = 0;
+    f.bytes = mp_factor_bytes(N, NB);
+    void *lp = nullptr;
+    float *scale = nullptr;
+    int *info = nullptr;
+    double *anorm = nullptr;
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    if (!f.W && (!ok(salloc(&f.W, tiles * 8, s0)) || !ok(salloc(&f.Z, (size_t)bb * NB * NB * 8, s0)) ||
+                 !ok(salloc(&f.dinv, (size_t)Np * 64 * 8, s0))))
+        return OOCDAG_ENOMEM;
+    if (!ok(salloc(&lp, (size_t)2 * bb * NB * NB * 4, s0)) || !ok(salloc(&scale, (size_t)2 * bb * 4, s0)) ||
+        !ok(salloc(&info, sizeof(int), s0)) || !ok(salloc(&anorm, (size_t)Np * 8, s0)))
+        return OOCDAG_ENOMEM;
+    int rc = OOCDAG_OK;
+    do {
+        // ||A||_inf
+        if (!ok(hipMemsetAsync(anorm, 0, (size_t)Np * 8, s0))) { rc = OOCDAG_EHIP; break; }
+        hipLaunchKernelGGL(k_tile_rowabs, dim3((NB + 255) / 256, bb, bb), dim3(256), 0, s0, dA, NB,
+                           bb, anorm);
+        std::vector<double> hrow(Np);
+        int big = INT_MAX, hinfo = INT_MAX;
+        if (!ok(hipMemcpyAsync(hrow.data(), anorm, Np * 8, hipMemcpyDeviceToHost, s0)) ||
+            !ok(hipMemcpyAsync(f.W, dA, tiles * 8, hipMemcpyDeviceToDevice, s0)) ||
+            !ok(hipMemcpyAsync(info, &big, sizeof(int), hipMemcpyHostToDevice, s0))) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        int r = factor_mixed(f.W, N, NB, precision, f.dinv, info, lp, scale, s0, s1);
+        if (r) { rc = OOCDAG_EHIP; break; }
+        if (!ok(tile_inverses(f.W, N, NB, f.dinv, f.Z, s0)) ||
+            !ok(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s0)) ||
+            !ok(hipStreamSynchronize(s0))) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        f.Anorm = *std::max_element(hrow.begin(), hrow.end());
+        f.info = hinfo == INT_MAX ? 0 : std::min(hinfo, N);
+    } while (0);
+    (void)hipFreeAsync(lp, s0);
+    (void)hipFreeAsync(scale, s0);
+    (void)hipFreeAsync(info, s0);
+    (void)hipFreeAsync(anorm, s0);
+    f.factor_seconds = now() - t0;
+    return rc;
+}
+
+int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX, int ldx,
+             double tol, int max_iter, hipStream_t s, int *iters, double *eta_out) {
+    const int N = f.N, NB = f.NB, bb = (N + NB - 1) / NB;
+    const long long Np = (long long)bb * NB;
+    const long long ldt = nrhs + (nrhs & 1);
+    double *Bt = nullptr, *Xt = nullptr, *Rt = nullptr, *T = nullptr, *nrm = nullptr;
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    const size_t vb = (size_t)(ldt * Np) * 8;
+    if (!ok(salloc(&Bt, vb, s)) || !ok(salloc(&Xt, vb, s)) || !ok(salloc(&Rt, vb, s)) ||
+        !ok(salloc(&T, (size_t)ldt * NB * 8, s)) || !ok(salloc(&nrm, (size_t)2 * ldt * 8, s)))
+        return OOCDAG_ENOMEM;
+    int rc = OOCDAG_OK;
+    int it = 0;
+    double eta = INFINITY, prev = INFINITY;
+    std::vector<double> bnorm(nrhs), hx(nrhs), hr(nrhs);
+    do {
+        if (!ok(hipMemsetAsync(Bt, 0, vb, s)) || !ok(launch_transpose(dB, ldb, Bt, ldt, N, nrhs, s))) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Bt, ldt, nrhs, N, nrm);
+        if (!ok(hipMemcpyAsync(bnorm.data(), nrm, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipMemcpyAsync(Xt, Bt, vb, hipMemcpyDeviceToDevice, s)) ||
+            !ok(potrs_inv_t(f.W, N, NB, f.Z, Xt, nrhs, ldt, T, s))) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        for (;;) {
+            if (!ok(hipMemcpyAsync(Rt, Bt, vb, hipMemcpyDeviceToDevice, s)) ||
+                !ok(symm_t(f.A, N, NB, Xt, Rt, ldt, nrhs, -1.0, 1.0, s))) {
+                rc = OOCDAG_EHIP;
+                break;
+            }
+            hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Xt, ldt, nrhs, N, nrm);
+            hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Rt, ldt, nrhs, N, nrm + ldt);
+            if (!ok(hipMemcpyAsync(hx.data(), nrm, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+                !ok(hipMemcpyAsync(hr.data(), nrm + ldt, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+                !ok(hipStreamSynchronize(s))) {
+                rc = OOCDAG_EHIP;
+                break;
+            }
+            prev = eta;
+            eta = 0;
+            for (int k = 0; k < nrhs; ++k) eta = std::max(eta, hr[k] / (f.Anorm * hx[k] + bnorm[k]));
+            if (!(eta == eta)) eta = INFINITY;
+            if (eta <= tol) break;
+            if (it >= max_iter || !(eta < 0.5 * prev)) {
+                rc = 1;  // stalled
+                break;
+            }
+            if (!ok(potrs_inv_t(f.W, N, NB, f.Z, Rt, nrhs, ldt, T, s))) {
+                rc = OOCDAG_EHIP;
+                break;
+            }
+            hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(256), 0, s, Xt, Rt, ldt * Np, 1.0);
+            ++it;
+        }
+        if (rc < 0) break;
+        if (!ok(launch_transpose(Xt, ldt, dX, ldx, nrhs, N, s)) || !ok(hipStreamSynchronize(s)))
+            rc = OOCDAG_EHIP;
+    } while (0);
+    for (double *p : {Bt, Xt, Rt, T, nrm})
+        if (p) (void)hipFreeAsync(p, s);
+    if (iters) *iters = it;
+    if (eta_out) *eta_out = eta;
+    return rc;
+}
+
+int mp_factor_solve(const double *dA, int N, int NB, const double *dB, int nrhs, int ldb,
+                    double *dX, int ldx, int precision, double tol, int max_iter, hipStream_t s,
+                    hipStream_t s1, oocdag_mpir_stats *st, MpFactor *keep) {
+    const double t0 = now();
+    MpFactor f;
+    int prec = precision, rc = OOCDAG_OK;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        rc = mp_factor(f, dA, N, NB, prec, s, s1);
+        if (rc) break;
+        st->factor_seconds += f.factor_seconds;
+        if (f.info && prec == 0) {
+            st->info = f.info;
+            st->precision_used = 0;
+            break;
+        }
+        int it = 0;
+        double eta = INFINITY;
+        if (!f.info) {
+            const double ts = now();
+            rc = mp_solve(f, dB, nrhs, ldb, dX, ldx, tol, max_iter, s, &it, &eta);
+            st->solve_seconds += now() - ts;
+            if (rc < 0) break;
+        }
+        st->iterations += it;
+        st->backward_error = eta;
+        if ((!f.info && rc == 0) || prec == 0) {
+            st->precision_used = prec;
+            rc = OOCDAG_OK;
+            break;
+        }
+        st->fell_back = 1;
+        prec = 0;
+    }
+    if (keep && rc == OOCDAG_OK && !st->info)
+        *keep = f;
+    else
+        mp_release(f, s);
+    (void)hipStreamSynchronize(s);
+    st->total_seconds = now() - t0;
+    return rc;
+}
+
+}  // namespace ooc
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
 #include "../../include/oocdag.h"
 
 using namespace ooc;
-
-namespace {
-struct Dev {
-    void *p = nullptr;
-    ~Dev() {
-        if (p) (void)hipFree(p);
-    }
-    bool alloc(size_t n) { return hipMalloc(&p, n) == hipSuccess; }
-    double *d() { return (double *)p; }
-};
-}  // namespace
 
 extern "C" int oocdag_gram_tiles(double *dY, int N, int NB, double mu, unsigned long long seed,
                                  void *stream) {
@@ -379,126 +564,17 @@ extern "C" int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const do
     if (!st) st = &dummy;
     *st = oocdag_mpir_stats{};
     if (!dA || !dB || !dX || N <= 0 || nrhs <= 0 || ldb < N || ldx < N || NB < 128 || NB % 128 ||
-        precision < 0 || precision > 2)
+        precision < 0 || precision > 2 || (ldb & 1) || (ldx & 1 && ldx != N))
         return OOCDAG_EARG;
-    if (precision == 2 && NB % 64) return OOCDAG_EARG;
     hipStream_t s = nullptr, s1 = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return OOCDAG_EHIP;
     if (hipStreamCreateWithFlags(&s1, hipStreamNonBlocking) != hipSuccess) {
         (void)hipStreamDestroy(s);
         return OOCDAG_EHIP;
     }
-    struct SD {
-        hipStream_t s, s1;
-        ~SD() {
-            (void)hipStreamDestroy(s1);
-            (void)hipStreamDestroy(s);
-        }
-    } sd{s, s1};
-    const double t0 = now();
-    const int bb = (N + NB - 1) / NB;
-    const long long Np = (long long)bb * NB;
-    const long long ldt = nrhs + (nrhs & 1);
-    const size_t tiles = (size_t)bb * (bb + 1) / 2 * NB * NB;
-    Dev W, dinv, info, lp, scale, Bt, Xt, Rt, anorm, nrm, Z, T;
-    if (!W.alloc(tiles * 8) || !dinv.alloc((size_t)Np * 64 * 8) || !info.alloc(sizeof(int)) ||
-        !Z.alloc((size_t)bb * NB * NB * 8) || !T.alloc((size_t)ldt * NB * 8) ||
-        !lp.alloc((size_t)2 * bb * NB * NB * 4) || !scale.alloc((size_t)2 * bb * 4) ||
-        !Bt.alloc((size_t)(ldt * Np) * 8) || !Xt.alloc((size_t)(ldt * Np) * 8) ||
-        !Rt.alloc((size_t)(ldt * Np) * 8) || !anorm.alloc((size_t)Np * 8) ||
-        !nrm.alloc((size_t)2 * ldt * 8))
-        return OOCDAG_ENOMEM;
-    auto ok = [](hipError_t e) { return e == hipSuccess; };
-    // ||A||_inf
-    if (!ok(hipMemsetAsync(anorm.p, 0, (size_t)Np * 8, s))) return OOCDAG_EHIP;
-    hipLaunchKernelGGL(k_tile_rowabs, dim3((NB + 255) / 256, bb, bb), dim3(256), 0, s, dA, NB, bb,
-                       anorm.d());
-    std::vector<double> hrow(Np);
-    if (!ok(hipMemcpyAsync(hrow.data(), anorm.p, Np * 8, hipMemcpyDeviceToHost, s)) ||
-        !ok(hipStreamSynchronize(s)))
-        return OOCDAG_EHIP;
-    const double Anorm = *std::max_element(hrow.begin(), hrow.end());
-    // B^T, padded with zeros past N
-    if (!ok(hipMemsetAsync(Bt.p, 0, (size_t)(ldt * Np) * 8, s)) ||
-        !ok(launch_transpose(dB, ldb, Bt.d(), ldt, N, nrhs, s)))
-        return OOCDAG_EHIP;
-    std::vector<double> bnorm(nrhs), hx(nrhs), hr(nrhs);
-    hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Bt.d(), ldt, nrhs, N, nrm.d());
-    if (!ok(hipMemcpyAsync(bnorm.data(), nrm.p, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
-        !ok(hipStreamSynchronize(s)))
-        return OOCDAG_EHIP;
-
-    int prec = precision;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        const double tf = now();
-        int big = INT_MAX;
-        if (!ok(hipMemcpyAsync(W.p, dA, tiles * 8, hipMemcpyDeviceToDevice, s)) ||
-            !ok(hipMemcpyAsync(info.p, &big, sizeof(int), hipMemcpyHostToDevice, s)))
-            return OOCDAG_EHIP;
-        int rc = factor_mixed(W.d(), N, NB, prec, dinv.d(), (int *)info.p, lp.p, (float *)scale.p, s,
-                              s1);
-        if (rc == -1) return OOCDAG_EHIP;  // MFMA layout probe failed
-        if (rc) return OOCDAG_EHIP;
-        if (!ok(tile_inverses(W.d(), N, NB, dinv.d(), Z.d(), s))) return OOCDAG_EHIP;
-        int hinfo = INT_MAX;
-        if (!ok(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s)) ||
-            !ok(hipStreamSynchronize(s)))
-            return OOCDAG_EHIP;
-        st->factor_seconds += now() - tf;
-        const double ts = now();
-        bool good = hinfo == INT_MAX;
-        if (!good && prec == 0) {
-            st->info = std::min(hinfo, N);
-            return OOCDAG_OK;
-        }
-        double eta = INFINITY, prev = INFINITY;
-        int it = 0;
-        if (good) {
-            // x = W^-1 b
-            if (!ok(hipMemcpyAsync(Xt.p, Bt.p, (size_t)(ldt * Np) * 8, hipMemcpyDeviceToDevice, s)) ||
-                !ok(potrs_inv_t(W.d(), N, NB, Z.d(), Xt.d(), nrhs, ldt, T.d(), s)))
-                return OOCDAG_EHIP;
-            for (;;) {
-                // r = b - A x
-                if (!ok(hipMemcpyAsync(Rt.p, Bt.p, (size_t)(ldt * Np) * 8, hipMemcpyDeviceToDevice, s)) ||
-                    !ok(symm_t(dA, N, NB, Xt.d(), Rt.d(), ldt, nrhs, -1.0, 1.0, s)))
-                    return OOCDAG_EHIP;
-                hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Xt.d(), ldt, nrhs, N,
-                                   nrm.d());
-                hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Rt.d(), ldt, nrhs, N,
-                                   nrm.d() + ldt);
-                if (!ok(hipMemcpyAsync(hx.data(), nrm.p, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
-                    !ok(hipMemcpyAsync(hr.data(), nrm.d() + ldt, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
-                    !ok(hipStreamSynchronize(s)))
-                    return OOCDAG_EHIP;
-                prev = eta;
-                eta = 0;
-                for (int k = 0; k < nrhs; ++k)
-                    eta = std::max(eta, hr[k] / (Anorm * hx[k] + bnorm[k]));
-                if (!(eta == eta)) eta = INFINITY;
-                if (eta <= tol) break;
-                if (it >= max_iter || !(eta < 0.5 * prev)) {
-                    good = false;
-                    break;
-                }
-                // x += W^-1 r
-                if (!ok(potrs_inv_t(W.d(), N, NB, Z.d(), Rt.d(), nrhs, ldt, T.d(), s))) return OOCDAG_EHIP;
-                hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(256), 0, s, Xt.d(), Rt.d(), ldt * Np, 1.0);
-                ++it;
-            }
-        }
-        st->solve_seconds += now() - ts;
-        st->iterations += it;
-        st->backward_error = eta;
-        if (good || prec == 0) {
-            st->precision_used = prec;
-            break;
-        }
-        st->fell_back = 1;
-        prec = 0;  // plain FP64 factorization
-    }
-    if (!ok(launch_transpose(Xt.d(), ldt, dX, ldx, nrhs, N, s)) || !ok(hipStreamSynchronize(s)))
-        return OOCDAG_EHIP;
-    st->total_seconds = now() - t0;
-    return OOCDAG_OK;
+    int rc = mp_factor_solve(dA, N, NB, dB, nrhs, ldb, dX, ldx, precision, tol, max_iter, s, s1, st,
+                             nullptr);
+    (void)hipStreamDestroy(s1);
+    (void)hipStreamDestroy(s);
+    return rc;
 }

@@ -153,6 +153,27 @@ hipError_t resid_upper_block(const double *U, long long ldu, int N, int NB, int 
                              char diag, int iseed, double *rowsum, hipStream_t s);
 
 // ---- mixed precision (mixed.hip) ----
+// A resident factorization of one matrix: W = factor (packed FP64 tiles),
+// Z = inverses of its diagonal tiles, dinv = its 64-block inverses; A is the
+// matrix itself (not owned) for the residuals.
+struct MpFactor {
+    const double *A = nullptr;
+    double *W = nullptr, *Z = nullptr, *dinv = nullptr;
+    int N = 0, NB = 0, precision = 0, info = 0;
+    double Anorm = 0, factor_seconds = 0;
+    size_t bytes = 0;
+};
+size_t mp_factor_bytes(int N, int NB);
+int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipStream_t s0,
+              hipStream_t s1);
+void mp_release(MpFactor &f, hipStream_t s);
+// refinement solve: 0 converged, 1 stalled, < 0 error
+int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX, int ldx,
+             double tol, int max_iter, hipStream_t s, int *iters, double *eta_out);
+hipError_t gram_tiles(double *dY, int N, int NB, double mu, unsigned long long seed,
+                      hipStream_t s);
+hipError_t normal_block(double *dB, int rows, int cols, long long ld, unsigned long long seed,
+                        unsigned long long stream, hipStream_t s);
 hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt, long long ld,
                   int nrhs, double alpha, double beta, hipStream_t s);
 
