@@ -158,7 +158,7 @@ int oocdag_normal_fill(double *dB, int rows, int cols, int ld, unsigned long lon
 
 /* ---------------------------------------------------------------------------
  * Batched small SPD factor + solve (no reference counterpart).  nsys
- * independent systems A_s x_s = b_s, n_s in [1, 1024], nrhs_s in [1, 8], in
+ * independent systems A_s x_s = b_s, n_s in [1, 1024], nrhs_s in [1, 64], in
  * device arenas laid out by oocdag_batch_layout (A_s n x n and B_s/X_s
  * n x nrhs, column-major with leading dimension n rounded up to even).  One
  * persistent kernel (work queue over the systems, largest first; each
@@ -191,9 +191,10 @@ int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long lon
 /* ---------------------------------------------------------------------------
  * Continuous-batching solve server (no reference counterpart).  Requests are
  * served by `slots` worker threads with their own HIP streams, admitted
- * against an HBM byte budget; small systems (n <= 1024, nrhs <= 8) that are
+ * against an HBM byte budget; small systems (n <= 1024) that are
  * pending together run in one launch of the batched kernel; larger ones are
- * factored in mixed precision + FP64 refinement and their factor is cached by
+ * factored in mixed precision (from n = 8192 on; FP64 below, where the
+ * refinement costs more than it saves) + FP64 refinement, their factor cached by
  * matrix_id (least recently used evicted under the budget); a request whose
  * working set exceeds the whole budget takes the out-of-core FP64 route.
  * ------------------------------------------------------------------------- */

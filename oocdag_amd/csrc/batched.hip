@@ -37,7 +37,8 @@ namespace {
 constexpr int NTB = 256;
 constexpr int PMAX = 1024;        // max staged panel rows
 constexpr int NMAX = PMAX + 64;   // max system order
-constexpr int RMAX = 8;           // max right-hand sides per system
+constexpr int RMAX = 8;           // right-hand sides resident at a time
+constexpr int RTOTAL = 64;        // max right-hand sides per system
 
 struct SysDesc {
     int n, nrhs, ld;       // ld: even leading dimension >= n of A, B, X and W
@@ -161,13 +162,13 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
         if (q >= a.nsys) break;
         const int sidx = a.order[q];
         const SysDesc sd = a.sys[sidx];
-        const int n = sd.n, nrhs = sd.nrhs, ld = sd.ld;
+        const int n = sd.n, nrhs_all = sd.nrhs, ld = sd.ld;
         const double *A = a.A + sd.a;
-        const double *B = a.B + sd.b;
-        double *X = a.X + sd.b;
+        const double *B0 = a.B + sd.b;
+        double *X0 = a.X + sd.b;
         FT *W = (FT *)a.W + sd.w;
         FT *Dinv = (FT *)a.Dinv + sd.d;
-        if (n > PMAX || nrhs > RMAX) {
+        if (n > PMAX || nrhs_all > RTOTAL) {
             if (tid == 0) a.status[sidx] = -1;
             continue;
         }
@@ -287,7 +288,14 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
             __syncthreads();
             continue;
         }
-        // 3./4. solve and refine (x in S.u.v.x, work vector in S.u.v.y)
+        // 3./4. solve and refine, RMAX right-hand sides at a time
+        // (x in S.u.v.x, work vector in S.u.v.y)
+        double eta_all = 0.0;
+        int it_all = 0;
+        for (int r0 = 0; r0 < nrhs_all; r0 += RMAX) {
+        const int nrhs = min(RMAX, nrhs_all - r0);
+        const double *B = B0 + (long long)r0 * ld;
+        double *X = X0 + (long long)r0 * ld;
         double *y = S.u.v.y, *x = S.u.v.x;
         for (int idx = tid; idx < n * nrhs; idx += NTB) {
             int i = idx % n, k = idx / n;
@@ -368,10 +376,14 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
             int i = idx % n, k = idx / n;
             X[i + (long long)k * ld] = x[k * NMAX + i];
         }
+        eta_all = fmax(eta_all, eta);
+        it_all = max(it_all, it);
+        __syncthreads();
+        }
         if (tid == 0) {
-            a.berr[sidx] = eta;
-            a.iters[sidx] = it;
-            a.status[sidx] = (eta <= a.tol) ? 0 : -2;
+            a.berr[sidx] = eta_all;
+            a.iters[sidx] = it_all;
+            a.status[sidx] = (eta_all <= a.tol) ? 0 : -2;
         }
         __syncthreads();
     }
@@ -531,7 +543,7 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     if (nsys <= 0 || !n || !nrhs || !dA || !dB || !dX || (precision != 0 && precision != 1))
         return OOCDAG_EARG;
     for (int s = 0; s < nsys; ++s)
-        if (n[s] < 1 || n[s] > PMAX || nrhs[s] < 1 || nrhs[s] > RMAX) return OOCDAG_EARG;
+        if (n[s] < 1 || n[s] > PMAX || nrhs[s] < 1 || nrhs[s] > RTOTAL) return OOCDAG_EARG;
     // descriptors and workspace offsets (FT elements), largest systems first
     std::vector<SysDesc> sd(nsys);
     const int Tmax = 32;

@@ -57,8 +57,12 @@ constexpr int MAX_BATCH = 512;
 
 int tile_nb(int n) { return n < 4096 ? 256 : 512; }
 
-// the batched kernel holds systems up to 1024 with at most 8 right-hand sides
-bool is_small(const oocdag_request &r) { return r.n <= SMALL_N && r.nrhs <= 8; }
+// the batched kernel holds systems up to 1024 with up to 64 right-hand sides
+bool is_small(const oocdag_request &r) { return r.n <= SMALL_N && r.nrhs <= 64; }
+
+// Below this order a low-precision factor does not pay for its refinement
+// steps on this GPU (measured): such requests are factored in FP64.
+constexpr int MP_MIN_N = 8192;
 
 // row sums of |A| for a column block (rows < n, w columns, ld)
 __global__ void k_rowabs_block(const double *A, long long ld, int n, int w, double *rowsum) {
@@ -317,8 +321,9 @@ void oocdag_server::run_small(std::vector<Job> &jobs, hipStream_t s, double t_st
             rc = OOCDAG_EHIP;
     }
     if (rc == OOCDAG_OK && hipStreamSynchronize(s) != hipSuccess) rc = OOCDAG_EHIP;
-    // precision of the batch: FP32 + refinement unless a request asks for FP64
-    int prec = 1;
+    // small systems: FP64 factor (FP32 + refinement does not pay at these
+    // orders); the batched kernel still refines when the error is above tol
+    int prec = 0;
     double tol = 1.0;
     for (auto &j : jobs) {
         if (j.rq.precision == 0) prec = 0;
@@ -397,7 +402,8 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
         else if (gram_tiles(ce->A, n, nb, q.mu, q.seed, s) != hipSuccess)
             rc = OOCDAG_EHIP;
         if (rc == OOCDAG_OK) {
-            rc = mp_factor_solve(ce->A, n, nb, B, q.nrhs, ldb, X, ldb, q.precision, q.tol, 30, s, s1, &st,
+            const int prec = n >= MP_MIN_N ? q.precision : 0;
+            rc = mp_factor_solve(ce->A, n, nb, B, q.nrhs, ldb, X, ldb, prec, q.tol, 30, s, s1, &st,
                                  &ce->f);
             if (rc == OOCDAG_OK && st.info) rc = st.info;
         }

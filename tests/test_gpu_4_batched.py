@@ -22,6 +22,8 @@ def test_batched_matches_fp64_oracle(gpu, prec):
     ooc = gpu
     n, r = sizes(120, 5)
     n[:3] = [1024, 33, 1]   # extremes: the largest order, an odd one, a scalar system
+    r[3] = 37               # more right-hand sides than are resident at once (chunks of 8)
+    r[4] = 64
     bt = ooc.Batch(n, r)
     bt.generate(mu=0.1, seed=5)
     tol = 8 * EPS
