@@ -195,12 +195,82 @@ struct StreamT {
     }
 };
 
-template <int BX, int TR>
+// Full-block operand stream: the block's BX rows lie inside one tile (or
+// the operand is a plain matrix) and K is a multiple of BK, so the address
+// of the block's current K step is uniform across the workgroup (scalar
+// registers) and each thread adds a fixed 32-bit offset.  No bounds checks.
+template <int BX>
+struct StreamF {
+    static constexpr int PPC = BX / 2;
+    static constexpr int CPP = NT / PPC;
+    static constexpr int NPASS = BK / CPP;
+
+    const double *p, *base;
+    long long ld, te, rowoff;
+    int ts, ym, tri, ti, tk, kc;
+    int voff, rp, col0;
+
+    __device__ __forceinline__ void init(const Opnd &o, int rb, int, int tid) {
+        rp = tid % PPC;
+        col0 = tid / PPC;
+        p = o.p;
+        ld = o.ld;
+        te = o.te;
+        ym = o.ym;
+        tri = o.tri;
+        const int gr = o.r0 + rb;
+        if (o.ts > 0) {
+            ts = o.ts;
+            ti = gr / ts;
+            rowoff = gr - (long long)ti * ts;
+            tk = o.c0 / ts;
+            kc = o.c0 - tk * ts;
+        } else {
+            ts = 0x40000000;
+            ti = 0;
+            rowoff = gr;
+            tk = 0;
+            kc = o.c0;
+        }
+        base = p + tix(ti, tk, ym, tri) * te + rowoff + (long long)kc * ld;
+        voff = 2 * rp + col0 * (int)ld;
+    }
+
+    __device__ __forceinline__ void load(d2 (&v)[NPASS], int, int) const {
+#pragma unroll
+        for (int s = 0; s < NPASS; ++s)
+            v[s] = *reinterpret_cast<const d2 *>(base + (voff + s * CPP * (int)ld));
+    }
+
+    __device__ __forceinline__ void advance() {
+        kc += BK;
+        if (kc >= ts) {
+            kc -= ts;
+            ++tk;
+            base = p + tix(ti, tk, ym, tri) * te + rowoff + (long long)kc * ld;
+        } else {
+            base += BK * ld;
+        }
+    }
+
+    __device__ __forceinline__ void store_lds(double *lds, int stride, const d2 (&v)[NPASS],
+                                              int) const {
+#pragma unroll
+        for (int s = 0; s < NPASS; ++s)
+            *reinterpret_cast<d2 *>(lds + (col0 + s * CPP) * stride + 2 * rp) = v[s];
+    }
+};
+
+template <int BX, int TR, int FULL>
 struct StreamSel {
     typedef Stream<BX> type;
 };
 template <int BX>
-struct StreamSel<BX, 1> {
+struct StreamSel<BX, 0, 1> {
+    typedef StreamF<BX> type;
+};
+template <int BX, int FULL>
+struct StreamSel<BX, 1, FULL> {
     typedef StreamT<BX> type;
 };
 
@@ -241,7 +311,7 @@ __device__ __forceinline__ long long xcd_remap(long long bid, long long G) {
     return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
-template <int BMT, int BN, int MODE, int KIND, int TB>
+template <int BMT, int BN, int MODE, int KIND, int TB, int FULL>
 __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     constexpr int BM = BMT;         // block rows (128, or 32 for few right-hand sides)
     constexpr int WM = BM / 2;      // wave tile rows
@@ -266,8 +336,9 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     if (m0 >= a.M || n0 >= a.N) return;
     if (a.lower && (a.C.r0 + m0 + BM - 1) < (a.C.c0 + n0)) return;
 
-    typedef typename StreamSel<BN, TB>::type SB_t;
-    Stream<BM> sa;
+    typedef typename StreamSel<BM, 0, FULL>::type SA_t;
+    typedef typename StreamSel<BN, TB, FULL>::type SB_t;
+    SA_t sa;
     SB_t sb;
     sa.init(a.A, m0, a.M, tid);
     sb.init(a.B, n0, a.N, tid);
@@ -278,7 +349,7 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    d2 ra[Stream<BM>::NPASS], rb[SB_t::NPASS];
+    d2 ra[SA_t::NPASS], rb[SB_t::NPASS];
     const int nk = (a.K + BK - 1) / BK;
 
     if (nk > 0) {
@@ -326,6 +397,38 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     // Epilogue.  v_mfma_f64_16x16x4_f64 leaves element r (0..3) of a lane's
     // accumulator at row (lane/16) + 4*r, column lane%16 of its 16x16 block
     // (checked on gfx950 against a CPU GEMM).
+    if constexpr (FULL != 0) {
+        // the whole BM x BN block lies in one tile of C (or C is plain)
+        const Opnd &Co = a.C;
+        const int gr0 = Co.r0 + m0, gc0 = Co.c0 + n0;
+        const double *cb;
+        if (Co.ts > 0) {
+            const int ti = gr0 / Co.ts, tk = gc0 / Co.ts;
+            cb = Co.p + tix(ti, tk, Co.ym, Co.tri) * Co.te + (gr0 - ti * Co.ts) +
+                 (long long)(gc0 - tk * Co.ts) * Co.ld;
+        } else {
+            cb = Co.p + gr0 + (long long)gc0 * Co.ld;
+        }
+        const bool diag = a.lower && gr0 < gc0 + BN;
+        const int ldc = (int)Co.ld;
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int rl = wm * WM + i * 16 + (lane >> 4);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int cl = wn * WN + j * 16 + (lane & 15);
+                double *c = const_cast<double *>(cb) + (rl + cl * ldc);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (diag && gr0 + rl + 4 * r < gc0 + cl) continue;
+                    double v = a.alpha * acc[i][j][r];
+                    if (a.beta != 0.0) v += a.beta * c[4 * r];
+                    c[4 * r] = v;
+                }
+            }
+        }
+        return;
+    }
     const Opnd &Co = a.C;
     const int cts = Co.ts > 0 ? Co.ts : 0x40000000;
 #pragma unroll
@@ -361,6 +464,12 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     }
 }
 
+// The block's rows of o (starting at a multiple of X past o.r0) stay in one
+// tile: tiles and the start row are multiples of the block height.
+static bool block_aligned(const Opnd &o, int X) {
+    return o.ts <= 0 || ((o.ts % X) == 0 && (o.r0 % X) == 0);
+}
+
 template <int BN, int MODE, int KIND, int TB = 0, int BM = 128>
 hipError_t launch_one(const Gemm &g, hipStream_t s) {
     KArgs a;
@@ -385,8 +494,17 @@ hipError_t launch_one(const Gemm &g, hipStream_t s) {
     }
     if (a.nblk <= 0) return hipSuccess;
     if (a.nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_dgemm_nt<BM, BN, MODE, KIND, TB>), dim3((unsigned)a.nblk), dim3(NT), 0, s,
-                       a);
+    const bool full = !TB && (g.M % BM) == 0 && (g.N % BN) == 0 && (g.K % BK) == 0 &&
+                      block_aligned(g.A, BM) && block_aligned(g.B, BN) && block_aligned(g.C, BM) &&
+                      (g.C.ts <= 0 || ((g.C.ts % BN) == 0 && (g.C.c0 % BN) == 0)) &&
+                      g.A.ld * BK < 0x7fffffffLL && g.B.ld * BK < 0x7fffffffLL &&
+                      g.C.ld * BN < 0x7fffffffLL;
+    if (full)
+        hipLaunchKernelGGL((k_dgemm_nt<BM, BN, MODE, KIND, TB, !TB>), dim3((unsigned)a.nblk),
+                           dim3(NT), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_dgemm_nt<BM, BN, MODE, KIND, TB, 0>), dim3((unsigned)a.nblk),
+                           dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
