@@ -32,55 +32,128 @@ __device__ __forceinline__ long long opnd_off(const Opnd &o, int r, int c) {
     return t * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
 }
 
-// One wave64 factors a 64 x 64 diagonal block held in LDS and inverts the
-// resulting triangle; lane i owns row i during the factorization (left-
-// looking: column j is a dot product of row i with row j, then one square
-// root and a scaling) and column i during the inversion (independent forward
-// substitutions, no barriers).  do_factor = 0: the block already holds a
-// lower-triangular factor; only its inverse is formed and the block is left
-// untouched.
+// Columns [K0, K1) of the trailing update of factorization step J: lane i
+// subtracts l_ij * l_kj from its a_ik, column J of L read from LDS.
+template <int J, int K0, int K1>
+__device__ __forceinline__ void chol_cols(double (&a)[PB], const double *col) {
+    if constexpr (K1 > K0) {
+        double cv[K1 - K0];
+#pragma unroll
+        for (int k = K0; k < K1; ++k) cv[k - K0] = col[k];   // one LDS round trip per chunk
+#pragma unroll
+        for (int k = K0; k < K1; ++k) {
+            a[k] -= a[J] * cv[k - K0];
+            // pin the update here: otherwise the compiler sinks it to the
+            // next read of a[k] and keeps every earlier column alive
+            asm volatile("" : "+v"(a[k]));
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);   // bounded register use
+}
+
+// Step J of the register-resident factorization (template recursion keeps
+// every register index a compile-time constant).  bad: first failed pivot
+// (1-based), uniform.
+template <int J>
+__device__ __forceinline__ void chol_step(double (&a)[PB], double *col, int lane, int &bad) {
+    if constexpr (J < PB) {
+        // an opaque copy of the lane index keeps the compiler from forming
+        // all 64 steps' lane masks up front (they would not fit in SGPRs)
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        if (ln == J) col[PB] = a[J];
+        __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): single wave, in-order LDS
+        const double d = col[PB];
+        if (bad == 0 && !(d > 0.0)) bad = J + 1;
+        const double sd = sqrt(d);
+        const double rd = 1.0 / sd;
+        // lanes above the pivot keep 0 so later steps leave them unchanged
+        a[J] = ln > J ? a[J] * rd : (ln == J ? sd : 0.0);
+        col[ln] = a[J];
+        __builtin_amdgcn_sched_barrier(0);
+        chol_cols<J, J + 1, (J + 1 > 16 ? J + 1 : 16)>(a, col);
+        chol_cols<J, (J + 1 > 16 ? J + 1 : 16), (J + 1 > 32 ? J + 1 : 32)>(a, col);
+        chol_cols<J, (J + 1 > 32 ? J + 1 : 32), (J + 1 > 48 ? J + 1 : 48)>(a, col);
+        chol_cols<J, (J + 1 > 48 ? J + 1 : 48), PB>(a, col);
+        chol_step<J + 1>(a, col, lane, bad);
+    }
+}
+
+// Rows [I0, I1) of step K of the inversion.
+template <int K, int I0, int I1>
+__device__ __forceinline__ void inv_rows(double (&y)[PB], const double *S) {
+#pragma unroll
+    for (int i = I0; i < I1; ++i) y[i] -= S[K * PS + i] * y[K];
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Step K of the inversion: lane c holds column c of L^-1 being formed;
+// column K of L is read from LDS (broadcast reads) 16 rows at a time.
+template <int K>
+__device__ __forceinline__ void inv_step(double (&y)[PB], const double *S) {
+    if constexpr (K < PB) {
+        y[K] *= 1.0 / S[K * PS + K];
+        inv_rows<K, K + 1, (K + 1 > 16 ? K + 1 : 16)>(y, S);
+        inv_rows<K, (K + 1 > 16 ? K + 1 : 16), (K + 1 > 32 ? K + 1 : 32)>(y, S);
+        inv_rows<K, (K + 1 > 32 ? K + 1 : 32), (K + 1 > 48 ? K + 1 : 48)>(y, S);
+        inv_rows<K, (K + 1 > 48 ? K + 1 : 48), PB>(y, S);
+        inv_step<K + 1>(y, S);
+    }
+}
+
+// One wave64 factors a 64 x 64 diagonal block and inverts the triangle,
+// register-resident.  Factorization (right-looking, lane i owns row i in
+// registers): at step j the pivot is read back from LDS, column j is scaled
+// by 1/l_jj (as LAPACK dpotf2 scales by the reciprocal) and published in
+// LDS, and every lane subtracts l_ij * l_kj from its a_ik, k > j -- the same
+// subtraction order per element as the left-looking dot products.
+// Inversion (lane c owns column c of L^-1): y = e_c, then for k = 0..63
+// y_k *= 1/l_kk and y_i -= l_ik y_k for i > k, with column k of L read as
+// LDS broadcasts.  do_factor = 0: the block already holds L; only the
+// inverse is formed and the block is left untouched.
 __global__ __launch_bounds__(64) void k_potrf64(Opnd P, int gcol0, double *dinv, int *info,
                                                 int do_factor) {
-    __shared__ double S[PB * PS];
-    __shared__ double X[PB * PS];
+    __shared__ double S[PB * PS];   // L (then L^-1), column-major, column stride PS
+    __shared__ double col[PB + 1];  // column j of L, [PB]: the pivot
     const int lane = threadIdx.x;
     const long long base = opnd_off(P, 0, 0);  // block never straddles a tile
     const long long ld = P.ld;
     double *g = P.p + base;
 
-    for (int c = 0; c < PB; ++c) S[c * PS + lane] = (lane >= c) ? g[lane + c * ld] : 0.0;
-    __syncthreads();
+    double a[PB];
+#pragma unroll
+    for (int c = 0; c < PB; ++c) {
+        const double v = g[lane + c * ld];   // whole block is in bounds
+        a[c] = lane >= c ? v : 0.0;
+    }
 
     if (do_factor) {
-        for (int j = 0; j < PB; ++j) {
-            double v = 0.0;
-            if (lane >= j) {
-                v = S[j * PS + lane];
-                for (int k = 0; k < j; ++k) v -= S[k * PS + lane] * S[k * PS + j];
-            }
-            if (lane == j) {
-                if (!(v > 0.0)) atomicMin(info, gcol0 + j + 1);
-                S[j * PS + j] = sqrt(v);
-            }
-            __syncthreads();
-            if (lane > j) S[j * PS + lane] = v / S[j * PS + j];
-            __syncthreads();
-        }
+        int bad = 0;
+        chol_step<0>(a, col, lane, bad);
+        if (bad && lane == 0) atomicMin(info, gcol0 + bad);
+        // opaque base: the write-back recomputes its addresses instead of
+        // keeping the 64 load addresses alive through the factorization
+        double *gs = g;
+        asm volatile("" : "+s"(gs));
+#pragma unroll
         for (int c = 0; c < PB; ++c)
-            if (lane >= c) g[lane + c * ld] = S[c * PS + lane];
+            if (lane >= c) gs[lane + c * ld] = a[c];
     }
 
-    // X = L^-1, lane = column
-    const int col = lane;
-    for (int i = 0; i < col; ++i) X[col * PS + i] = 0.0;
-    X[col * PS + col] = 1.0 / S[col * PS + col];
-    for (int i = col + 1; i < PB; ++i) {
-        double sum = 0.0;
-        for (int d = col; d < i; ++d) sum += S[d * PS + i] * X[col * PS + d];
-        X[col * PS + i] = -sum / S[i * PS + i];
-    }
+#pragma unroll
+    for (int c = 0; c < PB; ++c) S[c * PS + lane] = a[c];   // entries above the diagonal unused
     __syncthreads();
-    for (int c = 0; c < PB; ++c) dinv[lane + c * PB] = (lane >= c) ? X[c * PS + lane] : 0.0;
+
+    double y[PB];
+#pragma unroll
+    for (int i = 0; i < PB; ++i) y[i] = (i == lane) ? 1.0 : 0.0;
+    inv_step<0>(y, S);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PB; ++i) S[lane * PS + i] = y[i];   // column lane of L^-1
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < PB; ++c) dinv[lane + c * PB] = S[c * PS + lane];
 }
 
 __global__ void k_pad_tile(double *t, int nb, int vr, int vc, int diag) {

@@ -3,12 +3,15 @@
 //
 //   C64 -= s_i * s_j * (X_i X_j^T)     over the lower trapezoid of a panel
 //
-// C stays in the FP64 packed tiles; X is a low-precision copy of the tile
-// column that drives the update, produced by k_convert_column:
-//   FP32 : column-major tiles (ld = NB), s = 1, v_mfma_f32_16x16x4_f32
-//   BF16 : row-major tiles (each row's NB values contiguous, so one 16-byte
-//          LDS read feeds a lane of v_mfma_f32_16x16x32_bf16), one power-of-
-//          two scale per tile so every tile is O(1) before rounding.
+// C stays in the FP64 packed tiles; X is a low-precision copy of a super-
+// column (several factored tile columns side by side, so one update has
+// depth K = sw * NB and the FP64 trailing matrix is read and written once per
+// super-column, not once per tile column), produced by convert_super:
+//   FP32 : column-major, s = 1, v_mfma_f32_16x16x4_f32
+//   BF16 : row-major (each row's K values contiguous, so one 16-byte LDS read
+//          feeds a lane of v_mfma_f32_16x16x32_bf16), one power-of-two scale
+//          per tile row of the super-column so every row block is O(1)
+//          before rounding.
 // Accumulation is FP32 in both cases; the diagonal POTRF, the TRSM, the
 // residual and the solution update remain FP64 (factor.hip, solve.hip).
 //
@@ -17,6 +20,7 @@
 // the operand lane layouts assumed below; a mismatch is reported as an error.
 #include "ooc_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 
@@ -88,59 +92,101 @@ __global__ void k_probe_mfma(int kind, float *out) {
 }
 
 // ---- conversion ------------------------------------------------------------
-// Tiles jt..bb-1 of tile column jt of the packed FP64 panel -> rect
-// low-precision buffer.  mode 0: FP32 column-major; mode 1: BF16 row-major
-// with per-tile scale 2^e (scale[t] holds it), e = exponent of the tile's
-// max |x|.
-__global__ void k_tile_absmax(const double *Y, int nb, int ym, int jt, int ntiles,
-                              float *scale) {
-    const int t = blockIdx.x;
-    if (t >= ntiles) return;
-    const double *tile = Y + tix(jt + t, jt, ym, 1) * (long long)nb * nb;
+// A super-column (tile columns j0 .. j0+sw-1 of the packed FP64 panel, tile
+// rows i0 .. i0+nrt-1) -> low-precision operand X, local row r = global row
+// i0*nb + r, local k = global column j0*nb + k (K = sw*nb):
+//   FP32: column-major, element (r, k) at X[k * ldx + r], ldx = nrt*nb
+//   BF16: row-major,    element (r, k) at X[r * ldx + k], ldx = K, divided by
+//         the row block's scale 2^e (e = exponent of the largest |x| over the
+//         sw tiles of tile row i) before rounding.
+__global__ void k_zero_u32(unsigned *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// grid: (16 chunks, sw, nrt); max |x| of each tile into amax[tile row]
+// (non-negative floats order like their bit patterns)
+__global__ void k_absmax_super(const double *W, int nb, int ym, int j0, int i0, unsigned *amax) {
+    const int t = blockIdx.y, i = blockIdx.z;
+    const double *tile = W + tix(i0 + i, j0 + t, ym, 1) * (long long)nb * nb;
+    const long long per = (long long)nb * nb;
+    const long long chunk = (per + gridDim.x - 1) / gridDim.x;
+    const long long b0 = blockIdx.x * chunk, b1 = min(per, b0 + chunk);
     double m = 0.0;
-    for (long long i = threadIdx.x; i < (long long)nb * nb; i += blockDim.x) m = fmax(m, fabs(tile[i]));
+    for (long long e = b0 + threadIdx.x; e < b1; e += blockDim.x) m = fmax(m, fabs(tile[e]));
     __shared__ double red[256];
     red[threadIdx.x] = m;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        int e = 0;
-        if (red[0] > 0.0) frexp(red[0], &e);
-        scale[t] = ldexpf(1.0f, e);
-    }
+    if (threadIdx.x == 0) atomicMax(&amax[i], __float_as_uint((float)red[0]));
 }
 
-__global__ void k_convert_column(const double *Y, int nb, int ym, int jt, int ntiles, int mode,
-                                 const float *scale, void *out) {
-    const long long per = (long long)nb * nb;
-    const long long total = per * ntiles;
+__global__ void k_scale_super(const unsigned *amax, int nrt, float *scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrt) return;
+    const float m = __uint_as_float(amax[i]);
+    int e = 0;
+    if (m > 0.f) frexpf(m, &e);
+    scale[i] = ldexpf(1.0f, e);
+}
+
+// FP32: grid-stride over the output in column-major order (reads and writes
+// both run down columns)
+__global__ void k_convert_super_f32(const double *W, int nb, int ym, int j0, int sw, int i0, int nrt,
+                                    float *X) {
+    const long long R = (long long)nrt * nb, K = (long long)sw * nb;
+    const long long total = R * K;
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
-        long long t = idx / per, w = idx - t * per;
-        const double *tile = Y + tix(jt + (int)t, jt, ym, 1) * per;
-        if (mode == 0) {
-            ((float *)out)[idx] = (float)tile[w];
-        } else {
-            // w enumerates the output row-major: (row, k) = (w / nb, w % nb)
-            int row = (int)(w / nb), k = (int)(w - (long long)row * nb);
-            float v = (float)(tile[row + (long long)k * nb] / (double)scale[t]);
-            ((unsigned short *)out)[idx] = f2bf(v);
-        }
+        const long long k = idx / R, r = idx - k * R;
+        const int ti = (int)(r / nb), tk = (int)(k / nb);
+        const double v = W[tix(i0 + ti, j0 + tk, ym, 1) * (long long)nb * nb + (r - (long long)ti * nb) +
+                           (k - (long long)tk * nb) * nb];
+        X[idx] = (float)v;
     }
 }
 
-// ---- FP32 update ------------------------------------------------------------
+// BF16: one 64 x 64 patch per block, transposed through LDS so both the
+// column-major FP64 reads and the row-major BF16 writes are contiguous
+__global__ __launch_bounds__(256) void k_convert_super_bf16(const double *W, int nb, int ym, int j0,
+                                                            int i0, const float *scale,
+                                                            unsigned short *X, long long ldx) {
+    __shared__ float T[64][65];
+    const int pr = blockIdx.x, pk = blockIdx.y;   // patch row / column (64 each)
+    const int r0 = pr * 64, k0 = pk * 64;
+    const int ti = r0 / nb, tk = k0 / nb;
+    const double *tile = W + tix(i0 + ti, j0 + tk, ym, 1) * (long long)nb * nb;
+    const int rr = r0 - ti * nb, kk = k0 - tk * nb;
+    const double inv = 1.0 / (double)scale[ti];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = ty + 4 * q;
+        T[k][tx] = (float)(tile[(rr + tx) + (long long)(kk + k) * nb] * inv);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int r = ty + 4 * q;
+        X[(long long)(r0 + r) * ldx + k0 + tx] = f2bf(T[tx][r]);
+    }
+}
+
+// ---- FP32 / BF16 update ------------------------------------------------------
+// C (packed FP64 tiles, diagonal-aligned lower trapezoid at global r0,
+// M x N) -= s_m s_n X_m X_n^T, local row r of C <-> row r of X.
 struct LpArgs {
     double *C;          // packed FP64 panel
     int c_ym;           // tile rows of the panel
+    int nb;             // tile edge of C
     int r0;             // global row/col of the region's top-left (diagonal)
-    const void *X;      // low-precision column: tile t = rows [r0 + t*nb, ...)
-    const float *scale; // BF16: per-tile scale
-    int nb;
-    int M, N, K;        // region rows, cols, K = nb
+    const void *X;      // low-precision operand, row 0 = global row r0
+    long long ldx;      // FP32: column stride; BF16: row stride (elements)
+    const float *scale; // BF16: scale of the row block holding local row r: scale[r / nb]
+    int M, N, K;        // region rows, cols, depth
     int mb, nbk;        // block grid
     long long nblk;
     int rowmap;         // 0: row = 4*(lane/16) + r, 1: row = lane/16 + 4*r
@@ -163,13 +209,8 @@ __global__ __launch_bounds__(NT, 2) void k_update_f32(LpArgs a) {
     // each thread streams 4 consecutive rows (16 B) of 4 columns per step:
     // q = tid + 256 s -> column q / 32, rows 4 * (q % 32)
     const int quad = tid & 31, col0 = tid >> 5;  // col0 in 0..7, columns col0 + 8 s
-    auto rowptr = [&](int r) -> const float * {
-        int t = r / a.nb;
-        return X + t * per + (r - t * a.nb);
-    };
-    const float *pa = rowptr(m0 + 4 * quad);
-    const float *pb = rowptr(n0 + 4 * quad);
-    const bool oka = (m0 + 4 * quad) < a.M, okb = (n0 + 4 * quad) < a.N;
+    const float *pa = X + m0 + 4 * quad;
+    const float *pb = X + n0 + 4 * quad;
 
     f4 acc[4][4];
 #pragma unroll
@@ -180,9 +221,9 @@ __global__ __launch_bounds__(NT, 2) void k_update_f32(LpArgs a) {
     auto gload = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            int c = k0 + col0 + 8 * s;
-            ra[s] = oka ? *reinterpret_cast<const f4 *>(pa + (long long)c * a.nb) : f4{0, 0, 0, 0};
-            rb[s] = okb ? *reinterpret_cast<const f4 *>(pb + (long long)c * a.nb) : f4{0, 0, 0, 0};
+            const long long c = k0 + col0 + 8 * s;
+            ra[s] = *reinterpret_cast<const f4 *>(pa + c * a.ldx);
+            rb[s] = *reinterpret_cast<const f4 *>(pb + c * a.ldx);
         }
     };
     auto lstore = [&](int buf) {
@@ -225,7 +266,6 @@ __global__ __launch_bounds__(NT, 2) void k_update_f32(LpArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int n = n0 + wn * 64 + j * 16 + (lane & 15);
-            if (n >= a.N) continue;
             const int gr0 = a.r0 + mb16, gc = a.r0 + n;
             const int ti = gr0 / a.nb, tk = gc / a.nb;
             double *c = a.C + tix(ti, tk, a.c_ym, 1) * per + (gr0 - ti * a.nb) +
@@ -233,14 +273,13 @@ __global__ __launch_bounds__(NT, 2) void k_update_f32(LpArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int rr = a.rowmap ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
-                if (mb16 + rr >= a.M || gr0 + rr < gc) continue;
+                if (gr0 + rr < gc) continue;
                 c[rr] -= (double)acc[i][j][r];
             }
         }
     }
 }
 
-// ---- BF16 update ------------------------------------------------------------
 constexpr int BK16 = 64;            // bf16 per K step
 constexpr int S16 = BK16 + 8;       // LDS row stride (bf16), rows are m
 
@@ -255,35 +294,23 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
     const int m0 = mbi * BM, n0 = nbi * BM;
     const unsigned short *X = (const unsigned short *)a.X;
     const long long per = (long long)a.nb * a.nb;
-    // row-major tiles: element (r, k) at tile(r)*per + (r % nb)*nb + k.
+    // row-major X: element (r, k) at r * ldx + k.
     // q = tid + 256 s -> row q / 8 (32 rows per pass), 8-wide k chunk q % 8
     const int kc = tid & 7, rr0 = tid >> 3;
-    auto rowbase = [&](int r) -> long long {
-        int t = r / a.nb;
-        return t * per + (long long)(r - t * a.nb) * a.nb;
-    };
-    long long ba[4], bb_[4];
-    bool oka[4], okb[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        int ra_ = m0 + rr0 + 32 * s, rb_ = n0 + rr0 + 32 * s;
-        oka[s] = ra_ < a.M;
-        okb[s] = rb_ < a.N;
-        ba[s] = oka[s] ? rowbase(ra_) : 0;
-        bb_[s] = okb[s] ? rowbase(rb_) : 0;
-    }
+    const unsigned short *pa = X + (long long)(m0 + rr0) * a.ldx + 8 * kc;
+    const unsigned short *pb = X + (long long)(n0 + rr0) * a.ldx + 8 * kc;
+    const long long pass = 32 * a.ldx;
     f4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
     s8 ra[4], rb[4];
-    const s8 z = {0, 0, 0, 0, 0, 0, 0, 0};
     auto gload = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            ra[s] = oka[s] ? *reinterpret_cast<const s8 *>(X + ba[s] + k0 + 8 * kc) : z;
-            rb[s] = okb[s] ? *reinterpret_cast<const s8 *>(X + bb_[s] + k0 + 8 * kc) : z;
+            ra[s] = *reinterpret_cast<const s8 *>(pa + s * pass + k0);
+            rb[s] = *reinterpret_cast<const s8 *>(pb + s * pass + k0);
         }
     };
     auto lstore = [&](int buf) {
@@ -329,7 +356,6 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int n = n0 + wn * 64 + j * 16 + (lane & 15);
-            if (n >= a.N) continue;
             const double s = (double)si * (double)a.scale[n / a.nb];
             const int gr0 = a.r0 + mb16, gc = a.r0 + n;
             const int ti = gr0 / a.nb, tk = gc / a.nb;
@@ -338,7 +364,7 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int rr = a.rowmap ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
-                if (mb16 + rr >= a.M || gr0 + rr < gc) continue;
+                if (gr0 + rr < gc) continue;
                 c[rr] -= s * (double)acc[i][j][r];
             }
         }
@@ -379,36 +405,44 @@ int mfma_rowmap(int kind, hipStream_t s) {
     return -1;
 }
 
-hipError_t convert_column(const double *Y, int nb, int ym, int jt, int ntiles, int mode,
-                          float *scale, void *out, hipStream_t s) {
-    if (mode == 1) {
-        hipLaunchKernelGGL(k_tile_absmax, dim3(ntiles), dim3(256), 0, s, Y, nb, ym, jt, ntiles,
-                           scale);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+hipError_t convert_super(const double *W, int nb, int ym, int j0, int sw, int i0, int nrt, int mode,
+                         unsigned *amax, float *scale, void *X, hipStream_t s) {
+    if (nrt <= 0) return hipSuccess;
+    if (mode == 0) {
+        const long long total = (long long)nrt * nb * sw * nb;
+        const int blocks = (int)std::min<long long>((total + 255) / 256, 32768);
+        hipLaunchKernelGGL(k_convert_super_f32, dim3(blocks), dim3(256), 0, s, W, nb, ym, j0, sw, i0,
+                           nrt, (float *)X);
+        return hipGetLastError();
     }
-    long long total = (long long)nb * nb * ntiles;
-    int blocks = (int)std::min<long long>((total + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_convert_column, dim3(blocks), dim3(256), 0, s, Y, nb, ym, jt, ntiles,
-                       mode, (const float *)scale, out);
+    if (nb % 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_u32, dim3((nrt + 255) / 256), dim3(256), 0, s, amax, nrt);
+    hipLaunchKernelGGL(k_absmax_super, dim3(16, sw, nrt), dim3(256), 0, s, W, nb, ym, j0, i0, amax);
+    hipLaunchKernelGGL(k_scale_super, dim3((nrt + 255) / 256), dim3(256), 0, s, amax, nrt, scale);
+    hipLaunchKernelGGL(k_convert_super_bf16, dim3(nrt * nb / 64, sw * nb / 64), dim3(256), 0, s, W,
+                       nb, ym, j0, i0, (const float *)scale, (unsigned short *)X,
+                       (long long)sw * nb);
     return hipGetLastError();
 }
 
-hipError_t launch_update_lp(int mode, double *C, int c_ym, int r0, const void *X,
-                            const float *scale, int nb, int M, int N, int rowmap,
+hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const void *X,
+                            long long ldx, int K, const float *scale, int M, int N, int rowmap,
                             hipStream_t s) {
-    if (M <= 0 || N <= 0) return hipSuccess;
-    if (nb % BK16 || nb % BM || M % BM || N % BM || r0 % BM) return hipErrorInvalidValue;
+    if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+    if (nb % BM || M % BM || N % BM || r0 % BM || M < N) return hipErrorInvalidValue;
+    if (K % (mode == 0 ? BK32 : BK16)) return hipErrorInvalidValue;
+    if (mode == 0 ? (ldx % 4) != 0 : (ldx % 8) != 0) return hipErrorInvalidValue;
     LpArgs a;
     a.C = C;
     a.c_ym = c_ym;
+    a.nb = nb;
     a.r0 = r0;
     a.X = X;
+    a.ldx = ldx;
     a.scale = scale;
-    a.nb = nb;
     a.M = M;
     a.N = N;
-    a.K = nb;
+    a.K = K;
     a.mb = M / BM;
     a.nbk = N / BM;
     long long n = 0;

@@ -25,10 +25,10 @@
 namespace ooc {
 
 int mfma_rowmap(int kind, hipStream_t s);
-hipError_t convert_column(const double *Y, int nb, int ym, int jt, int ntiles, int mode,
-                          float *scale, void *out, hipStream_t s);
-hipError_t launch_update_lp(int mode, double *C, int c_ym, int r0, const void *X,
-                            const float *scale, int nb, int M, int N, int rowmap,
+hipError_t convert_super(const double *W, int nb, int ym, int j0, int sw, int i0, int nrt, int mode,
+                         unsigned *amax, float *scale, void *X, hipStream_t s);
+hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const void *X,
+                            long long ldx, int K, const float *scale, int M, int N, int rowmap,
                             hipStream_t s);
 
 namespace {
@@ -279,64 +279,124 @@ hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double 
     return hipSuccess;
 }
 
-// Trailing updates of the mixed-precision factorization.  precision 0:
-// FP64 GEMM; 1/2: convert the final tile column to FP32/BF16 (double-buffered
-// by column parity, so column j+1's conversion never overwrites what the
-// bulk update of column j is still reading) and run the low-precision MFMA
-// update into the FP64 tiles.
-struct MixedUpdater : PanelUpdater {
+// FP64 trailing updates for the plain path of the look-ahead panel loop.
+struct Fp64PanelUpdater : PanelUpdater {
     double *W;
-    int NB, bb, precision, rowmap;
-    void *lp[2];
-    float *scale[2];
-    hipError_t prepare(int j, hipStream_t s) override {
-        if (precision == 0) return hipSuccess;
-        return convert_column(W, NB, bb, j, bb - j, precision - 1, scale[j & 1], lp[j & 1], s);
-    }
+    int NB, bb;
+    hipError_t prepare(int, hipStream_t) override { return hipSuccess; }
     hipError_t update(int j, int c0, int nc, hipStream_t s) override {
-        if (precision == 0) {
-            const Opnd P = panel(W, NB, bb, 1);
-            Gemm g;
-            g.A = P; g.A.r0 = c0 * NB; g.A.c0 = j * NB;
-            g.B = g.A;
-            g.C = P; g.C.r0 = c0 * NB; g.C.c0 = c0 * NB;
-            g.M = (bb - c0) * NB; g.N = nc * NB; g.K = NB;
-            g.alpha = -1.0; g.beta = 1.0; g.lower = 1;
-            return launch_gemm(g, GEMM_UPDATE, s);
-        }
-        const int mode = precision - 1;
-        const size_t esz = mode == 0 ? 4 : 2;
-        const size_t skip = (size_t)(c0 - j) * NB * NB * esz;  // tiles j .. c0-1
-        return launch_update_lp(mode, W, bb, c0 * NB, (const char *)lp[j & 1] + skip,
-                                scale[j & 1] + (c0 - j), NB, (bb - c0) * NB, nc * NB, rowmap, s);
+        const Opnd P = panel(W, NB, bb, 1);
+        Gemm g;
+        g.A = P; g.A.r0 = c0 * NB; g.A.c0 = j * NB;
+        g.B = g.A;
+        g.C = P; g.C.r0 = c0 * NB; g.C.c0 = c0 * NB;
+        g.M = (bb - c0) * NB; g.N = nc * NB; g.K = NB;
+        g.alpha = -1.0; g.beta = 1.0; g.lower = 1;
+        return launch_gemm(g, GEMM_UPDATE, s);
     }
 };
 
+// Tile columns per super-column: low-precision updates of depth ~2048.
+int mp_super_width(int NB, int bb) { return std::max(1, std::min(bb, 2048 / NB)); }
+
 // Factor W (packed tiles) with the given update precision; 64-block inverses
-// of every diagonal tile land in dinv_all for the solves.  lpbuf: two
-// low-precision tile columns (bb tiles of 4 bytes each, each), scale: 2*bb.
+// of every diagonal tile land in dinv_all for the solves.
+//
+// precision 0: the FP64 look-ahead panel loop (one tile column at a time).
+// precision 1/2 (FP32/BF16): super-columns of sw tile columns.  Inside a
+// super-column every tile column is factored in FP64 (factor_column) and
+// applied to the rest of the super-column with the FP64 GEMM; the finished
+// super-column is converted once (convert_super) and drives
+//   narrow: the next super-column        (low precision, K = sw*NB, stream s)
+//   bulk  : everything to its right      (low precision, stream s1)
+// so the next super-column's FP64 work overlaps the bulk update.  lpbuf
+// holds two converted super-columns (by parity), scale 2*bb floats, amax bb.
 int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int *dinfo,
-                 void *lpbuf, float *scale, hipStream_t s, hipStream_t s1) {
+                 void *lpbuf, float *scale, unsigned *amax, hipStream_t s, hipStream_t s1) {
     const int bb = (N + NB - 1) / NB;
-    MixedUpdater up;
-    up.W = W;
-    up.NB = NB;
-    up.bb = bb;
-    up.precision = precision;
-    up.rowmap = 0;
-    const size_t col_bytes = (size_t)bb * NB * NB * 4;
-    up.lp[0] = lpbuf;
-    up.lp[1] = (char *)lpbuf + col_bytes;
-    up.scale[0] = scale;
-    up.scale[1] = scale + bb;
-    if (precision > 0) {
-        up.rowmap = mfma_rowmap(precision - 1, s);
-        if (up.rowmap < 0) return -1;
-    }
     const size_t inv_per_tile = (size_t)(NB / 64) * 64 * 64;
-    int rc = factor_panel_la(panel(W, NB, bb, 1), NB, bb, bb, 0, dinv_all, inv_per_tile, dinfo, s,
-                             s1, up, [](int) { return 0; });
-    return rc ? -2 : 0;
+    const Opnd P = panel(W, NB, bb, 1);
+    if (precision == 0) {
+        Fp64PanelUpdater up;
+        up.W = W;
+        up.NB = NB;
+        up.bb = bb;
+        int rc = factor_panel_la(P, NB, bb, bb, 0, dinv_all, inv_per_tile, dinfo, s, s1, up,
+                                 [](int) { return 0; });
+        return rc ? -2 : 0;
+    }
+    const int mode = precision - 1;
+    const int rowmap = mfma_rowmap(mode, s);
+    if (rowmap < 0) return -1;
+    const int sw = mp_super_width(NB, bb);
+    const int nsc = (bb + sw - 1) / sw;
+    const size_t esz = mode == 0 ? 4 : 2;
+    const size_t lp_bytes = (size_t)bb * NB * sw * NB * esz;
+    void *lp[2] = {lpbuf, (char *)lpbuf + lp_bytes};
+    float *sc[2] = {scale, scale + bb};
+
+    hipEvent_t ev_fact, ev_bulk[2];
+    if (hipEventCreateWithFlags(&ev_fact, hipEventDisableTiming) != hipSuccess) return -4;
+    if (hipEventCreateWithFlags(&ev_bulk[0], hipEventDisableTiming) != hipSuccess) return -4;
+    if (hipEventCreateWithFlags(&ev_bulk[1], hipEventDisableTiming) != hipSuccess) return -4;
+    struct Cleanup {
+        hipEvent_t e[3];
+        ~Cleanup() {
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
+        }
+    } cl{{ev_fact, ev_bulk[0], ev_bulk[1]}};
+    bool any_bulk = false;
+#define MP_TRY(x)                           \
+    do {                                    \
+        if ((x) != hipSuccess) return -4;   \
+    } while (0)
+    for (int J = 0; J < nsc; ++J) {
+        const int j0 = J * sw, j1 = std::min(j0 + sw, bb);
+        // super-column J received bulk(J-2) on s1 (bulks run in order)
+        if (J >= 2) MP_TRY(hipStreamWaitEvent(s, ev_bulk[J & 1], 0));
+        for (int t = j0; t < j1; ++t) {
+            MP_TRY(factor_column(P, t * NB, (bb - t) * NB, NB, t * NB,
+                                 dinv_all + (size_t)t * inv_per_tile, dinfo, s));
+            if (t + 1 < j1) {
+                Gemm g;
+                g.A = P; g.A.r0 = (t + 1) * NB; g.A.c0 = t * NB;
+                g.B = g.A;
+                g.C = P; g.C.r0 = (t + 1) * NB; g.C.c0 = (t + 1) * NB;
+                g.M = (bb - t - 1) * NB; g.N = (j1 - t - 1) * NB; g.K = NB;
+                g.alpha = -1.0; g.beta = 1.0; g.lower = 1;
+                MP_TRY(launch_gemm(g, GEMM_UPDATE, s));
+            }
+        }
+        if (j1 >= bb) break;
+        const int nrt = bb - j1;
+        const int K = (j1 - j0) * NB;
+        const long long ldx = mode == 0 ? (long long)nrt * NB : (long long)K;
+        // lp[J & 1] was last read by bulk(J-2), finished above
+        MP_TRY(convert_super(W, NB, bb, j0, j1 - j0, j1, nrt, mode, amax, sc[J & 1], lp[J & 1], s));
+        // the next super-column also received bulk(J-1)
+        if (J >= 1) MP_TRY(hipStreamWaitEvent(s, ev_bulk[(J - 1) & 1], 0));
+        const int k1 = std::min(j1 + sw, bb);
+        MP_TRY(launch_update_lp(mode, W, bb, NB, j1 * NB, lp[J & 1], ldx, K, sc[J & 1],
+                                (bb - j1) * NB, (k1 - j1) * NB, rowmap, s));
+        if (k1 < bb) {
+            const long long roff = (long long)(k1 - j1) * NB;   // X row of global row k1*NB
+            const void *Xb = mode == 0 ? (const void *)((const float *)lp[J & 1] + roff)
+                                       : (const void *)((const unsigned short *)lp[J & 1] +
+                                                        roff * ldx);
+            MP_TRY(hipEventRecord(ev_fact, s));
+            MP_TRY(hipStreamWaitEvent(s1, ev_fact, 0));
+            MP_TRY(launch_update_lp(mode, W, bb, NB, k1 * NB, Xb, ldx, K, sc[J & 1] + (k1 - j1),
+                                    (bb - k1) * NB, (bb - k1) * NB, rowmap, s1));
+            MP_TRY(hipEventRecord(ev_bulk[J & 1], s1));
+            any_bulk = true;
+        }
+    }
+    if (any_bulk) {
+        MP_TRY(hipEventRecord(ev_fact, s1));
+        MP_TRY(hipStreamWaitEvent(s, ev_fact, 0));
+    }
+#undef MP_TRY
+    return 0;
 }
 
 }  // namespace ooc
@@ -358,7 +418,8 @@ hipError_t salloc(T **p, size_t bytes, hipStream_t s) {
 size_t mp_factor_bytes(int N, int NB) {
     const size_t bb = (size_t)((N + NB - 1) / NB);
     const size_t tiles = bb * (bb + 1) / 2 * NB * NB;
-    return (tiles + bb * NB * NB + bb * NB * 64) * 8 + 2 * bb * NB * NB * 4;
+    const size_t sw = (size_t)mp_super_width(NB, (int)bb);
+    return (tiles + bb * NB * NB + bb * NB * 64) * 8 + 2 * bb * NB * sw * NB * 4;
 }
 
 void mp_release(MpFactor &f, hipStream_t s) {
@@ -382,14 +443,18 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
     f.bytes = mp_factor_bytes(N, NB);
     void *lp = nullptr;
     float *scale = nullptr;
+    unsigned *amax = nullptr;
     int *info = nullptr;
     double *anorm = nullptr;
     auto ok = [](hipError_t e) { return e == hipSuccess; };
     if (!f.W && (!ok(salloc(&f.W, tiles * 8, s0)) || !ok(salloc(&f.Z, (size_t)bb * NB * NB * 8, s0)) ||
                  !ok(salloc(&f.dinv, (size_t)Np * 64 * 8, s0))))
         return OOCDAG_ENOMEM;
-    if (!ok(salloc(&lp, (size_t)2 * bb * NB * NB * 4, s0)) || !ok(salloc(&scale, (size_t)2 * bb * 4, s0)) ||
-        !ok(salloc(&info, sizeof(int), s0)) || !ok(salloc(&anorm, (size_t)Np * 8, s0)))
+    const size_t sw = (size_t)mp_super_width(NB, bb);
+    const size_t lp_bytes = precision > 0 ? 2 * (size_t)bb * NB * sw * NB * (precision == 1 ? 4 : 2) : 8;
+    if (!ok(salloc(&lp, lp_bytes, s0)) || !ok(salloc(&scale, (size_t)2 * bb * 4, s0)) ||
+        !ok(salloc(&amax, (size_t)bb * 4, s0)) || !ok(salloc(&info, sizeof(int), s0)) ||
+        !ok(salloc(&anorm, (size_t)Np * 8, s0)))
         return OOCDAG_ENOMEM;
     int rc = OOCDAG_OK;
     do {
@@ -405,7 +470,7 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
             rc = OOCDAG_EHIP;
             break;
         }
-        int r = factor_mixed(f.W, N, NB, precision, f.dinv, info, lp, scale, s0, s1);
+        int r = factor_mixed(f.W, N, NB, precision, f.dinv, info, lp, scale, amax, s0, s1);
         if (r) { rc = OOCDAG_EHIP; break; }
         if (!ok(tile_inverses(f.W, N, NB, f.dinv, f.Z, s0)) ||
             !ok(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s0)) ||
@@ -418,6 +483,7 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
     } while (0);
     (void)hipFreeAsync(lp, s0);
     (void)hipFreeAsync(scale, s0);
+    (void)hipFreeAsync(amax, s0);
     (void)hipFreeAsync(info, s0);
     (void)hipFreeAsync(anorm, s0);
     f.factor_seconds = now() - t0;

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 static double wall() {
@@ -59,12 +60,14 @@ int main(int argc, char **argv) {
     }
     const int LDA = N;
     const bool do_io = N <= 32;
-    std::vector<double> A((size_t)LDA * N);
+    // left uninitialized: the generator writes every element
+    std::unique_ptr<double[]> Abuf(new double[(size_t)LDA * N]);
+    double *Ap = Abuf.get();
 
     printf("Initialize A ... ");
     fflush(stdout);
     double t = wall();
-    int inf = oocdag_pdmatgen('S', 'D', N, N, NB, NB, A.data(), LDA, 100);
+    int inf = oocdag_pdmatgen('S', 'D', N, N, NB, NB, Ap, LDA, 100);
     if (inf) {
         printf("pdmatgen failed: %d %s\n", inf, oocdag_last_error());
         return 1;
@@ -72,7 +75,7 @@ int main(int argc, char **argv) {
     printf("time %lf\n", wall() - t);
     if (do_io) {
         printf("Original A=\n\n");
-        matprint(A.data(), N, LDA, 'A');
+        matprint(Ap, N, LDA, 'A');
     }
 
     int widths[4096];
@@ -88,7 +91,7 @@ int main(int argc, char **argv) {
 
     printf("LLT start %lf\n", wall());
     int info = 0;
-    double llttime = oocdag_cholesky(A.data(), N, NB, LDA, memsize, &info);
+    double llttime = oocdag_cholesky(Ap, N, NB, LDA, memsize, &info);
     printf("LLT end %lf\n", wall());
     if (llttime < 0 || info < 0) {
         printf("oocdag_cholesky failed: info %d %s\n", info, oocdag_last_error());
@@ -96,7 +99,7 @@ int main(int argc, char **argv) {
     }
     if (info > 0) printf("dpotrf info %d: leading minor not positive definite\n", info);
     printf("llt time %lf %lf\n", llttime, wall());
-    printf("%lf %lf\n", A[(size_t)(N - 1) * LDA + N - 1], wall());
+    printf("%lf %lf\n", Ap[(size_t)(N - 1) * LDA + N - 1], wall());
 
     double h2d = 0, d2h = 0, th = 0, td = 0;
     oocdag_last_stats(nullptr, &h2d, &d2h, &th, &td);
@@ -112,11 +115,11 @@ int main(int argc, char **argv) {
            d2h / 1e9, td, td > 0 ? d2h / td / 1e9 : 0.0);
     if (do_io) {
         printf("L:\n\n");
-        matprint(A.data(), N, LDA, 'L');
+        matprint(Ap, N, LDA, 'L');
     }
     if (resid) {
         double r = 0;
-        int rc = oocdag_llt_resid(A.data(), N, LDA, 'S', 'D', 100, &r);
+        int rc = oocdag_llt_resid(Ap, N, LDA, 'S', 'D', 100, &r);
         if (rc) {
             printf("residual failed: %s\n", oocdag_last_error());
             return 1;
@@ -129,7 +132,7 @@ int main(int argc, char **argv) {
         double tv = wall();
         std::vector<double> B((size_t)LDA * N);
         oocdag_pdmatgen('S', 'D', N, N, NB, NB, B.data(), LDA, 100);
-        int rc = oocdag_potrs(A.data(), N, LDA, B.data(), N, LDA);
+        int rc = oocdag_potrs(Ap, N, LDA, B.data(), N, LDA);
         if (rc) {
             printf("potrs failed: %s\n", oocdag_last_error());
             return 1;

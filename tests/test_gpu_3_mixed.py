@@ -40,11 +40,15 @@ def test_gram_generator_properties(gpu):
     assert np.array_equal(buf2.to_array((ooc.tiles_elems(n, nb),)), tiles)
 
 
+# (2048, 256): one super-column (all updates FP64 inside it);
+# (6144, 128): three super-columns of 16 tile columns -- narrow and bulk
+# low-precision updates, both buffer parities.
+@pytest.mark.parametrize("n,nb", [(2048, 256), (6144, 128)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
 @pytest.mark.parametrize("mu", [1e-2, 1e-6])
-def test_mpir_matches_fp64_oracle(gpu, prec, mu):
+def test_mpir_matches_fp64_oracle(gpu, prec, mu, n, nb):
     ooc = gpu
-    n, nb, nrhs = 2048, 256, 4
+    nrhs = 4
     buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
     ooc.gram_tiles(buf, n, nb, mu, seed=3)
     A = gram_host(ooc, buf, n, nb)
