@@ -464,6 +464,92 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     }
 }
 
+// ---- few right-hand sides -------------------------------------------------
+// C (M x N) = beta*C + alpha * A * B^T for the solves' right-hand-side
+// panels: A and C plain, B a packed (or plain) operand streamed once.  One
+// wave owns 16 rows m x 16 columns n; A and B go from global memory straight
+// into v_mfma_f64_16x16x4_f64 operand registers (A lane (m = l%16, k = l/16),
+// B lane (n = l%16, k = l/16)), no LDS and no barriers, and the K loop is
+// unrolled so each wave keeps many loads in flight: the kernel streams B
+// (the factor's tile column or tile row) at memory speed.
+struct RArgs {
+    Opnd A, B, C;
+    int M, N, K;
+    double alpha, beta;
+};
+
+__device__ __forceinline__ long long elem_off(const Opnd &o, int gr, int gc) {
+    if (o.ts <= 0) return gr + (long long)gc * o.ld;
+    const int ti = gr / o.ts, tk = gc / o.ts;
+    return tix(ti, tk, o.ym, o.tri) * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
+}
+
+template <int TB>
+__global__ __launch_bounds__(256) void k_rhs(RArgs a) {
+    constexpr int U = 16;   // K steps per batch; two batches in flight
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n0 = (blockIdx.x * 4 + wave) * 16;
+    const int m0 = blockIdx.y * 16;
+    if (n0 >= a.N) return;   // whole wave
+    const int il = lane & 15, kl = lane >> 4;
+    // lanes past M or N read a valid row/column instead: they only feed
+    // output rows/columns that are never stored
+    const int mi = min(m0 + il, a.M - 1), ni = min(n0 + il, a.N - 1);
+    const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kl);
+    const long long astep = 4 * a.A.ld;
+    const double *pb;
+    long long bstep;
+    if (TB == 0) {
+        pb = a.B.p + elem_off(a.B, a.B.r0 + ni, a.B.c0 + kl);
+        bstep = 4 * a.B.ld;
+    } else {
+        pb = a.B.p + elem_off(a.B, a.B.r0 + kl, a.B.c0 + ni);
+        bstep = 4;
+    }
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int nk = a.K / 4;
+    int t = 0;
+    if (nk >= U) {
+        double av[U], bv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            av[u] = pa[u * astep];
+            bv[u] = pb[u * bstep];
+        }
+        for (; t + U <= nk; t += U) {
+            double an[U], bq[U];
+            const bool more = t + 2 * U <= nk;
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    an[u] = pa[(t + U + u) * astep];
+                    bq[u] = pb[(t + U + u) * bstep];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    av[u] = an[u];
+                    bv[u] = bq[u];
+                }
+            }
+        }
+    }
+    for (; t < nk; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t * astep], pb[t * bstep], acc, 0, 0, 0);
+    // element r of a lane sits at row lane/16 + 4r, column lane%16
+    if (n0 + il >= a.N) return;
+    double *c = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + il);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (m0 + kl + 4 * r >= a.M) break;
+        double v = a.alpha * acc[r];
+        if (a.beta != 0.0) v += a.beta * c[4 * r];
+        c[4 * r] = v;
+    }
+}
+
 // The block's rows of o (starting at a multiple of X past o.r0) stay in one
 // tile: tiles and the start row are multiples of the block height.
 static bool block_aligned(const Opnd &o, int X) {
@@ -508,6 +594,32 @@ hipError_t launch_one(const Gemm &g, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
+    if (g.A.ts > 0 || g.C.ts > 0 || (g.K & 3)) return hipErrorInvalidValue;
+    const Opnd &B = g.B;
+    if (B.ts > 0) {
+        // each 16-column group and the whole K range inside one tile
+        if (B.ts % 16) return hipErrorInvalidValue;
+        if (!tb && ((B.r0 % 16) || (B.c0 % B.ts) + g.K > B.ts)) return hipErrorInvalidValue;
+        if (tb && ((B.c0 % 16) || (B.r0 % B.ts) + g.K > B.ts)) return hipErrorInvalidValue;
+    }
+    RArgs a;
+    a.A = g.A;
+    a.B = g.B;
+    a.C = g.C;
+    a.M = g.M;
+    a.N = g.N;
+    a.K = g.K;
+    a.alpha = g.alpha;
+    a.beta = g.beta;
+    dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 15) / 16));
+    if (tb)
+        hipLaunchKernelGGL(k_rhs<1>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_rhs<0>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
@@ -521,7 +633,7 @@ hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
         if (((uintptr_t)o->p) & 15) return hipErrorInvalidValue;
     }
     const bool tb = kind == GEMM_GENERAL_TB || kind == GEMM_NARROW_TB || kind == GEMM_APPLY_TB ||
-                    kind == GEMM_SKINNY_TB;
+                    kind == GEMM_SKINNY_TB || kind == GEMM_RHS_TB;
     if ((g.A.ts > 0 && (g.A.c0 % BK))) return hipErrorInvalidValue;
     if (!tb && g.B.ts > 0 && (g.B.c0 % BK)) return hipErrorInvalidValue;
     if (tb && g.B.ts > 0 && (g.B.r0 % BK)) return hipErrorInvalidValue;
@@ -546,6 +658,10 @@ hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
         return launch_one<128, 0, 7, 0, 32>(g, s);
     case GEMM_SKINNY_TB:
         return launch_one<128, 0, 8, 1, 32>(g, s);
+    case GEMM_RHS:
+        return launch_rhs(g, 0, s);
+    case GEMM_RHS_TB:
+        return launch_rhs(g, 1, s);
     case GEMM_GENERAL:
     default:
         return launch_one<128, 0, 3>(g, s);

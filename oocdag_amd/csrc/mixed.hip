@@ -190,7 +190,7 @@ hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt,
         g.C = plain(Yt, ld, 0, 0);
         g.M = nrhs; g.N = (i + 1) * NB; g.K = NB;
         g.alpha = alpha; g.beta = 1.0; g.lower = 0;
-        if ((e = launch_gemm(g, nrhs <= 32 ? GEMM_SKINNY_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
+        if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
         if (i + 1 < bb) {
             Gemm h;
             h.A = plain((double *)Xt, ld, 0, i * NB);
@@ -198,7 +198,7 @@ hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt,
             h.C = plain(Yt, ld, 0, (i + 1) * NB);
             h.M = nrhs; h.N = (bb - i - 1) * NB; h.K = NB;
             h.alpha = alpha; h.beta = 1.0; h.lower = 0;
-            if ((e = launch_gemm(h, nrhs <= 32 ? GEMM_SKINNY : GEMM_GENERAL, s)) != hipSuccess) return e;
+            if ((e = launch_gemm(h, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL, s)) != hipSuccess) return e;
         }
     }
     return hipSuccess;
@@ -253,7 +253,7 @@ hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double 
                                 hipMemcpyDeviceToDevice, s);
     };
     for (int k = 0; k < bb; ++k) {  // Y^T_k = B^T_k L_kk^-T = B^T_k Z_k
-        if ((e = diag(k, nrhs <= 32 ? GEMM_SKINNY_TB : GEMM_GENERAL_TB)) != hipSuccess) return e;
+        if ((e = diag(k, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB)) != hipSuccess) return e;
         if (k + 1 < bb) {
             Gemm g;
             g.A = plain(Xt, ldx, 0, k * NB);
@@ -261,11 +261,11 @@ hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double 
             g.C = plain(Xt, ldx, 0, (k + 1) * NB);
             g.M = nrhs; g.N = (bb - k - 1) * NB; g.K = NB;
             g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
-            if ((e = launch_gemm(g, nrhs <= 32 ? GEMM_SKINNY : GEMM_GENERAL, s)) != hipSuccess) return e;
+            if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL, s)) != hipSuccess) return e;
         }
     }
     for (int k = bb - 1; k >= 0; --k) {  // X^T_k = Y^T_k L_kk^-1 = Y^T_k Z_k^T
-        if ((e = diag(k, nrhs <= 32 ? GEMM_SKINNY : GEMM_GENERAL)) != hipSuccess) return e;
+        if ((e = diag(k, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL)) != hipSuccess) return e;
         if (k > 0) {
             Gemm g;
             g.A = plain(Xt, ldx, 0, k * NB);
@@ -273,7 +273,7 @@ hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double 
             g.C = plain(Xt, ldx, 0, 0);
             g.M = nrhs; g.N = k * NB; g.K = NB;
             g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
-            if ((e = launch_gemm(g, nrhs <= 32 ? GEMM_SKINNY_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
+            if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
         }
     }
     return hipSuccess;

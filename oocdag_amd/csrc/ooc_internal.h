@@ -69,7 +69,12 @@ enum GemmKind {
     GEMM_APPLY_TB = 6,
     // 32 x 128 blocks for M <= 32 (a few right-hand sides in the solves)
     GEMM_SKINNY = 7,
-    GEMM_SKINNY_TB = 8
+    GEMM_SKINNY_TB = 8,
+    // right-hand-side panels: A and C plain, any M (16-row groups), one
+    // wave per 16 columns, operands streamed straight into MFMA registers;
+    // a tiled B must keep each 16-column group and the K range in one tile
+    GEMM_RHS = 9,
+    GEMM_RHS_TB = 10
 };
 
 hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s);

@@ -281,7 +281,7 @@ hipError_t potrs_tiles_t(const double *dY, int N, int NB, const double *dinv_all
             g.C = plain(Xt, ldx, 0, (k + 1) * NB);
             g.M = nrhs; g.N = (bb - k - 1) * NB; g.K = NB;
             g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
-            if ((e = launch_gemm(g, nrhs <= 32 ? GEMM_SKINNY : GEMM_GENERAL, s)) != hipSuccess) return e;
+            if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL, s)) != hipSuccess) return e;
         }
     }
     // backward: X^T = Y^T L^-1
@@ -297,7 +297,7 @@ hipError_t potrs_tiles_t(const double *dY, int N, int NB, const double *dinv_all
             g.C = plain(Xt, ldx, 0, 0);
             g.M = nrhs; g.N = k * NB; g.K = NB;
             g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
-            if ((e = launch_gemm(g, nrhs <= 32 ? GEMM_SKINNY_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
+            if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
         }
     }
     return hipSuccess;
