@@ -274,13 +274,19 @@ struct StreamSel<BX, 1, FULL> {
     typedef StreamT<BX> type;
 };
 
-// Linear block id -> (mb, nb).  MODE 0: rectangular grid, 8-row groups,
-// column-major inside a group.  MODE 1: lower trapezoid (mb >= nb),
-// column-major.
+// Blocks in rows [0, R) of a lower trapezoid with Nb block columns.
+__device__ __forceinline__ long long trap_rows(long long R, int Nb) {
+    return R <= Nb ? R * (R + 1) / 2 : (long long)Nb * (Nb + 1) / 2 + (R - Nb) * Nb;
+}
+
+// Linear block id -> (mb, nb), in groups of GM block rows, column-major
+// inside a group, so the blocks an XCD runs at once form a GM x GM patch
+// that shares GM operand strips of each kind in its L2.
+// MODE 0: rectangular grid.  MODE 1: lower trapezoid (mb >= nb).
 template <int MODE>
 __device__ __forceinline__ void decode_block(long long L, int Mb, int Nb, int &mb, int &nb) {
+    const int GM = 8;
     if (MODE == 0) {
-        const int GM = 8;
         long long per_group = (long long)GM * Nb;
         int g = (int)(L / per_group);
         int first = g * GM;
@@ -289,16 +295,28 @@ __device__ __forceinline__ void decode_block(long long L, int Mb, int Nb, int &m
         mb = first + w % gsz;
         nb = w / gsz;
     } else {
-        // column n holds Mb - n blocks; S(n) = n*Mb - n(n-1)/2 blocks precede it
-        double a = (double)Mb + 0.5;
-        double disc = a * a - 2.0 * (double)L;
-        int n = (int)floor(a - sqrt(fmax(disc, 0.0)));
-        n = max(0, min(n, Nb - 1));
-        auto S = [&](int c) { return (long long)c * Mb - ((long long)c * (c - 1)) / 2; };
-        while (n > 0 && S(n) > L) --n;
-        while (n + 1 < Nb && S(n + 1) <= L) ++n;
-        nb = n;
-        mb = n + (int)(L - S(n));
+        // group g holds rows [g GM, g GM + gs); trap_rows(g GM) blocks precede it
+        int lo = 0, hi = (Mb + GM - 1) / GM - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (trap_rows((long long)mid * GM, Nb) <= L) lo = mid; else hi = mid - 1;
+        }
+        const int R = lo * GM, gs = min(GM, Mb - R);
+        long long w = L - trap_rows(R, Nb);
+        const int n1 = min(R + 1, Nb);   // columns 0..n1-1: all gs rows of the group
+        if (w < (long long)n1 * gs) {
+            nb = (int)(w / gs);
+            mb = R + (int)(w - (long long)nb * gs);
+        } else {
+            w -= (long long)n1 * gs;     // column n > R: rows n .. R+gs-1
+            int n = n1;
+            while (w >= R + gs - n) {
+                w -= R + gs - n;
+                ++n;
+            }
+            nb = n;
+            mb = n + (int)w;
+        }
     }
 }
 

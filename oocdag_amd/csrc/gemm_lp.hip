@@ -47,16 +47,36 @@ __device__ __forceinline__ long long xcd_remap(long long bid, long long G) {
     return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
+// Blocks in rows [0, R) of a lower trapezoid with Nb block columns.
+__device__ __forceinline__ long long trap_rows(long long R, int Nb) {
+    return R <= Nb ? R * (R + 1) / 2 : (long long)Nb * (Nb + 1) / 2 + (R - Nb) * Nb;
+}
+
+// Lower-trapezoid block id -> (mb, nb) in groups of 8 block rows, column-
+// major inside a group (the blocks an XCD runs at once share operand strips).
 __device__ __forceinline__ void tri_decode(long long L, int Mb, int Nb, int &mb, int &nb) {
-    double a = (double)Mb + 0.5;
-    double disc = a * a - 2.0 * (double)L;
-    int n = (int)floor(a - sqrt(fmax(disc, 0.0)));
-    n = max(0, min(n, Nb - 1));
-    auto S = [&](int c) { return (long long)c * Mb - ((long long)c * (c - 1)) / 2; };
-    while (n > 0 && S(n) > L) --n;
-    while (n + 1 < Nb && S(n + 1) <= L) ++n;
-    nb = n;
-    mb = n + (int)(L - S(n));
+    const int GM = 8;
+    int lo = 0, hi = (Mb + GM - 1) / GM - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (trap_rows((long long)mid * GM, Nb) <= L) lo = mid; else hi = mid - 1;
+    }
+    const int R = lo * GM, gs = min(GM, Mb - R);
+    long long w = L - trap_rows(R, Nb);
+    const int n1 = min(R + 1, Nb);
+    if (w < (long long)n1 * gs) {
+        nb = (int)(w / gs);
+        mb = R + (int)(w - (long long)nb * gs);
+    } else {
+        w -= (long long)n1 * gs;
+        int n = n1;
+        while (w >= R + gs - n) {
+            w -= R + gs - n;
+            ++n;
+        }
+        nb = n;
+        mb = n + (int)w;
+    }
 }
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
