@@ -75,6 +75,43 @@ def cpu_baseline(n: int, nb: int) -> dict:
                       f"{threads} threads, {kern}, {t:.2f} s"}
 
 
+def update_algorithmic_bytes(n: int, nb: int) -> int:
+    """Average algorithmic HBM bytes per update-GEMM launch of the look-ahead
+    factorization: each launch reads its lower trapezoid C once, writes it
+    once and reads the K = nb wide panel column once (B is rows of that same
+    column); 16 B per C element + 8 B per panel element."""
+    bb = -(-n // nb)
+    tot = cnt = 0
+    for j in range(bb - 1):
+        m = (bb - j - 1) * nb
+        tot += 16 * (m * nb - nb * (nb - 1) // 2) + 8 * m * nb
+        cnt += 1
+        if j + 2 < bb:
+            m = (bb - j - 2) * nb
+            tot += 16 * (m * (m + 1) // 2) + 8 * m * nb
+            cnt += 1
+    return tot // max(cnt, 1)
+
+
+def measured_traffic(n: int, nb: int):
+    """HBM bytes per update-GEMM launch from the committed PMC summary of the
+    same workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    reading of a copy kernel of known size); None when no summary covers this
+    (n, nb)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"update_gemm_n{n}_pmc_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    fields = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+                    if int(fields["nb"]) == nb:
+                        return int(fields["bytes"]), os.path.relpath(path, os.path.dirname(path) + "/..")
+    except (OSError, KeyError, ValueError):
+        pass
+    return None, None
+
+
 def bench_mpir(ooc, n: int, nb: int, nrhs: int, mus, precs) -> dict:
     """Mixed-precision + IR workload (n=65536, nb=512, 16 right-hand sides,
     A = G G^T / n + mu I): time to FP64-level backward error per precision,
@@ -339,6 +376,7 @@ def main() -> int:
     t = sum(times) / len(times)
     value = flops / t / 1e12
     ach = prof_f / max(prof_t, 1e-12) / 1e12
+    traffic, traffic_note = measured_traffic(n, nb)
     resid = None if args.no_verify else verify_small(ooc)
     rec = {
         "metric": "Cholesky TFLOP/s (n^3/3) vs FP64 peak",
@@ -357,7 +395,9 @@ def main() -> int:
                    "n": n, "nb": nb, "fraction_of_fp64_peak": round(value / FP64_PEAK, 4),
                    "small_check_scaled_residual_n4096": resid},
         "roofline": {"bound": "mfma", "achieved": round(ach, 4), "peak": FP64_PEAK,
-                     "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK, 4), "traffic": traffic,
+                     "traffic_source": traffic_note,
+                     "algorithmic_bytes_per_launch": update_algorithmic_bytes(n, nb),
                      "kernel": "k_dgemm_nt<128,1,0> panel update",
                      "launches": prof_n, "kernel_seconds": round(prof_t, 4)},
     }
