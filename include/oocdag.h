@@ -48,6 +48,33 @@ extern "C" {
  * ------------------------------------------------------------------------- */
 double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *info);
 
+/* ---------------------------------------------------------------------------
+ * Factorization, host packed tiles in / out (out of core).
+ * Replaces: the same Cholesky() panel loop (oocdag.cpp:18-112; staging
+ *   A2Y/A2X/D2H oocquark.cpp:16-63,253-319) for a caller that keeps A on the
+ *   host as packed lower tiles -- tile (i,j), i >= j, NB x NB column-major at
+ *   element offset (i + j*bb - j(j+1)/2)*NB*NB, bb = ceil(N/NB), padded past N
+ *   with zeros and a unit diagonal (the device layout; oocdag_tiles_elems
+ *   gives the size).  Every tile-column copy is then one contiguous DMA and
+ *   nothing is packed on the host: T in pinned memory (oocdag_host_alloc) is
+ *   copied directly; pageable T is staged through the library's pinned
+ *   double-buffered slots with a parallel memcpy.  Same panel schedule,
+ *   budget, return value and *info as oocdag_cholesky; the tiles are
+ *   overwritten with L.
+ * ------------------------------------------------------------------------- */
+double oocdag_cholesky_host_tiles(double *T, int N, int NB, size_t memsize, int *info);
+
+/* PDMATGEN('S'/'H'/'F', DIAG, ISEED) straight into host packed lower tiles
+ * (the layout above), generated on the GPU tile column group by group and
+ * copied down; values bit-identical to oocdag_pdmatgen.
+ * Replaces: pdmatgen (oocdag.h:85-90) for the host-tiles entry point. */
+int oocdag_pdmatgen_host_tiles(double *T, int N, int NB, char aform, char diag, int iseed);
+
+/* Pinned (page-locked) host memory for the host-tiles entry points:
+ * hipHostMalloc / hipHostFree.  NULL on failure (oocdag_last_error). */
+void *oocdag_host_alloc(size_t bytes);
+int oocdag_host_free(void *p);
+
 /* Panel widths (tile columns per Y panel) that oocdag_cholesky will use for
  * (N, NB, memsize): widths[0..return-1].  Returns the panel count, or < 0. */
 int oocdag_panel_plan(int N, int NB, size_t memsize, int *widths, int maxp);

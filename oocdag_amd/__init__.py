@@ -29,6 +29,11 @@ _lib = None
 # name -> (restype, argtypes); every symbol declared in include/oocdag.h
 SIGNATURES = {
     "oocdag_cholesky": (C.c_double, [_dp, C.c_int, C.c_int, C.c_int, C.c_size_t, _ip]),
+    "oocdag_cholesky_host_tiles": (C.c_double, [C.c_void_p, C.c_int, C.c_int, C.c_size_t, _ip]),
+    "oocdag_pdmatgen_host_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char, C.c_char,
+                                             C.c_int]),
+    "oocdag_host_alloc": (C.c_void_p, [C.c_size_t]),
+    "oocdag_host_free": (C.c_int, [C.c_void_p]),
     "oocdag_panel_plan": (C.c_int, [C.c_int, C.c_int, C.c_size_t, _ip, C.c_int]),
     "oocdag_panel_plan_nx": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.c_int, _ip, C.c_int]),
     "oocdag_pdmatgen": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
@@ -187,6 +192,72 @@ def cholesky(A: np.ndarray, nb: int, memsize: int):
     if info.value < 0:
         raise OocdagError(f"oocdag_cholesky failed ({info.value}): {last_error()}")
     return t, info.value
+
+
+class HostTiles:
+    """Packed lower tiles of an n x n matrix in pinned host memory (the
+    layout of the device tiles): the host side of
+    :func:`cholesky_host_tiles`."""
+
+    def __init__(self, n: int, nb: int, pinned: bool | None = True):
+        """pinned=True: page-locked (error if refused); False: pageable;
+        None: page-locked if the system grants it, else pageable."""
+        self.n, self.nb = n, nb
+        self.elems = tiles_elems(n, nb)
+        self._np = None
+        self.ptr = lib().oocdag_host_alloc(self.elems * 8) if pinned is not False else None
+        self.pinned = bool(self.ptr)
+        if not self.ptr:
+            if pinned:
+                raise OocdagError(f"pinned host allocation failed: {last_error()}")
+            self._np = np.empty(self.elems, dtype=np.float64)
+            self.ptr = self._np.ctypes.data
+
+    def array(self) -> np.ndarray:
+        """A numpy view of the tiles (no copy; valid until :meth:`free`)."""
+        if self._np is not None:
+            return self._np
+        buf = (C.c_double * self.elems).from_address(self.ptr)
+        return np.ctypeslib.as_array(buf)
+
+    def generate(self, aform: str = "S", diag: str = "D", iseed: int = 100) -> None:
+        _check(lib().oocdag_pdmatgen_host_tiles(self.ptr, self.n, self.nb, aform.encode(),
+                                                diag.encode(), iseed), "pdmatgen_host_tiles")
+
+    def free(self) -> None:
+        if self.ptr and self._np is None:
+            lib().oocdag_host_free(self.ptr)
+        self.ptr = None
+        self._np = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def cholesky_host_tiles(t: "HostTiles", memsize: int):
+    """Out-of-core factorization of host packed tiles in place
+    (Cholesky(), oocdag.cpp:18, on the tile layout).  Returns (seconds, info)."""
+    info = C.c_int(0)
+    secs = lib().oocdag_cholesky_host_tiles(t.ptr, t.n, t.nb, memsize, C.byref(info))
+    if info.value < 0:
+        raise OocdagError(f"oocdag_cholesky_host_tiles failed ({info.value}): {last_error()}")
+    return secs, info.value
+
+
+def tiles_to_dense(tiles: np.ndarray, n: int, nb: int) -> np.ndarray:
+    """Lower triangle of the matrix held in packed tiles, as a dense array."""
+    bb = -(-n // nb)
+    out = np.zeros((bb * nb, bb * nb), order="F")
+    k = 0
+    for j in range(bb):
+        for i in range(j, bb):
+            out[i * nb:(i + 1) * nb, j * nb:(j + 1) * nb] = \
+                tiles[k * nb * nb:(k + 1) * nb * nb].reshape(nb, nb, order="F")
+            k += 1
+    return np.tril(out[:n, :n])
 
 
 def panel_plan(n: int, nb: int, memsize: int, nx: int | None = None):

@@ -222,6 +222,26 @@ void unpack_column(double *A, int N, int NB, int LDA, int tj, int ti0, const dou
     }
 }
 
+// Host copy split across the OpenMP threads (staging of host tiles that are
+// not page-locked).
+void par_memcpy(double *dst, const double *src, size_t n) {
+    const size_t chunk = (size_t)1 << 20;   // doubles
+    const long long nch = (long long)((n + chunk - 1) / chunk);
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < nch; ++c) {
+        const size_t o = (size_t)c * chunk;
+        memcpy(dst + o, src + o, std::min(chunk, n - o) * sizeof(double));
+    }
+}
+
+// True when p is page-locked host memory known to HIP.
+bool host_pinned(const void *p) {
+    hipPointerAttribute_t at;
+    const hipError_t e = hipPointerGetAttributes(&at, p);
+    (void)hipGetLastError();   // an unknown pointer is an error here, not a failure
+    return e == hipSuccess && at.type == hipMemoryTypeHost;
+}
+
 struct Streams {
     hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr, bulk = nullptr;
     bool own_comp = true;
@@ -346,7 +366,12 @@ struct Scratch {
     }
 };
 
-int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, double *secs) {
+// Out-of-core factorization.  Host side: the column-major array A (LDA) of
+// the reference entry point, packed through pinned slots, or -- tiles_host --
+// A already holds the packed lower tiles (the device layout; pinned memory
+// for full-speed DMA), so every tile-column copy is one contiguous DMA.
+int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, double *secs,
+                 bool tiles_host = false) {
     Plan pl;
     if (!make_plan(N, NB, memsize, pl)) {
         set_err("budget %zu bytes below two tile columns for N=%d NB=%d", memsize, N, NB);
@@ -361,6 +386,9 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     HIPTRY(hipStreamCreateWithFlags(&st.comp, hipStreamNonBlocking));
     HIPTRY(hipStreamCreateWithFlags(&st.d2h, hipStreamNonBlocking));
     HIPTRY(hipStreamCreateWithFlags(&st.bulk, hipStreamNonBlocking));
+    // host tiles in page-locked memory are copied directly; otherwise (and
+    // for the column-major array) through pinned slots
+    const bool direct = tiles_host && host_pinned(A);
 
     double *dY = nullptr;
     if (hipMalloc(&dY, (size_t)pl.tiles_needed * nb2 * sizeof(double)) != hipSuccess) {
@@ -384,7 +412,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
                 if (s[i]) (void)hipHostFree(s[i]);
         }
     } fh{slot};
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4 && !direct; ++i)
         if (hipHostMalloc(&slot[i], col_bytes, hipHostMallocDefault) != hipSuccess) {
             set_err("pinned host allocation of %zu bytes failed", col_bytes);
             return OOCDAG_ENOMEM;
@@ -417,15 +445,24 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
         const int Ym = bb - (int)jb;
         // --- A2Y (oocquark.cpp:253-275): the panel's tile columns -> Y
         for (int jj = 0; jj < Yn; ++jj) {
-            if ((rc = take_slot(hslot))) return rc;
             const int tj = (int)jb + jj;
-            pack_column(A, N, NB, LDA, tj, tj, slot[hslot]);
             size_t bytes = (size_t)(bb - tj) * nb2 * sizeof(double);
             double *dst = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
-            HIPTRY(t_h2d.copy(dst, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
-            HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
+            if (direct) {
+                HIPTRY(t_h2d.copy(dst, A + (size_t)tile_index(tj, tj, bb, 1) * nb2, bytes,
+                                  hipMemcpyHostToDevice, st.h2d));
+            } else {
+                if ((rc = take_slot(hslot))) return rc;
+                if (tiles_host)
+                    par_memcpy(slot[hslot], A + (size_t)tile_index(tj, tj, bb, 1) * nb2,
+                               bytes / sizeof(double));
+                else
+                    pack_column(A, N, NB, LDA, tj, tj, slot[hslot]);
+                HIPTRY(t_h2d.copy(dst, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
+                HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
+                hslot ^= 1;
+            }
             h2d_bytes += bytes;
-            hslot ^= 1;
         }
         HIPTRY(hipEventRecord(ev_y, st.h2d));
         HIPTRY(hipStreamWaitEvent(st.comp, ev_y, 0));
@@ -436,15 +473,24 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             const int b = jjb % nxb;
             double *X = dY + (size_t)(ytiles + (long long)b * Ym) * nb2;
             if (jjb >= nxb) HIPTRY(hipStreamWaitEvent(st.h2d, ev_xfree[b], 0));
-            if ((rc = take_slot(hslot))) return rc;
             // A2X (oocquark.cpp:301-319): rows jb.. of tile column jjb
-            pack_column(A, N, NB, LDA, jjb, (int)jb, slot[hslot]);
             size_t bytes = (size_t)Ym * nb2 * sizeof(double);
-            HIPTRY(t_h2d.copy(X, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
-            HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
+            if (direct) {
+                HIPTRY(t_h2d.copy(X, A + (size_t)tile_index((int)jb, jjb, bb, 1) * nb2, bytes,
+                                  hipMemcpyHostToDevice, st.h2d));
+            } else {
+                if ((rc = take_slot(hslot))) return rc;
+                if (tiles_host)
+                    par_memcpy(slot[hslot], A + (size_t)tile_index((int)jb, jjb, bb, 1) * nb2,
+                               bytes / sizeof(double));
+                else
+                    pack_column(A, N, NB, LDA, jjb, (int)jb, slot[hslot]);
+                HIPTRY(t_h2d.copy(X, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
+                HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
+                hslot ^= 1;
+            }
             HIPTRY(hipEventRecord(ev_xready[b], st.h2d));
             h2d_bytes += bytes;
-            hslot ^= 1;
             HIPTRY(hipStreamWaitEvent(st.comp, ev_xready[b], 0));
             // ooc_syrk(X, Y, YM, YN, NB) (oocquark.cpp:321-356) as one launch
             Gemm g;
@@ -470,12 +516,28 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             pending.erase(pending.begin());
             int s = 2 + (jj & 1);
             HIPTRY(hipEventSynchronize(ev_slot[s]));
-            unpack_column(A, N, NB, LDA, (int)jb + jj, (int)jb + jj, slot[s]);
+            if (tiles_host)
+                par_memcpy(A + (size_t)tile_index((int)jb + jj, (int)jb + jj, bb, 1) * nb2, slot[s],
+                           (size_t)(Ym - jj) * nb2);
+            else
+                unpack_column(A, N, NB, LDA, (int)jb + jj, (int)jb + jj, slot[s]);
             slot_used[s] = false;
             return OOCDAG_OK;
         };
         rc = factor_panel(dY, NB, Ym, Yn, (int)jb * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
                           [&](int jj) -> int {
+                              if (direct) {
+                                  // finished column straight back into the host tiles
+                                  HIPTRY(hipEventRecord(ev_col, st.comp));
+                                  HIPTRY(hipStreamWaitEvent(st.d2h, ev_col, 0));
+                                  const int tj = (int)jb + jj;
+                                  size_t bytes = (size_t)(Ym - jj) * nb2 * sizeof(double);
+                                  const double *src = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
+                                  HIPTRY(t_d2h.copy(A + (size_t)tile_index(tj, tj, bb, 1) * nb2, src,
+                                                    bytes, hipMemcpyDeviceToHost, st.d2h));
+                                  d2h_bytes += bytes;
+                                  return OOCDAG_OK;
+                              }
                               int s = 2 + (jj & 1);
                               if (slot_used[s]) {
                                   int r = drain_one();
@@ -642,6 +704,102 @@ double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *i
         return -1.0;
     }
     return secs;
+}
+
+double oocdag_cholesky_host_tiles(double *T, int N, int NB, size_t memsize, int *info) {
+    g_err.clear();
+    for (double &s : g_stat) s = 0;
+    int dummy;
+    if (!info) info = &dummy;
+    *info = 0;
+    if (!T || N < 0 || check_nb(N, NB)) {
+        if (g_err.empty()) set_err("invalid argument");
+        *info = OOCDAG_EARG;
+        return -1.0;
+    }
+    if (N == 0) return 0.0;
+    if (!device_ok()) {
+        *info = OOCDAG_ENODEV;
+        return -1.0;
+    }
+    double secs = 0;
+    int rc = cholesky_ooc(T, N, NB, N, memsize, info, &secs, true);
+    if (rc) {
+        *info = rc;
+        return -1.0;
+    }
+    return secs;
+}
+
+int oocdag_pdmatgen_host_tiles(double *T, int N, int NB, char aform, char diag, int iseed) {
+    g_err.clear();
+    if (!T || N <= 0 || NB <= 0) return OOCDAG_EARG;
+    char af = (char)(aform & ~0x20), dg = (char)(diag & ~0x20);
+    if (!(af == 'S' || af == 'H' || af == 'F') || !(dg == 'D' || dg == 'N')) {
+        set_err("tile generator supports AFORM S/H/F and DIAG D/N");
+        return OOCDAG_EARG;
+    }
+    if (!device_ok()) return OOCDAG_ENODEV;
+    const int bb = (N + NB - 1) / NB;
+    const size_t nb2 = (size_t)NB * NB;
+    // groups of tile columns up to ~4 GB, generated on the device into two
+    // alternating windows and copied down while the next group is generated
+    const size_t cap = std::max<size_t>((size_t)bb * nb2, ((size_t)4 << 30) / sizeof(double));
+    double *win[2] = {nullptr, nullptr};
+    hipStream_t s[2] = {nullptr, nullptr};
+    struct Free {
+        double **w;
+        hipStream_t *s;
+        ~Free() {
+            for (int i = 0; i < 2; ++i) {
+                if (s[i]) (void)hipStreamSynchronize(s[i]);
+                if (w[i]) (void)hipFree(w[i]);
+                if (s[i]) (void)hipStreamDestroy(s[i]);
+            }
+        }
+    } fr{win, s};
+    for (int i = 0; i < 2; ++i) {
+        HIPTRY(hipStreamCreateWithFlags(&s[i], hipStreamNonBlocking));
+        if (hipMalloc(&win[i], cap * sizeof(double)) != hipSuccess) {
+            set_err("device allocation of %zu bytes failed", cap * sizeof(double));
+            return OOCDAG_ENOMEM;
+        }
+    }
+    int k = 0;
+    for (int j0 = 0; j0 < bb; ++k) {
+        int j1 = j0;
+        size_t tiles = 0;
+        while (j1 < bb && (tiles + (size_t)(bb - j1)) * nb2 <= cap && j1 - j0 < 16) {
+            tiles += (size_t)(bb - j1);
+            ++j1;
+        }
+        const long long t0 = tile_index(j0, j0, bb, 1);
+        hipStream_t sk = s[k & 1];
+        HIPTRY(launch_gen_tiles(win[k & 1], N, NB, bb, j0, j1, aform, diag, iseed, sk, t0));
+        HIPTRY(hipMemcpyAsync(T + (size_t)t0 * nb2, win[k & 1], tiles * nb2 * sizeof(double),
+                              hipMemcpyDeviceToHost, sk));
+        j0 = j1;
+    }
+    HIPTRY(hipStreamSynchronize(s[0]));
+    HIPTRY(hipStreamSynchronize(s[1]));
+    return OOCDAG_OK;
+}
+
+void *oocdag_host_alloc(size_t bytes) {
+    g_err.clear();
+    void *p = nullptr;
+    if (bytes == 0) return nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();   // a refused request must not fail the next launch check
+        set_err("pinned host allocation of %zu bytes failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+int oocdag_host_free(void *p) {
+    if (p) HIPTRY(hipHostFree(p));
+    return OOCDAG_OK;
 }
 
 size_t oocdag_tiles_elems(int N, int NB) {

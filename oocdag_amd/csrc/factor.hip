@@ -184,6 +184,7 @@ struct GenArgs {
     int diagD;
     long long ntiles;  // tiles in the column range
     int runs;          // column runs per tile
+    long long tbase;   // packed index of the tile at Y (a window of the layout)
 };
 
 constexpr int GEN_RUN = 32;
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(256) void k_gen_tiles(GenArgs g) {
     int r = chunk * 256 + threadIdx.x;
     if (r >= g.nb) return;
     long long tidx = ti + (long long)tj * g.bb - ((long long)tj * (tj + 1)) / 2;
-    double *tile = g.Y + tidx * (long long)g.nb * g.nb;
+    double *tile = g.Y + (tidx - g.tbase) * (long long)g.nb * g.nb;
     const int gi = ti * g.nb + r;
     const int cbeg = run * GEN_RUN;
     const int cend = min(cbeg + GEN_RUN, g.nb);
@@ -375,9 +376,10 @@ hipError_t launch_pad_tile(double *tile, int nb, int vr, int vc, int diag, hipSt
 }
 
 hipError_t launch_gen_tiles(double *Y, int N, int nb, int bb, int jt0, int jt1, char aform,
-                            char diag, int iseed, hipStream_t s) {
+                            char diag, int iseed, hipStream_t s, long long tbase) {
     if (jt1 <= jt0) return hipSuccess;
     GenArgs g;
+    g.tbase = tbase;
     for (int k = 0; k < 40; ++k) g.pow2[k] = lcg_pow(1ull << k);
     g.stepN = lcg_pow((unsigned long long)N);
     g.x1 = (1103515245u * ((unsigned)iseed & 0x7fffffffu) + 12345u) & 0x7fffffffu;

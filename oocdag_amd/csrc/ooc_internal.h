@@ -128,9 +128,11 @@ hipError_t launch_pad_tile(double *tile, int nb, int vr, int vc, int diag, hipSt
 
 // PDMATGEN('S'/'H' or 'F', DIAG, ISEED) values written straight into the
 // packed lower tiles (tri = 1, ym = bb) of tile columns [jt0, jt1), padded
-// as launch_pad_tile does beyond N.
+// as launch_pad_tile does beyond N.  Y points at the tile with packed index
+// tbase (0: the whole layout; else a window holding those columns only).
 hipError_t launch_gen_tiles(double *Y, int N, int nb, int bb, int jt0, int jt1, char aform,
-                            char diag, int iseed, hipStream_t s);
+                            char diag, int iseed, hipStream_t s,
+                            long long tbase = 0);
 
 // ---- solves and residual (solve.hip) ----
 // 64x64 inverses of the diagonal blocks of the n x n lower triangle L

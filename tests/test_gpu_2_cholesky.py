@@ -42,6 +42,51 @@ def test_cholesky_host_ooc_matches_oracle(gpu, ora, n, nb, budget_tiles):
     assert A[n - 1, n - 1] == pytest.approx(Lref[n - 1, n - 1], rel=1e-12)
 
 
+@pytest.mark.parametrize("n,nb", [(1000, 128), (1536, 256)])
+def test_host_tiles_generator_matches_device(gpu, n, nb):
+    """PDMATGEN into pinned host tiles == the device tile generator (bit-exact,
+    padding included)."""
+    ooc = gpu
+    t = ooc.HostTiles(n, nb)
+    try:
+        t.generate()
+        dev = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+        ooc.pdmatgen_tiles(dev, n, nb)
+        assert np.array_equal(t.array(), dev.to_array((ooc.tiles_elems(n, nb),)))
+    finally:
+        t.free()
+
+
+@pytest.mark.parametrize("pinned", [True, False])
+@pytest.mark.parametrize("n,nb,budget_tiles", [
+    (4096, 256, 56),     # 4 panels
+    (1000, 64, 96),      # ragged edge, many panels
+    (1536, 128, 40),
+])
+def test_cholesky_host_tiles_ooc(gpu, ora, n, nb, budget_tiles, pinned):
+    """Out-of-core factorization of host packed tiles (page-locked: direct
+    DMA; pageable: staged through pinned slots): the same device work as the
+    column-major host path, so L is bit-identical to it, and within 1e-10 of
+    the oracle."""
+    ooc = gpu
+    mem = budget_tiles * nb * nb * 8
+    t = ooc.HostTiles(n, nb, pinned=pinned)
+    assert t.pinned == pinned
+    try:
+        t.generate()
+        secs, info = ooc.cholesky_host_tiles(t, mem)
+        assert info == 0 and secs > 0
+        L = ooc.tiles_to_dense(t.array(), n, nb)
+    finally:
+        t.free()
+    A = ooc.pdmatgen(n, nb)
+    ooc.cholesky(A, nb, mem)
+    assert np.array_equal(L, np.tril(A))
+    Lref = ooc.pdmatgen(n, nb)
+    ora.cholesky(Lref, nb, mem)
+    assert rel(L, Lref) < L_TOL
+
+
 def test_cholesky_matches_mkl_tiles(gpu, ora):
     """L against the driver run on MKL dpotrf/dtrsm/dsyrk/dgemm tiles, the
     library the reference linked for its host dpotrf (Makefile:17,45)."""
