@@ -214,6 +214,12 @@ int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long lon
                          const long long *b_off, const double *dA, const double *dB, double *dX,
                          int precision, double tol, int max_iter, int reps, double *berr,
                          int *iters, int *status, oocdag_batch_stats *stats);
+/* Fixes the kernel variant oocdag_batched_solve uses for a precision
+ * (0 FP64, 1 FP32+IR) instead of autotuning; variant -1 restores the
+ * autotuner.  Returns the number of variants, or OOCDAG_EARG.  Variants
+ * (T, RB): right-looking with T x T diagonal blocks and RB x RB register
+ * tiles, and the left-looking MFMA kernel (T = 64, RB = 0). */
+int oocdag_batch_set_variant(int precision, int variant);
 
 /* ---------------------------------------------------------------------------
  * Continuous-batching solve server (no reference counterpart).  Requests are

@@ -107,6 +107,7 @@ SIGNATURES.update({
     "oocdag_batch_layout": (C.c_longlong, [C.c_int, _ip, _ip, _llp, _llp]),
     "oocdag_batch_generate": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
                                         C.c_double, C.c_ulonglong]),
+    "oocdag_batch_set_variant": (C.c_int, [C.c_int, C.c_int]),
     "oocdag_batched_solve": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _ip,
                                        _ip, C.POINTER(BatchStats)]),
@@ -433,6 +434,15 @@ class Batch:
         _check(lib().oocdag_batch_generate(len(self.n), self._i(self.n), self._i(self.nrhs),
                                            self._l(self.a_off), self._l(self.b_off), self.A.ptr,
                                            self.B.ptr, mu, seed), "batch_generate")
+
+    @staticmethod
+    def set_variant(precision: str, variant: int) -> int:
+        """Fix the batched kernel variant (-1: autotune); returns the count."""
+        k = {"fp64": 0, "fp32": 1}[precision]
+        rc = lib().oocdag_batch_set_variant(k, variant)
+        if rc < 0:
+            raise OocdagError(f"batch_set_variant({precision}, {variant}) failed")
+        return rc
 
     def solve(self, precision="fp64", tol=None, max_iter=10, reps=1):
         k = len(self.n)

@@ -34,6 +34,9 @@
 namespace ooc {
 namespace {
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
 constexpr int NTB = 256;
 constexpr int PMAX = 1024;        // max staged panel rows
 constexpr int NMAX = PMAX + 64;   // max system order
@@ -57,6 +60,7 @@ struct BatchArgs {
     int *counter;
     double tol;
     int max_iter;
+    int rowmap;   // FP32 MFMA result rows: 0: 4*(lane/16) + r, 1: lane/16 + 4r
 };
 
 template <typename FT, int T>
@@ -389,6 +393,442 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Left-looking variant for larger systems.  Block columns of LT = 64:
+//   U. C_j = A[k0:, k0:k0+64] - L[k0:, :k0] L[k0:k0+64, :k0]^T as one LDS-tiled
+//      MFMA GEMM per 64-row tile (every element of the trailing matrix is
+//      read and written once over the whole factorization, instead of once
+//      per block step as in the right-looking kernel),
+//   D. the 64 x 64 diagonal block of C_j factored in LDS (16-column sub-
+//      blocks: one wave factors, all threads solve and update) and inverted
+//      (column per thread group, one barrier per step),
+//   T. the rows below solved as X = C_j Di^T with MFMA (Di from LDS),
+// then the same blocked substitution, FP64 residual and refinement as the
+// right-looking kernel, 4 right-hand sides resident at a time.
+// ---------------------------------------------------------------------------
+constexpr int LT = 64;
+constexpr int LS = LT + 1;
+constexpr int LBK = 16;
+constexpr int LSA = LT + 8;    // staging row stride (elements)
+constexpr int LRMAX = 4;
+
+template <typename FT>
+struct SmemLL {
+    union {
+        struct {
+            FT A[2][LBK * LSA];
+            FT B[2][LBK * LSA];
+        } g;
+        struct {
+            double D[LT * LS];
+            double Di[LT * LS];
+        } f;
+        struct {
+            double y[LRMAX * NMAX];
+            double x[LRMAX * NMAX];
+        } v;
+    } u;
+    double red[NTB];
+    double tmp[LRMAX * LT];
+    double rd[LT];
+    int bad;
+    int sid;
+};
+
+template <typename FT> struct AccT;
+template <> struct AccT<double> { typedef d4 type; };
+template <> struct AccT<float> { typedef f4 type; };
+
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma4(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// row of accumulator element r of this lane inside its 16 x 16 block
+template <typename FT>
+__device__ __forceinline__ int acc_row(int lane, int r, int rowmap) {
+    if (sizeof(FT) == 8) return (lane >> 4) + 4 * r;   // FP64 MFMA
+    return rowmap ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+}
+
+// x <- W^-1 y with 64-wide blocks and the stored block inverses
+template <typename FT>
+__device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, int ld, int nrhs) {
+    const int tid = threadIdx.x;
+    double *y = S.u.v.y;
+    const int nbk = (n + LT - 1) / LT;
+    for (int kb = 0; kb < nbk; ++kb) {
+        const int k0 = kb * LT, w = min(LT, n - k0);
+        const FT *Di = Dinv + (long long)kb * LT * LT;   // Di(c, d) at c + d*LT
+        for (int idx = tid; idx < w * nrhs; idx += NTB) {
+            const int c = idx % w, k = idx / w;
+            double s = 0.0;
+            for (int d = 0; d <= c; ++d) s += (double)Di[c + d * LT] * y[k * NMAX + k0 + d];
+            S.tmp[k * LT + c] = s;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < w * nrhs; idx += NTB) {
+            const int c = idx % w, k = idx / w;
+            y[k * NMAX + k0 + c] = S.tmp[k * LT + c];
+        }
+        __syncthreads();
+        const int rows = n - k0 - w;
+        for (int idx = tid; idx < rows * nrhs; idx += NTB) {
+            const int i = k0 + w + idx % rows, k = idx / rows;
+            double s = 0.0;
+            for (int c = 0; c < w; ++c) s += (double)W[i + (long long)(k0 + c) * ld] * y[k * NMAX + k0 + c];
+            y[k * NMAX + i] -= s;
+        }
+        __syncthreads();
+    }
+    for (int kb = nbk - 1; kb >= 0; --kb) {
+        const int k0 = kb * LT, w = min(LT, n - k0);
+        const FT *Di = Dinv + (long long)kb * LT * LT;
+        for (int idx = tid; idx < w * nrhs; idx += NTB) {
+            const int c = idx % w, k = idx / w;
+            double s = 0.0;
+            for (int d = c; d < w; ++d) s += (double)Di[d + c * LT] * y[k * NMAX + k0 + d];
+            S.tmp[k * LT + c] = s;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < w * nrhs; idx += NTB) {
+            const int c = idx % w, k = idx / w;
+            y[k * NMAX + k0 + c] = S.tmp[k * LT + c];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < k0 * nrhs; idx += NTB) {
+            const int i = idx % k0, k = idx / k0;
+            const FT *col = W + (long long)i * ld + k0;   // L(k0.., i), contiguous
+            double s = 0.0;
+            for (int r = 0; r < w; ++r) s += (double)col[r] * y[k * NMAX + k0 + r];
+            y[k * NMAX + i] -= s;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
+    typedef typename AccT<FT>::type acc_t;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    SmemLL<FT> &S = *reinterpret_cast<SmemLL<FT> *>(smem_raw);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int il = lane & 15, kl = lane >> 4;
+    for (;;) {
+        if (tid == 0) S.sid = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int q = S.sid;
+        __syncthreads();
+        if (q >= a.nsys) break;
+        const int sidx = a.order[q];
+        const SysDesc sd = a.sys[sidx];
+        const int n = sd.n, nrhs_all = sd.nrhs, ld = sd.ld;
+        const double *A = a.A + sd.a;
+        const double *B0 = a.B + sd.b;
+        double *X0 = a.X + sd.b;
+        FT *W = (FT *)a.W + sd.w;
+        FT *Dinv = (FT *)a.Dinv + sd.d;
+        if (n > PMAX || nrhs_all > RTOTAL) {
+            if (tid == 0) a.status[sidx] = -1;
+            continue;
+        }
+        if (tid == 0) S.bad = 0;
+        __syncthreads();
+        const int nbk = (n + LT - 1) / LT;
+        for (int jb = 0; jb < nbk; ++jb) {
+            const int k0 = jb * LT, w = min(LT, n - k0), m = n - k0;
+            // ---- U: left-looking update of block column jb, 64-row tiles
+            const int nrt = (m + LT - 1) / LT;
+            for (int rt = 0; rt < nrt; ++rt) {
+                const int r0 = k0 + rt * LT;
+                acc_t acc[2][2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+                if (k0 > 0) {
+                    FT ra[4], rb[4];
+                    auto gload = [&](int kk) {
+#pragma unroll
+                        for (int qd = 0; qd < 4; ++qd) {
+                            const int idx = tid + NTB * qd;
+                            const int r = idx & (LT - 1), kc = idx >> 6;
+                            ra[qd] = (r0 + r < n) ? W[(r0 + r) + (long long)(kk + kc) * ld] : (FT)0;
+                            rb[qd] = (k0 + r < n) ? W[(k0 + r) + (long long)(kk + kc) * ld] : (FT)0;
+                        }
+                    };
+                    auto lstore = [&](int buf) {
+#pragma unroll
+                        for (int qd = 0; qd < 4; ++qd) {
+                            const int idx = tid + NTB * qd;
+                            const int r = idx & (LT - 1), kc = idx >> 6;
+                            S.u.g.A[buf][kc * LSA + r] = ra[qd];
+                            S.u.g.B[buf][kc * LSA + r] = rb[qd];
+                        }
+                    };
+                    const int nk = k0 / LBK;
+                    gload(0);
+                    lstore(0);
+                    __syncthreads();
+                    for (int kt = 0; kt < nk; ++kt) {
+                        const int cur = kt & 1;
+                        if (kt + 1 < nk) gload((kt + 1) * LBK);
+#pragma unroll
+                        for (int ks = 0; ks < LBK / 4; ++ks) {
+                            const int kk = ks * 4 + kl;
+                            FT fa[2], fb[2];
+#pragma unroll
+                            for (int i = 0; i < 2; ++i) fa[i] = S.u.g.A[cur][kk * LSA + wm * 32 + i * 16 + il];
+#pragma unroll
+                            for (int j = 0; j < 2; ++j) fb[j] = S.u.g.B[cur][kk * LSA + wn * 32 + j * 16 + il];
+#pragma unroll
+                            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                                for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[i], fb[j], acc[i][j]);
+                        }
+                        if (kt + 1 < nk) lstore(cur ^ 1);
+                        __syncthreads();
+                    }
+                }
+                // C = A - acc (lower part only in the diagonal tile)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        const int c = wn * 32 + j * 16 + il;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int rr = wm * 32 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
+                            const int gr = r0 + rr;
+                            if (gr >= n || c >= w || gr < k0 + c) continue;
+                            const long long o = gr + (long long)(k0 + c) * ld;
+                            W[o] = (FT)(A[o] - (double)acc[i][j][r]);
+                        }
+                    }
+                __syncthreads();
+            }
+            // ---- D: factor and invert the diagonal block in LDS (double)
+            double *D = S.u.f.D, *Di = S.u.f.Di;
+            for (int idx = tid; idx < LT * LT; idx += NTB) {
+                const int r = idx & (LT - 1), c = idx >> 6;
+                D[c * LS + r] = (r < w && c < w && r >= c) ? (double)W[(k0 + r) + (long long)(k0 + c) * ld]
+                                                           : (r == c ? 1.0 : 0.0);
+            }
+            __syncthreads();
+            for (int s0 = 0; s0 < w; s0 += 16) {
+                const int sw = min(16, w - s0);
+                if (wave == 0) {
+                    for (int jj = 0; jj < sw; ++jj) {
+                        double v = 0.0;
+                        if (lane >= jj && lane < sw) {
+                            v = D[(s0 + jj) * LS + s0 + lane];
+                            for (int k = 0; k < jj; ++k) v -= D[(s0 + k) * LS + s0 + lane] * D[(s0 + k) * LS + s0 + jj];
+                        }
+                        if (lane == jj) {
+                            if (!(v > 0.0) && S.bad == 0) S.bad = k0 + s0 + jj + 1;
+                            D[(s0 + jj) * LS + s0 + jj] = sqrt(v);
+                        }
+                        wave_sync();
+                        if (lane > jj && lane < sw) D[(s0 + jj) * LS + s0 + lane] = v / D[(s0 + jj) * LS + s0 + jj];
+                        wave_sync();
+                    }
+                }
+                __syncthreads();
+                if (S.bad) break;
+                // rows of the 64-block below the sub-block: x L_ss^T = row
+                for (int i = s0 + sw + tid; i < w; i += NTB) {
+                    for (int c = 0; c < sw; ++c) {
+                        double v = D[(s0 + c) * LS + i];
+                        for (int d = 0; d < c; ++d) v -= D[(s0 + d) * LS + i] * D[(s0 + d) * LS + s0 + c];
+                        D[(s0 + c) * LS + i] = v / D[(s0 + c) * LS + s0 + c];
+                    }
+                }
+                __syncthreads();
+                const int mt = w - s0 - sw;
+                for (int idx = tid; idx < mt * mt; idx += NTB) {
+                    int i = idx % mt, k = idx / mt;
+                    if (i < k) continue;
+                    i += s0 + sw;
+                    k += s0 + sw;
+                    double v = D[k * LS + i];
+                    for (int c = 0; c < sw; ++c) v -= D[(s0 + c) * LS + i] * D[(s0 + c) * LS + k];
+                    D[k * LS + i] = v;
+                }
+                __syncthreads();
+            }
+            if (S.bad) break;
+            // inverse: column c of Di, rows split over the 4 waves
+            {
+                const int c = tid & (LT - 1), part = tid >> 6;
+                for (int i = part; i < LT; i += 4) Di[c * LS + i] = (i == c) ? 1.0 : 0.0;
+                if (tid < LT) S.rd[tid] = 1.0 / D[tid * LS + tid];
+                __syncthreads();
+                for (int k = 0; k < w; ++k) {
+                    if (part == 0 && c <= k) Di[c * LS + k] *= S.rd[k];
+                    __syncthreads();
+                    const double yk = Di[c * LS + k];
+                    if (c <= k)
+                        for (int i = k + 1 + part; i < w; i += 4) Di[c * LS + i] -= D[k * LS + i] * yk;
+                }
+                __syncthreads();
+            }
+            // L_D into W, Di into the inverse store
+            for (int idx = tid; idx < LT * LT; idx += NTB) {
+                const int r = idx & (LT - 1), c = idx >> 6;
+                if (r < w && c < w && r >= c) W[(k0 + r) + (long long)(k0 + c) * ld] = (FT)D[c * LS + r];
+                Dinv[(long long)jb * LT * LT + idx] = (FT)((r >= c) ? Di[c * LS + r] : 0.0);
+            }
+            __syncthreads();
+            // ---- T: rows below, X = C Di^T (Di(c, d) = Di[d * LS + c])
+            const int mb = m - w;
+            for (int r0 = k0 + w; r0 < k0 + w + mb; r0 += LT) {
+                acc_t acc[2][2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+                int rows[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) rows[i] = min(r0 + wm * 32 + i * 16 + il, n - 1);
+#pragma unroll 4
+                for (int ks = 0; ks < LT / 4; ++ks) {
+                    const int kk = ks * 4 + kl;
+                    FT fa[2], fb[2];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) fa[i] = W[rows[i] + (long long)(k0 + kk) * ld];
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) fb[j] = (FT)Di[kk * LS + wn * 32 + j * 16 + il];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[i], fb[j], acc[i][j]);
+                }
+                __syncthreads();   // every wave has read its rows before any is overwritten
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        const int c = wn * 32 + j * 16 + il;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int gr = r0 + wm * 32 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
+                            if (gr >= n) continue;
+                            W[gr + (long long)(k0 + c) * ld] = (FT)acc[i][j][r];
+                        }
+                    }
+                __syncthreads();
+            }
+        }
+        if (S.bad) {
+            if (tid == 0) {
+                a.status[sidx] = S.bad;
+                a.berr[sidx] = INFINITY;
+                a.iters[sidx] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- solve and refine, LRMAX right-hand sides at a time
+        double eta_all = 0.0;
+        int it_all = 0;
+        for (int c0 = 0; c0 < nrhs_all; c0 += LRMAX) {
+            const int nrhs = min(LRMAX, nrhs_all - c0);
+            const double *B = B0 + (long long)c0 * ld;
+            double *X = X0 + (long long)c0 * ld;
+            double *y = S.u.v.y, *x = S.u.v.x;
+            for (int idx = tid; idx < n * nrhs; idx += NTB) {
+                const int i = idx % n, k = idx / n;
+                y[k * NMAX + i] = B[i + (long long)k * ld];
+                x[k * NMAX + i] = 0.0;
+            }
+            __syncthreads();
+            int it = 0;
+            double eta = INFINITY, prev = INFINITY;
+            for (;;) {
+                tri_solve_ll<FT>(S, W, Dinv, n, ld, nrhs);
+                for (int idx = tid; idx < n * nrhs; idx += NTB) {
+                    const int i = idx % n, k = idx / n;
+                    x[k * NMAX + i] += y[k * NMAX + i];
+                }
+                __syncthreads();
+                double rmax[LRMAX], xmax[LRMAX], bmax[LRMAX];
+#pragma unroll
+                for (int k = 0; k < LRMAX; ++k) rmax[k] = xmax[k] = bmax[k] = 0.0;
+                double amax = 0.0;
+                for (int i = tid; i < n; i += NTB) {
+                    double rs[LRMAX], as = 0.0;
+#pragma unroll
+                    for (int k = 0; k < LRMAX; ++k) rs[k] = (k < nrhs) ? B[i + (long long)k * ld] : 0.0;
+                    for (int j = 0; j < n; ++j) {
+                        const double aij = A[i + (long long)j * ld];
+                        as += fabs(aij);
+#pragma unroll
+                        for (int k = 0; k < LRMAX; ++k)
+                            if (k < nrhs) rs[k] -= aij * x[k * NMAX + j];
+                    }
+                    amax = fmax(amax, as);
+#pragma unroll
+                    for (int k = 0; k < LRMAX; ++k) {
+                        if (k >= nrhs) break;
+                        y[k * NMAX + i] = rs[k];
+                        rmax[k] = fmax(rmax[k], fabs(rs[k]));
+                        xmax[k] = fmax(xmax[k], fabs(x[k * NMAX + i]));
+                        bmax[k] = fmax(bmax[k], fabs(B[i + (long long)k * ld]));
+                    }
+                }
+                double e = 0.0;
+                S.red[tid] = amax;
+                __syncthreads();
+                for (int s2 = NTB / 2; s2 > 0; s2 >>= 1) {
+                    if (tid < s2) S.red[tid] = fmax(S.red[tid], S.red[tid + s2]);
+                    __syncthreads();
+                }
+                const double An = S.red[0];
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < LRMAX; ++k) {
+                    if (k >= nrhs) break;
+                    double vals[3] = {rmax[k], xmax[k], bmax[k]};
+                    double out[3];
+#pragma unroll
+                    for (int v = 0; v < 3; ++v) {
+                        S.red[tid] = vals[v];
+                        __syncthreads();
+                        for (int s2 = NTB / 2; s2 > 0; s2 >>= 1) {
+                            if (tid < s2) S.red[tid] = fmax(S.red[tid], S.red[tid + s2]);
+                            __syncthreads();
+                        }
+                        out[v] = S.red[0];
+                        __syncthreads();
+                    }
+                    e = fmax(e, out[0] / (An * out[1] + out[2]));
+                }
+                prev = eta;
+                eta = e;
+                if (eta <= a.tol) break;
+                if (it >= a.max_iter || !(eta < 0.5 * prev)) break;
+                ++it;
+            }
+            for (int idx = tid; idx < n * nrhs; idx += NTB) {
+                const int i = idx % n, k = idx / n;
+                X[i + (long long)k * ld] = x[k * NMAX + i];
+            }
+            eta_all = fmax(eta_all, eta);
+            it_all = max(it_all, it);
+            __syncthreads();
+        }
+        if (tid == 0) {
+            a.berr[sidx] = eta_all;
+            a.iters[sidx] = it_all;
+            a.status[sidx] = (eta_all <= a.tol) ? 0 : -2;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
 __global__ void k_add_diag_one(double *A, int n, int ld, double mu) {
@@ -413,18 +853,21 @@ struct Variant {
     BatchKernel k;
     size_t smem;
     int T, RB, fp32;
+    int wgs;   // workgroups per CU
 };
 
+// RB = 0 marks the left-looking MFMA kernel (T = 64)
 Variant variants_fp64[] = {
-    {k_batched<double, 16, 4>, smem_bytes<double, 16, 4>(), 16, 4, 0},
-    {k_batched<double, 16, 8>, smem_bytes<double, 16, 8>(), 16, 8, 0},
-    {k_batched<double, 32, 4>, 0, 32, 4, 0},  // needs PMAX*33*8 > LDS: disabled (smem 0)
+    {k_batched<double, 16, 4>, smem_bytes<double, 16, 4>(), 16, 4, 0, 1},
+    {k_batched<double, 16, 8>, smem_bytes<double, 16, 8>(), 16, 8, 0, 1},
+    {k_batched_ll<double>, sizeof(SmemLL<double>), 64, 0, 0, 2},
 };
 Variant variants_fp32[] = {
-    {k_batched<float, 16, 4>, smem_bytes<float, 16, 4>(), 16, 4, 1},
-    {k_batched<float, 16, 8>, smem_bytes<float, 16, 8>(), 16, 8, 1},
-    {k_batched<float, 32, 4>, smem_bytes<float, 32, 4>(), 32, 4, 1},
-    {k_batched<float, 32, 8>, smem_bytes<float, 32, 8>(), 32, 8, 1},
+    {k_batched<float, 16, 4>, smem_bytes<float, 16, 4>(), 16, 4, 1, 1},
+    {k_batched<float, 16, 8>, smem_bytes<float, 16, 8>(), 16, 8, 1, 1},
+    {k_batched<float, 32, 4>, smem_bytes<float, 32, 4>(), 32, 4, 1, 1},
+    {k_batched<float, 32, 8>, smem_bytes<float, 32, 8>(), 32, 8, 1, 1},
+    {k_batched_ll<float>, sizeof(SmemLL<float>), 64, 0, 1, 2},
 };
 
 double now_s() {
@@ -503,7 +946,17 @@ extern "C" int oocdag_batch_generate(int nsys, const int *n, const int *nrhs, co
 namespace {
 std::mutex g_tune_mu;
 int g_tuned[2] = {-1, -1};
+int g_forced[2] = {-1, -1};
+constexpr int kVariants[2] = {3, 5};   // entries of variants_fp64 / variants_fp32
 }  // namespace
+
+extern "C" int oocdag_batch_set_variant(int precision, int variant) {
+    if (precision != 0 && precision != 1) return OOCDAG_EARG;
+    if (variant < -1 || variant >= kVariants[precision]) return OOCDAG_EARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_forced[precision] = variant;
+    return kVariants[precision];
+}
 
 namespace ooc {
 // Generates one A = (1/n) G G^T + mu I (full, ld even) with G from the
@@ -557,7 +1010,8 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         sd[s].w = w;
         sd[s].d = d;
         w += ((long long)sd[s].ld * n[s] + 31) / 32 * 32;
-        d += (long long)((n[s] + 15) / 16) * Tmax * Tmax;
+        d += std::max((long long)((n[s] + 15) / 16) * Tmax * Tmax,
+                      (long long)((n[s] + LT - 1) / LT) * LT * LT);
     }
     std::vector<int> order(nsys);
     std::iota(order.begin(), order.end(), 0);
@@ -603,6 +1057,9 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess)
             ncu = pr.multiProcessorCount;
     }
+    // FP32 MFMA result row order (the FP64 one is fixed)
+    const int rowmap = precision == 0 ? 1 : mfma_rowmap(0, s);
+    if (rowmap < 0) return OOCDAG_EHIP;
     auto make_args = [&](const int *ord, int cnt) {
         BatchArgs a;
         a.sys = dsd;
@@ -619,16 +1076,17 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         a.counter = dcounter;
         a.tol = tol;
         a.max_iter = max_iter;
+        a.rowmap = rowmap;
         return a;
     };
     auto run = [&](const Variant &v, const int *ord, int cnt) -> hipError_t {
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, dcounter, 0);
         BatchArgs a = make_args(ord, cnt);
-        hipLaunchKernelGGL(v.k, dim3(ncu), dim3(NTB), v.smem, s, a);
+        hipLaunchKernelGGL(v.k, dim3(ncu * v.wgs), dim3(NTB), v.smem, s, a);
         return hipGetLastError();
     };
     Variant *vs = precision == 0 ? variants_fp64 : variants_fp32;
-    const int nv = precision == 0 ? 3 : 4;
+    const int nv = kVariants[precision];
     for (int i = 0; i < nv; ++i)
         if (vs[i].smem > 0 && vs[i].smem <= 160 * 1024)
             (void)hipFuncSetAttribute((const void *)vs[i].k,
@@ -636,7 +1094,8 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     int best = -1;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        if (reps <= 0) best = g_tuned[precision];
+        if (g_forced[precision] >= 0) best = g_forced[precision];
+        else if (reps <= 0) best = g_tuned[precision];
     }
     // --- autotune on a sample of the sorted order (skipped when a tuned
     // variant is cached and the caller asked for a plain run)
@@ -673,6 +1132,7 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     }
     }
     st->tune_seconds = now_s() - tt0;
+    if (vs[best].smem == 0 || vs[best].smem > 160 * 1024) return OOCDAG_EARG;
     st->variant_T = vs[best].T;
     st->variant_RB = vs[best].RB;
     int rc = OOCDAG_OK;

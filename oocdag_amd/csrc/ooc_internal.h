@@ -79,6 +79,11 @@ enum GemmKind {
 
 hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s);
 
+// Row order of the FP32-output MFMA results (kind 0: f32 16x16x4, 1: bf16
+// 16x16x32), read once from a probe: 0 row = 4*(lane/16) + r, 1 row =
+// lane/16 + 4*r; -1 if the layouts differ from both (gemm_lp.hip).
+int mfma_rowmap(int kind, hipStream_t s);
+
 // Largest block edge of the GEMM kernels: operands must provide this many
 // addressable rows past any block start they cover.
 constexpr int GEMM_BM = 128;

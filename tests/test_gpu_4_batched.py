@@ -17,9 +17,19 @@ def sizes(k, seed, lo=32, hi=1024):
     return n, r
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_batched_matches_fp64_oracle(gpu, prec):
+@pytest.mark.parametrize("prec,variant", [("fp64", v) for v in (-1, 0, 1, 2)] +
+                         [("fp32", v) for v in (-1, 0, 1, 2, 3, 4)])
+def test_batched_matches_fp64_oracle(gpu, prec, variant):
+    """Every kernel variant (and the autotuned choice, -1) against the oracle."""
     ooc = gpu
+    ooc.Batch.set_variant(prec, variant)
+    try:
+        _check_batched(ooc, prec)
+    finally:
+        ooc.Batch.set_variant(prec, -1)
+
+
+def _check_batched(ooc, prec):
     n, r = sizes(120, 5)
     n[:3] = [1024, 33, 1]   # extremes: the largest order, an odd one, a scalar system
     r[3] = 37               # more right-hand sides than are resident at once (chunks of 8)
@@ -43,8 +53,19 @@ def test_batched_matches_fp64_oracle(gpu, prec):
         assert iters.max() >= 1
 
 
-def test_batched_alone_equals_in_mix(gpu):
+# a fixed variant: the property is per kernel (separate autotuned calls may pick
+# different kernels)
+@pytest.mark.parametrize("variant", [0, 4])
+def test_batched_alone_equals_in_mix(gpu, variant):
     ooc = gpu
+    ooc.Batch.set_variant("fp32", variant)
+    try:
+        _alone_in_mix(ooc)
+    finally:
+        ooc.Batch.set_variant("fp32", -1)
+
+
+def _alone_in_mix(ooc):
     n, r = sizes(40, 9)
     bt = ooc.Batch(n, r)
     bt.generate(mu=0.1, seed=9)
@@ -60,8 +81,17 @@ def test_batched_alone_equals_in_mix(gpu):
         assert np.array_equal(X, X1)
 
 
-def test_batched_non_spd_status(gpu):
+@pytest.mark.parametrize("variant", [-1, 2])
+def test_batched_non_spd_status(gpu, variant):
     ooc = gpu
+    ooc.Batch.set_variant("fp64", variant)
+    try:
+        _non_spd(ooc)
+    finally:
+        ooc.Batch.set_variant("fp64", -1)
+
+
+def _non_spd(ooc):
     n = np.array([64, 100, 200], dtype=np.int32)
     r = np.array([1, 2, 3], dtype=np.int32)
     bt = ooc.Batch(n, r)
