@@ -410,27 +410,28 @@ constexpr int LT = 64;
 constexpr int LS = LT + 1;
 constexpr int LBK = 16;
 constexpr int LSA = LT + 8;    // staging row stride (elements)
-constexpr int LRMAX = 4;
+constexpr int LRT = 128;       // rows per update tile
+constexpr int LSA2 = LRT + 8;
+constexpr int LRMAX = 8;
 
 template <typename FT>
 struct SmemLL {
     union {
         struct {
-            FT A[2][LBK * LSA];
-            FT B[2][LBK * LSA];
+            FT A[1][LBK * LSA2];
+            FT B[1][LBK * LSA];
         } g;
         struct {
             double D[LT * LS];
             double Di[LT * LS];
+            double T[3 * 256];   // blocked inverse: one block row of products
         } f;
         struct {
-            double y[LRMAX * NMAX];
-            double x[LRMAX * NMAX];
+            double y[LRMAX * NMAX];   // x lives in the caller's X (global)
         } v;
     } u;
     double red[NTB];
     double tmp[LRMAX * LT];
-    double rd[LT];
     int bad;
     int sid;
 };
@@ -474,12 +475,20 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
             y[k * NMAX + k0 + c] = S.tmp[k * LT + c];
         }
         __syncthreads();
-        const int rows = n - k0 - w;
-        for (int idx = tid; idx < rows * nrhs; idx += NTB) {
-            const int i = k0 + w + idx % rows, k = idx / rows;
-            double s = 0.0;
-            for (int c = 0; c < w; ++c) s += (double)W[i + (long long)(k0 + c) * ld] * y[k * NMAX + k0 + c];
-            y[k * NMAX + i] -= s;
+        // rows below: one thread per row, all right-hand sides (L read once)
+        for (int i = k0 + w + tid; i < n; i += NTB) {
+            double sk[LRMAX];
+#pragma unroll
+            for (int k = 0; k < LRMAX; ++k) sk[k] = 0.0;
+            for (int c = 0; c < w; ++c) {
+                const double wv = (double)W[i + (long long)(k0 + c) * ld];
+#pragma unroll
+                for (int k = 0; k < LRMAX; ++k)
+                    if (k < nrhs) sk[k] += wv * y[k * NMAX + k0 + c];
+            }
+#pragma unroll
+            for (int k = 0; k < LRMAX; ++k)
+                if (k < nrhs) y[k * NMAX + i] -= sk[k];
         }
         __syncthreads();
     }
@@ -498,12 +507,20 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
             y[k * NMAX + k0 + c] = S.tmp[k * LT + c];
         }
         __syncthreads();
-        for (int idx = tid; idx < k0 * nrhs; idx += NTB) {
-            const int i = idx % k0, k = idx / k0;
+        for (int i = tid; i < k0; i += NTB) {
             const FT *col = W + (long long)i * ld + k0;   // L(k0.., i), contiguous
-            double s = 0.0;
-            for (int r = 0; r < w; ++r) s += (double)col[r] * y[k * NMAX + k0 + r];
-            y[k * NMAX + i] -= s;
+            double sk[LRMAX];
+#pragma unroll
+            for (int k = 0; k < LRMAX; ++k) sk[k] = 0.0;
+            for (int r = 0; r < w; ++r) {
+                const double wv = (double)col[r];
+#pragma unroll
+                for (int k = 0; k < LRMAX; ++k)
+                    if (k < nrhs) sk[k] += wv * y[k * NMAX + k0 + r];
+            }
+#pragma unroll
+            for (int k = 0; k < LRMAX; ++k)
+                if (k < nrhs) y[k * NMAX + i] -= sk[k];
         }
         __syncthreads();
     }
@@ -540,68 +557,58 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
         const int nbk = (n + LT - 1) / LT;
         for (int jb = 0; jb < nbk; ++jb) {
             const int k0 = jb * LT, w = min(LT, n - k0), m = n - k0;
-            // ---- U: left-looking update of block column jb, 64-row tiles
-            const int nrt = (m + LT - 1) / LT;
+            // ---- U: left-looking update of block column jb, 128-row tiles
+            // (4 waves as 2 x 2, each 64 rows x 32 columns)
+            const int nrt = (m + LRT - 1) / LRT;
             for (int rt = 0; rt < nrt; ++rt) {
-                const int r0 = k0 + rt * LT;
-                acc_t acc[2][2];
+                const int r0 = k0 + rt * LRT;
+                acc_t acc[4][2];
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-                if (k0 > 0) {
-                    FT ra[4], rb[4];
-                    auto gload = [&](int kk) {
+                // single-buffered K loop (the other workgroup on the CU
+                // overlaps its loads with these MFMAs)
+                for (int kk = 0; kk < k0; kk += LBK) {
 #pragma unroll
-                        for (int qd = 0; qd < 4; ++qd) {
-                            const int idx = tid + NTB * qd;
-                            const int r = idx & (LT - 1), kc = idx >> 6;
-                            ra[qd] = (r0 + r < n) ? W[(r0 + r) + (long long)(kk + kc) * ld] : (FT)0;
-                            rb[qd] = (k0 + r < n) ? W[(k0 + r) + (long long)(kk + kc) * ld] : (FT)0;
-                        }
-                    };
-                    auto lstore = [&](int buf) {
-#pragma unroll
-                        for (int qd = 0; qd < 4; ++qd) {
-                            const int idx = tid + NTB * qd;
-                            const int r = idx & (LT - 1), kc = idx >> 6;
-                            S.u.g.A[buf][kc * LSA + r] = ra[qd];
-                            S.u.g.B[buf][kc * LSA + r] = rb[qd];
-                        }
-                    };
-                    const int nk = k0 / LBK;
-                    gload(0);
-                    lstore(0);
-                    __syncthreads();
-                    for (int kt = 0; kt < nk; ++kt) {
-                        const int cur = kt & 1;
-                        if (kt + 1 < nk) gload((kt + 1) * LBK);
-#pragma unroll
-                        for (int ks = 0; ks < LBK / 4; ++ks) {
-                            const int kk = ks * 4 + kl;
-                            FT fa[2], fb[2];
-#pragma unroll
-                            for (int i = 0; i < 2; ++i) fa[i] = S.u.g.A[cur][kk * LSA + wm * 32 + i * 16 + il];
-#pragma unroll
-                            for (int j = 0; j < 2; ++j) fb[j] = S.u.g.B[cur][kk * LSA + wn * 32 + j * 16 + il];
-#pragma unroll
-                            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                                for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[i], fb[j], acc[i][j]);
-                        }
-                        if (kt + 1 < nk) lstore(cur ^ 1);
-                        __syncthreads();
+                    for (int qd = 0; qd < 8; ++qd) {
+                        const int idx = tid + NTB * qd;
+                        const int r = idx & (LRT - 1), kc = idx >> 7;
+                        S.u.g.A[0][kc * LSA2 + r] =
+                            (r0 + r < n) ? W[(r0 + r) + (long long)(kk + kc) * ld] : (FT)0;
                     }
-                }
-                // C = A - acc (lower part only in the diagonal tile)
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                    for (int qd = 0; qd < 4; ++qd) {
+                        const int idx = tid + NTB * qd;
+                        const int r = idx & (LT - 1), kc = idx >> 6;
+                        S.u.g.B[0][kc * LSA + r] =
+                            (k0 + r < n) ? W[(k0 + r) + (long long)(kk + kc) * ld] : (FT)0;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int ks = 0; ks < LBK / 4; ++ks) {
+                        const int kq = ks * 4 + kl;
+                        FT fa[4], fb[2];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) fa[i] = S.u.g.A[0][kq * LSA2 + wm * 64 + i * 16 + il];
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) fb[j] = S.u.g.B[0][kq * LSA + wn * 32 + j * 16 + il];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[i], fb[j], acc[i][j]);
+                    }
+                    __syncthreads();
+                }
+                // C = A - acc (lower part only in the diagonal block)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         const int c = wn * 32 + j * 16 + il;
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int rr = wm * 32 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
+                            const int rr = wm * 64 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
                             const int gr = r0 + rr;
                             if (gr >= n || c >= w || gr < k0 + c) continue;
                             const long long o = gr + (long long)(k0 + c) * ld;
@@ -660,20 +667,48 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 __syncthreads();
             }
             if (S.bad) break;
-            // inverse: column c of Di, rows split over the 4 waves
+            // inverse, blocked by 16: diagonal blocks first (wave b inverts
+            // block b, lane = column), then block rows i = 1..3 from
+            // X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj (two barriers per row)
             {
-                const int c = tid & (LT - 1), part = tid >> 6;
-                for (int i = part; i < LT; i += 4) Di[c * LS + i] = (i == c) ? 1.0 : 0.0;
-                if (tid < LT) S.rd[tid] = 1.0 / D[tid * LS + tid];
-                __syncthreads();
-                for (int k = 0; k < w; ++k) {
-                    if (part == 0 && c <= k) Di[c * LS + k] *= S.rd[k];
-                    __syncthreads();
-                    const double yk = Di[c * LS + k];
-                    if (c <= k)
-                        for (int i = k + 1 + part; i < w; i += 4) Di[c * LS + i] -= D[k * LS + i] * yk;
+                for (int idx = tid; idx < LT * LT; idx += NTB) {
+                    const int r = idx & (LT - 1), c = idx >> 6;
+                    Di[c * LS + r] = (r == c) ? 1.0 : 0.0;
                 }
                 __syncthreads();
+                if (lane < 16 && 16 * wave < w) {
+                    const int b0 = 16 * wave, c = b0 + lane;
+                    const int bw = min(16, w - b0);
+                    if (lane < bw) {
+                        for (int k = lane; k < bw; ++k) {
+                            const double yk = Di[c * LS + b0 + k] / D[(b0 + k) * LS + b0 + k];
+                            Di[c * LS + b0 + k] = yk;
+                            for (int i = k + 1; i < bw; ++i)
+                                Di[c * LS + b0 + i] -= D[(b0 + k) * LS + b0 + i] * yk;
+                        }
+                    }
+                }
+                __syncthreads();
+                double *Tb = S.u.f.T;
+                for (int bi = 1; bi * 16 < w; ++bi) {
+                    // T_bj = sum_{k=j}^{bi-1} L_(bi,k) X_(k,j), j < bi: element per thread
+                    for (int idx = tid; idx < bi * 256; idx += NTB) {
+                        const int bj = idx >> 8, e = idx & 255, r = e & 15, c = e >> 4;
+                        const int gr = bi * 16 + r, gc = bj * 16 + c;
+                        double t = 0.0;
+                        for (int k = gc; k < bi * 16; ++k) t += D[k * LS + gr] * Di[gc * LS + k];
+                        Tb[bj * 256 + c * 16 + r] = t;
+                    }
+                    __syncthreads();
+                    for (int idx = tid; idx < bi * 256; idx += NTB) {
+                        const int bj = idx >> 8, e = idx & 255, r = e & 15, c = e >> 4;
+                        const int gr = bi * 16 + r, gc = bj * 16 + c;
+                        double x = 0.0;
+                        for (int k = 0; k <= r; ++k) x -= Di[(bi * 16 + k) * LS + gr] * Tb[bj * 256 + c * 16 + k];
+                        Di[gc * LS + gr] = x;
+                    }
+                    __syncthreads();
+                }
             }
             // L_D into W, Di into the inverse store
             for (int idx = tid; idx < LT * LT; idx += NTB) {
@@ -731,18 +766,19 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
             __syncthreads();
             continue;
         }
-        // ---- solve and refine, LRMAX right-hand sides at a time
+        // ---- solve and refine, LRMAX right-hand sides at a time; the
+        // correction d in LDS, x accumulated in the caller's X
         double eta_all = 0.0;
         int it_all = 0;
         for (int c0 = 0; c0 < nrhs_all; c0 += LRMAX) {
             const int nrhs = min(LRMAX, nrhs_all - c0);
             const double *B = B0 + (long long)c0 * ld;
             double *X = X0 + (long long)c0 * ld;
-            double *y = S.u.v.y, *x = S.u.v.x;
+            double *y = S.u.v.y;
             for (int idx = tid; idx < n * nrhs; idx += NTB) {
                 const int i = idx % n, k = idx / n;
                 y[k * NMAX + i] = B[i + (long long)k * ld];
-                x[k * NMAX + i] = 0.0;
+                X[i + (long long)k * ld] = 0.0;
             }
             __syncthreads();
             int it = 0;
@@ -751,32 +787,46 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 tri_solve_ll<FT>(S, W, Dinv, n, ld, nrhs);
                 for (int idx = tid; idx < n * nrhs; idx += NTB) {
                     const int i = idx % n, k = idx / n;
-                    x[k * NMAX + i] += y[k * NMAX + i];
+                    X[i + (long long)k * ld] += y[k * NMAX + i];
                 }
                 __syncthreads();
                 double rmax[LRMAX], xmax[LRMAX], bmax[LRMAX];
 #pragma unroll
                 for (int k = 0; k < LRMAX; ++k) rmax[k] = xmax[k] = bmax[k] = 0.0;
                 double amax = 0.0;
-                for (int i = tid; i < n; i += NTB) {
+                // r = b - A x with x staged through LDS 64 rows at a time;
+                // one pass per row slot (rows tid, tid + NTB, ...)
+                for (int i0 = 0; i0 < n; i0 += NTB) {
+                    const int i = i0 + tid;
                     double rs[LRMAX], as = 0.0;
 #pragma unroll
-                    for (int k = 0; k < LRMAX; ++k) rs[k] = (k < nrhs) ? B[i + (long long)k * ld] : 0.0;
-                    for (int j = 0; j < n; ++j) {
-                        const double aij = A[i + (long long)j * ld];
-                        as += fabs(aij);
+                    for (int k = 0; k < LRMAX; ++k) rs[k] = (i < n && k < nrhs) ? B[i + (long long)k * ld] : 0.0;
+                    for (int j0 = 0; j0 < n; j0 += LT) {
+                        const int jw = min(LT, n - j0);
+                        for (int idx = tid; idx < LT * LRMAX; idx += NTB) {
+                            const int jj = idx & (LT - 1), k = idx >> 6;
+                            S.tmp[k * LT + jj] = (jj < jw && k < nrhs) ? X[j0 + jj + (long long)k * ld] : 0.0;
+                        }
+                        __syncthreads();
+                        if (i < n)
+                            for (int jj = 0; jj < jw; ++jj) {
+                                const double aij = A[i + (long long)(j0 + jj) * ld];
+                                as += fabs(aij);
 #pragma unroll
-                        for (int k = 0; k < LRMAX; ++k)
-                            if (k < nrhs) rs[k] -= aij * x[k * NMAX + j];
+                                for (int k = 0; k < LRMAX; ++k) rs[k] -= aij * S.tmp[k * LT + jj];
+                            }
+                        __syncthreads();
                     }
-                    amax = fmax(amax, as);
+                    if (i < n) {
+                        amax = fmax(amax, as);
 #pragma unroll
-                    for (int k = 0; k < LRMAX; ++k) {
-                        if (k >= nrhs) break;
-                        y[k * NMAX + i] = rs[k];
-                        rmax[k] = fmax(rmax[k], fabs(rs[k]));
-                        xmax[k] = fmax(xmax[k], fabs(x[k * NMAX + i]));
-                        bmax[k] = fmax(bmax[k], fabs(B[i + (long long)k * ld]));
+                        for (int k = 0; k < LRMAX; ++k) {
+                            if (k >= nrhs) break;
+                            y[k * NMAX + i] = rs[k];
+                            rmax[k] = fmax(rmax[k], fabs(rs[k]));
+                            xmax[k] = fmax(xmax[k], fabs(X[i + (long long)k * ld]));
+                            bmax[k] = fmax(bmax[k], fabs(B[i + (long long)k * ld]));
+                        }
                     }
                 }
                 double e = 0.0;
@@ -811,10 +861,6 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 if (eta <= a.tol) break;
                 if (it >= a.max_iter || !(eta < 0.5 * prev)) break;
                 ++it;
-            }
-            for (int idx = tid; idx < n * nrhs; idx += NTB) {
-                const int i = idx % n, k = idx / n;
-                X[i + (long long)k * ld] = x[k * NMAX + i];
             }
             eta_all = fmax(eta_all, eta);
             it_all = max(it_all, it);
