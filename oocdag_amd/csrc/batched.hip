@@ -18,6 +18,9 @@
 // Systems whose factor is not positive definite or whose refinement stalls
 // report status != 0 so the caller can rerun them in FP64.
 //
+// A left-looking variant (64-wide block columns, MFMA updates, two
+// workgroups per CU) covers the same steps for larger systems.
+//
 // Variants (T, RB, FT) are compiled ahead; the host autotuner times them on
 // a sample of the batch and launches the fastest through a captured hipGraph.
 #include "ooc_internal.h"
@@ -396,15 +399,18 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
 // ---------------------------------------------------------------------------
 // Left-looking variant for larger systems.  Block columns of LT = 64:
 //   U. C_j = A[k0:, k0:k0+64] - L[k0:, :k0] L[k0:k0+64, :k0]^T as one LDS-tiled
-//      MFMA GEMM per 64-row tile (every element of the trailing matrix is
-//      read and written once over the whole factorization, instead of once
-//      per block step as in the right-looking kernel),
+//      MFMA GEMM per 128-row tile (4 waves as 2 x 2; every element of the
+//      trailing matrix is read and written once over the whole
+//      factorization, instead of once per block step as in the
+//      right-looking kernel),
 //   D. the 64 x 64 diagonal block of C_j factored in LDS (16-column sub-
 //      blocks: one wave factors, all threads solve and update) and inverted
-//      (column per thread group, one barrier per step),
+//      by 16 x 16 blocks (diagonal blocks first, then one block row of
+//      products per barrier),
 //   T. the rows below solved as X = C_j Di^T with MFMA (Di from LDS),
-// then the same blocked substitution, FP64 residual and refinement as the
-// right-looking kernel, 4 right-hand sides resident at a time.
+// then blocked substitution with one thread per row (each element of L read
+// once per pass, y in LDS, x in the caller's X), FP64 residual staged through
+// LDS and refinement, 8 right-hand sides per pass.
 // ---------------------------------------------------------------------------
 constexpr int LT = 64;
 constexpr int LS = LT + 1;
