@@ -448,7 +448,7 @@ def main() -> int:
                      "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK, 4), "traffic": traffic,
                      "traffic_source": traffic_note,
                      "algorithmic_bytes_per_launch": update_algorithmic_bytes(n, nb),
-                     "kernel": "k_dgemm_nt<128,1,0> panel update",
+                     "kernel": "k_dgemm_dtv<1,0,true> panel update",
                      "launches": prof_n, "kernel_seconds": round(prof_t, 4)},
     }
     del A_last

@@ -15,11 +15,13 @@
 //
 // Work decomposition: 256 threads (4 wave64s as 2 x 2) per 128 x BN output
 // block; each wave owns a 64 x (BN/2) sub-block held as 16x16 FP64
-// accumulators of v_mfma_f64_16x16x4_f64.  K advances 16 at a time through
-// double-buffered LDS (one barrier per step); the next step's global loads
-// are issued before the current step's MFMAs.  Block order is remapped so
-// that each XCD (8 per MI355X, round-robin dispatch) receives a contiguous
-// range of blocks that share operand rows in its own L2.
+// accumulators of v_mfma_f64_16x16x4_f64.  Whole 128 x 128 blocks (every
+// panel update) load their operands straight into MFMA registers, 3 K slabs
+// ahead, with no LDS and no barriers (k_dgemm_dtv); ragged and 64-wide
+// blocks step K 16 at a time through double-buffered LDS (k_dgemm_nt).
+// Block order is remapped so that each XCD (8 per MI355X, round-robin
+// dispatch) receives a contiguous range of blocks that share operand rows in
+// its own L2.
 #include "ooc_internal.h"
 
 namespace ooc {
@@ -482,6 +484,174 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     }
 }
 
+// ---- full 128 x 128 blocks: operands straight into MFMA registers ---------
+// Used when every block of C is a whole 128 x 128 block inside one tile and
+// K is a multiple of 16.  No LDS and no barriers: each wave (wm, wn) owns a
+// 64 x 64 block of C and loads its own operands.  Lane (r = l%16, k = l/16)
+// loads 4 consecutive rows 4r..4r+3 of its 64 A rows (two 16-byte loads) at
+// column k of the current 4-wide K slab, and the same for B, so MFMA block t
+// of the wave holds the rows 4R + t (R = row inside the 16 x 16 MFMA block):
+// a row permutation that the epilogue undoes.  A ring of 4 slabs keeps
+// DPD slabs of loads in flight ahead of the MFMAs.
+constexpr int DPD = 3;
+
+// One operand's uniform slab address along K (crossing tiles of a packed
+// operand) plus this lane's fixed offset.
+struct DStream {
+    const double *p, *base;
+    long long ld, te, rowoff;
+    int ts, ym, tri, ti, tk, kc, voff;
+
+    __device__ __forceinline__ void init(const Opnd &o, int rb, int lane_off) {
+        p = o.p;
+        ld = o.ld;
+        te = o.te;
+        ym = o.ym;
+        tri = o.tri;
+        const int gr = o.r0 + rb;
+        if (o.ts > 0) {
+            ts = o.ts;
+            ti = gr / ts;
+            rowoff = gr - (long long)ti * ts;
+            tk = o.c0 / ts;
+            kc = o.c0 - tk * ts;
+        } else {
+            ts = 0x40000000;
+            ti = 0;
+            rowoff = gr;
+            tk = 0;
+            kc = o.c0;
+        }
+        base = p + tix(ti, tk, ym, tri) * te + rowoff + (long long)kc * ld;
+        voff = lane_off;
+    }
+    __device__ __forceinline__ void load(d2 &x0, d2 &x1) const {
+        const double *q = base + voff;
+        x0 = *reinterpret_cast<const d2 *>(q);
+        x1 = *reinterpret_cast<const d2 *>(q + 2);
+    }
+    // FLAT: the K range never leaves the current tile (no tile crossing)
+    template <bool FLAT>
+    __device__ __forceinline__ void advance(bool go) {
+        if (FLAT) {
+            base += go ? 4 * ld : 0;
+            return;
+        }
+        if (!go) return;
+        kc += 4;
+        if (kc >= ts) {
+            kc -= ts;
+            ++tk;
+            base = p + tix(ti, tk, ym, tri) * te + rowoff + (long long)kc * ld;
+        } else {
+            base += 4 * ld;
+        }
+    }
+};
+
+template <int MODE, int KIND, bool FLAT>
+__global__ __launch_bounds__(NT, 2) void k_dgemm_dtv(KArgs a) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    long long L = xcd_remap(blockIdx.x, a.nblk);
+    int mbi, nbi;
+    decode_block<MODE>(L, a.mb, a.nb, mbi, nbi);
+    const int m0 = mbi * 128, n0 = nbi * 128;
+    if (a.lower && (a.C.r0 + m0 + 127) < (a.C.c0 + n0)) return;
+
+    const int r = lane & 15, kq = lane >> 4;
+    DStream la, lb;
+    la.init(a.A, m0, wm * 64 + 4 * r + kq * (int)a.A.ld);
+    lb.init(a.B, n0, wn * 64 + 4 * r + kq * (int)a.B.ld);
+
+    d4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    d2 ra[4][2], rb[4][2];
+    const int ns = a.K >> 2;   // K slabs (a multiple of 4, at least 4)
+#pragma unroll
+    for (int q = 0; q < DPD; ++q) {
+        la.load(ra[q][0], ra[q][1]);
+        lb.load(rb[q][0], rb[q][1]);
+        la.advance<FLAT>(true);
+        lb.advance<FLAT>(true);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int s0 = 0; s0 < ns; s0 += 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            // the load stream is at slab min(s + DPD, ns - 1)
+            const int nx = (q + DPD) & 3;
+            la.load(ra[nx][0], ra[nx][1]);
+            lb.load(rb[nx][0], rb[nx][1]);
+            const bool go = s0 + q + DPD + 1 < ns;
+            la.advance<FLAT>(go);
+            lb.advance<FLAT>(go);
+            __builtin_amdgcn_sched_barrier(0);
+            const double fa[4] = {ra[q][0].x, ra[q][0].y, ra[q][1].x, ra[q][1].y};
+            const double fb[4] = {rb[q][0].x, rb[q][0].y, rb[q][1].x, rb[q][1].y};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    // acc[t][u][e] (MFMA output element e at row kq + 4e, column r of its
+    // 16 x 16 block) is C row 4 (kq + 4e) + t, column 4 r + u of the wave
+    // block: for fixed (u, e) the four t are consecutive rows.
+    const Opnd &Co = a.C;
+    const int gr0 = Co.r0 + m0, gc0 = Co.c0 + n0;
+    double *cb;
+    if (Co.ts > 0) {
+        const int ti = gr0 / Co.ts, tk = gc0 / Co.ts;
+        cb = Co.p + tix(ti, tk, Co.ym, Co.tri) * Co.te + (gr0 - ti * Co.ts) +
+             (long long)(gc0 - tk * Co.ts) * Co.ld;
+    } else {
+        cb = Co.p + gr0 + (long long)gc0 * Co.ld;
+    }
+    const bool diag = a.lower && gr0 < gc0 + 128;
+    const int ldc = (int)Co.ld;
+    const double alpha = a.alpha, beta = a.beta;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int cl = wn * 64 + 4 * r + u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int rl = wm * 64 + 4 * (kq + 4 * e);
+            double *c = cb + (rl + cl * ldc);
+            double v[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = alpha * acc[t][u][e];
+            if (!diag) {
+                d2 *c2 = reinterpret_cast<d2 *>(c);
+                d2 w0 = {v[0], v[1]}, w1 = {v[2], v[3]};
+                if (beta != 0.0) {
+                    const d2 o0 = c2[0], o1 = c2[1];
+                    w0 += beta * o0;
+                    w1 += beta * o1;
+                }
+                c2[0] = w0;
+                c2[1] = w1;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (gr0 + rl + t < gc0 + cl) continue;
+                    c[t] = beta != 0.0 ? v[t] + beta * c[t] : v[t];
+                }
+            }
+        }
+    }
+}
+
 // ---- few right-hand sides -------------------------------------------------
 // C (M x N) = beta*C + alpha * A * B^T for the solves' right-hand-side
 // panels: A and C plain, B a packed (or plain) operand streamed once.  One
@@ -603,6 +773,18 @@ hipError_t launch_one(const Gemm &g, hipStream_t s) {
                       (g.C.ts <= 0 || ((g.C.ts % BN) == 0 && (g.C.c0 % BN) == 0)) &&
                       g.A.ld * BK < 0x7fffffffLL && g.B.ld * BK < 0x7fffffffLL &&
                       g.C.ld * BN < 0x7fffffffLL;
+    if constexpr (BM == 128 && BN == 128 && !TB) {
+        if (full) {
+            auto flat = [&](const Opnd &o) { return o.ts <= 0 || (o.c0 % o.ts) + g.K <= o.ts; };
+            if (flat(g.A) && flat(g.B))
+                hipLaunchKernelGGL((k_dgemm_dtv<MODE, KIND, true>), dim3((unsigned)a.nblk), dim3(NT),
+                                   0, s, a);
+            else
+                hipLaunchKernelGGL((k_dgemm_dtv<MODE, KIND, false>), dim3((unsigned)a.nblk), dim3(NT),
+                                   0, s, a);
+            return hipGetLastError();
+        }
+    }
     if (full)
         hipLaunchKernelGGL((k_dgemm_nt<BM, BN, MODE, KIND, TB, !TB>), dim3((unsigned)a.nblk),
                            dim3(NT), 0, s, a);
