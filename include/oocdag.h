@@ -221,6 +221,13 @@ int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long lon
  * tiles, and the left-looking MFMA kernel (T = 64, RB = 0). */
 int oocdag_batch_set_variant(int precision, int variant);
 
+/* Chooses the kernel of the FP32 trailing update in the mixed-precision
+ * factorization (oocdag_mpir_solve_tiles and the server): 0 = operands
+ * loaded straight into MFMA registers (default), 1 = LDS-staged.  A test and
+ * tuning hook with no reference counterpart; returns the number of variants
+ * (2) or OOCDAG_EARG. */
+int oocdag_mp_set_update_variant(int variant);
+
 /* ---------------------------------------------------------------------------
  * Continuous-batching solve server (no reference counterpart).  Requests are
  * served by `slots` worker threads with their own HIP streams, admitted
@@ -250,6 +257,9 @@ typedef struct {
     double backward_error;
     double queue_seconds, service_seconds, latency_seconds;
     double x_checksum;           /* sum of the solution's entries */
+    double input_seconds;        /* part of service_seconds spent materializing the
+                                    request's matrix (A = G G^T / n + mu I, n^3 flops)
+                                    and right-hand sides; 0 on a factor-cache hit */
 } oocdag_result;
 
 typedef struct oocdag_server oocdag_server;

@@ -108,6 +108,7 @@ SIGNATURES.update({
     "oocdag_batch_generate": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
                                         C.c_double, C.c_ulonglong]),
     "oocdag_batch_set_variant": (C.c_int, [C.c_int, C.c_int]),
+    "oocdag_mp_set_update_variant": (C.c_int, [C.c_int]),
     "oocdag_batched_solve": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _ip,
                                        _ip, C.POINTER(BatchStats)]),
@@ -124,7 +125,8 @@ class Result(C.Structure):
                 ("batch_size", C.c_int), ("precision_used", C.c_int), ("fell_back", C.c_int),
                 ("iterations", C.c_int), ("backward_error", C.c_double),
                 ("queue_seconds", C.c_double), ("service_seconds", C.c_double),
-                ("latency_seconds", C.c_double), ("x_checksum", C.c_double)]
+                ("latency_seconds", C.c_double), ("x_checksum", C.c_double),
+                ("input_seconds", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -402,6 +404,15 @@ def mpir_solve_tiles(A: DeviceBuffer, n: int, nb: int, B: DeviceBuffer, nrhs: in
                                          PRECISIONS.get(precision, precision), tol, max_iter,
                                          C.byref(st)), "mpir_solve_tiles")
     return st.as_dict()
+
+
+def mp_set_update_variant(variant: int) -> int:
+    """Choose the FP32 trailing-update kernel of the mixed-precision factor
+    (0: operands straight into MFMA registers, the default; 1: LDS-staged)."""
+    rc = lib().oocdag_mp_set_update_variant(variant)
+    if rc < 0:
+        raise OocdagError(f"mp_set_update_variant({variant}) failed")
+    return rc
 
 
 class Batch:

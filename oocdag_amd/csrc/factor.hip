@@ -32,14 +32,22 @@ __device__ __forceinline__ long long opnd_off(const Opnd &o, int r, int c) {
     return t * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
 }
 
+// Lane l's value of v, broadcast to the whole wave (two v_readlane_b32 into
+// scalar registers: no LDS round trip).
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
 // Columns [K0, K1) of the trailing update of factorization step J: lane i
-// subtracts l_ij * l_kj from its a_ik, column J of L read from LDS.
+// subtracts l_ij * l_kj from its a_ik, with l_kj read from lane k.
 template <int J, int K0, int K1>
-__device__ __forceinline__ void chol_cols(double (&a)[PB], const double *col) {
+__device__ __forceinline__ void chol_cols(double (&a)[PB]) {
     if constexpr (K1 > K0) {
         double cv[K1 - K0];
 #pragma unroll
-        for (int k = K0; k < K1; ++k) cv[k - K0] = col[k];   // one LDS round trip per chunk
+        for (int k = K0; k < K1; ++k) cv[k - K0] = lane_value(a[J], k);
 #pragma unroll
         for (int k = K0; k < K1; ++k) {
             a[k] -= a[J] * cv[k - K0];
@@ -55,66 +63,68 @@ __device__ __forceinline__ void chol_cols(double (&a)[PB], const double *col) {
 // every register index a compile-time constant).  bad: first failed pivot
 // (1-based), uniform.
 template <int J>
-__device__ __forceinline__ void chol_step(double (&a)[PB], double *col, int lane, int &bad) {
+__device__ __forceinline__ void chol_step(double (&a)[PB], int lane, int &bad) {
     if constexpr (J < PB) {
         // an opaque copy of the lane index keeps the compiler from forming
         // all 64 steps' lane masks up front (they would not fit in SGPRs)
         int ln = lane;
         asm volatile("" : "+v"(ln));
-        if (ln == J) col[PB] = a[J];
-        __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): single wave, in-order LDS
-        const double d = col[PB];
+        const double d = lane_value(a[J], J);
         if (bad == 0 && !(d > 0.0)) bad = J + 1;
         const double sd = sqrt(d);
         const double rd = 1.0 / sd;
         // lanes above the pivot keep 0 so later steps leave them unchanged
         a[J] = ln > J ? a[J] * rd : (ln == J ? sd : 0.0);
-        col[ln] = a[J];
         __builtin_amdgcn_sched_barrier(0);
-        chol_cols<J, J + 1, (J + 1 > 16 ? J + 1 : 16)>(a, col);
-        chol_cols<J, (J + 1 > 16 ? J + 1 : 16), (J + 1 > 32 ? J + 1 : 32)>(a, col);
-        chol_cols<J, (J + 1 > 32 ? J + 1 : 32), (J + 1 > 48 ? J + 1 : 48)>(a, col);
-        chol_cols<J, (J + 1 > 48 ? J + 1 : 48), PB>(a, col);
-        chol_step<J + 1>(a, col, lane, bad);
+        chol_cols<J, J + 1, (J + 1 > 16 ? J + 1 : 16)>(a);
+        chol_cols<J, (J + 1 > 16 ? J + 1 : 16), (J + 1 > 32 ? J + 1 : 32)>(a);
+        chol_cols<J, (J + 1 > 32 ? J + 1 : 32), (J + 1 > 48 ? J + 1 : 48)>(a);
+        chol_cols<J, (J + 1 > 48 ? J + 1 : 48), PB>(a);
+        chol_step<J + 1>(a, lane, bad);
     }
 }
 
-// Rows [I0, I1) of step K of the inversion.
+// Rows [I0, I1) of step K of the inversion: l_iK is lane i's a[K].
 template <int K, int I0, int I1>
-__device__ __forceinline__ void inv_rows(double (&y)[PB], const double *S) {
+__device__ __forceinline__ void inv_rows(double (&y)[PB], const double (&a)[PB]) {
+    if constexpr (I1 > I0) {
+        double lv[I1 - I0];
 #pragma unroll
-    for (int i = I0; i < I1; ++i) y[i] -= S[K * PS + i] * y[K];
+        for (int i = I0; i < I1; ++i) lv[i - I0] = lane_value(a[K], i);
+#pragma unroll
+        for (int i = I0; i < I1; ++i) y[i] -= lv[i - I0] * y[K];
+    }
     __builtin_amdgcn_sched_barrier(0);
 }
 
 // Step K of the inversion: lane c holds column c of L^-1 being formed;
-// column K of L is read from LDS (broadcast reads) 16 rows at a time.
+// column K of L comes from the lanes' registers (row i in lane i).
 template <int K>
-__device__ __forceinline__ void inv_step(double (&y)[PB], const double *S) {
+__device__ __forceinline__ void inv_step(double (&y)[PB], const double (&a)[PB]) {
     if constexpr (K < PB) {
-        y[K] *= 1.0 / S[K * PS + K];
-        inv_rows<K, K + 1, (K + 1 > 16 ? K + 1 : 16)>(y, S);
-        inv_rows<K, (K + 1 > 16 ? K + 1 : 16), (K + 1 > 32 ? K + 1 : 32)>(y, S);
-        inv_rows<K, (K + 1 > 32 ? K + 1 : 32), (K + 1 > 48 ? K + 1 : 48)>(y, S);
-        inv_rows<K, (K + 1 > 48 ? K + 1 : 48), PB>(y, S);
-        inv_step<K + 1>(y, S);
+        y[K] *= 1.0 / lane_value(a[K], K);
+        inv_rows<K, K + 1, (K + 1 > 16 ? K + 1 : 16)>(y, a);
+        inv_rows<K, (K + 1 > 16 ? K + 1 : 16), (K + 1 > 32 ? K + 1 : 32)>(y, a);
+        inv_rows<K, (K + 1 > 32 ? K + 1 : 32), (K + 1 > 48 ? K + 1 : 48)>(y, a);
+        inv_rows<K, (K + 1 > 48 ? K + 1 : 48), PB>(y, a);
+        inv_step<K + 1>(y, a);
     }
 }
 
 // One wave64 factors a 64 x 64 diagonal block and inverts the triangle,
 // register-resident.  Factorization (right-looking, lane i owns row i in
-// registers): at step j the pivot is read back from LDS, column j is scaled
-// by 1/l_jj (as LAPACK dpotf2 scales by the reciprocal) and published in
-// LDS, and every lane subtracts l_ij * l_kj from its a_ik, k > j -- the same
-// subtraction order per element as the left-looking dot products.
-// Inversion (lane c owns column c of L^-1): y = e_c, then for k = 0..63
-// y_k *= 1/l_kk and y_i -= l_ik y_k for i > k, with column k of L read as
-// LDS broadcasts.  do_factor = 0: the block already holds L; only the
-// inverse is formed and the block is left untouched.
+// registers): at step j the pivot and column j are read from the owning
+// lanes' registers (v_readlane), column j is scaled by 1/l_jj (as LAPACK
+// dpotf2 scales by the reciprocal) and every lane subtracts l_ij * l_kj from
+// its a_ik, k > j -- the same subtraction order per element as the
+// left-looking dot products.  Inversion (lane c owns column c of L^-1):
+// y = e_c, then for k = 0..63 y_k *= 1/l_kk and y_i -= l_ik y_k for i > k,
+// l_ik read from lane i.  LDS only transposes L^-1 for the coalesced store.
+// do_factor = 0: the block already holds L; only the inverse is formed and
+// the block is left untouched.
 __global__ __launch_bounds__(64) void k_potrf64(Opnd P, int gcol0, double *dinv, int *info,
                                                 int do_factor) {
-    __shared__ double S[PB * PS];   // L (then L^-1), column-major, column stride PS
-    __shared__ double col[PB + 1];  // column j of L, [PB]: the pivot
+    __shared__ double S[PB * PS];   // L^-1, column-major, column stride PS
     const int lane = threadIdx.x;
     const long long base = opnd_off(P, 0, 0);  // block never straddles a tile
     const long long ld = P.ld;
@@ -129,7 +139,7 @@ __global__ __launch_bounds__(64) void k_potrf64(Opnd P, int gcol0, double *dinv,
 
     if (do_factor) {
         int bad = 0;
-        chol_step<0>(a, col, lane, bad);
+        chol_step<0>(a, lane, bad);
         if (bad && lane == 0) atomicMin(info, gcol0 + bad);
         // opaque base: the write-back recomputes its addresses instead of
         // keeping the 64 load addresses alive through the factorization
@@ -140,15 +150,10 @@ __global__ __launch_bounds__(64) void k_potrf64(Opnd P, int gcol0, double *dinv,
             if (lane >= c) gs[lane + c * ld] = a[c];
     }
 
-#pragma unroll
-    for (int c = 0; c < PB; ++c) S[c * PS + lane] = a[c];   // entries above the diagonal unused
-    __syncthreads();
-
     double y[PB];
 #pragma unroll
     for (int i = 0; i < PB; ++i) y[i] = (i == lane) ? 1.0 : 0.0;
-    inv_step<0>(y, S);
-    __syncthreads();
+    inv_step<0>(y, a);
 #pragma unroll
     for (int i = 0; i < PB; ++i) S[lane * PS + i] = y[i];   // column lane of L^-1
     __syncthreads();
@@ -319,7 +324,8 @@ hipError_t factor_column(const Opnd &P, int row0, int mrows, int ncols, int gcol
 
 int factor_panel_la(const Opnd &P, int NB, int Ym, int Yn, int gcol0, double *dinv,
                     size_t dinv_stride, int *dinfo, hipStream_t s0, hipStream_t s1,
-                    PanelUpdater &up, const std::function<int(int)> &on_column) {
+                    PanelUpdater &up, const std::function<int(int)> &on_column,
+                    const std::function<int(int)> *after_updates) {
     hipEvent_t ev_fact, ev_bulk[2];
     if (hipEventCreateWithFlags(&ev_fact, hipEventDisableTiming) != hipSuccess) return -4;
     if (hipEventCreateWithFlags(&ev_bulk[0], hipEventDisableTiming) != hipSuccess) return -4;
@@ -346,7 +352,10 @@ int factor_panel_la(const Opnd &P, int NB, int Ym, int Yn, int gcol0, double *di
         LA_TRY(factor_column(P, j * NB, (Ym - j) * NB, NB, gcol0 + j * NB,
                              dinv + (size_t)j * dinv_stride, dinfo, s0));
         if ((rc = on_column(j)) != 0) return rc;
-        if (j + 1 >= Yn) break;
+        if (j + 1 >= Yn) {
+            if (after_updates && (rc = (*after_updates)(j)) != 0) return rc;
+            break;
+        }
         LA_TRY(up.prepare(j, s0));
         // column j+1 also received bulk(j-1)
         if (j >= 1) LA_TRY(hipStreamWaitEvent(s0, ev_bulk[(j - 1) & 1], 0));
@@ -358,6 +367,7 @@ int factor_panel_la(const Opnd &P, int NB, int Ym, int Yn, int gcol0, double *di
             LA_TRY(hipEventRecord(ev_bulk[j & 1], s1));
             any_bulk = true;
         }
+        if (after_updates && (rc = (*after_updates)(j)) != 0) return rc;
     }
     if (any_bulk) {
         LA_TRY(hipEventRecord(ev_fact, s1));

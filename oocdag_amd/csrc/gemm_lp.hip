@@ -19,8 +19,11 @@
 // probe the first time the library runs (k_probe_mfma), which also checks
 // the operand lane layouts assumed below; a mismatch is reported as an error.
 #include "ooc_internal.h"
+#include "../../include/oocdag.h"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdint>
 #include <cmath>
 #include <mutex>
 
@@ -28,6 +31,7 @@ namespace ooc {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef short s8 __attribute__((ext_vector_type(8)));
 
@@ -300,6 +304,102 @@ __global__ __launch_bounds__(NT, 2) void k_update_f32(LpArgs a) {
     }
 }
 
+// FP32, operands straight into MFMA registers (no LDS, no barriers).  Each
+// wave (wm, wn) owns a 64 x 64 block of C.  Lane (r = l%16, kq = l/16) loads
+// rows 4r..4r+3 of its 64 X rows at column kq of the current 4-wide K slab
+// (one 16-byte load; 16 lanes cover 256 contiguous bytes of a column-major X
+// column), and the same for B, so MFMA block t holds rows 4R + t of the wave
+// block (R = row inside the 16 x 16 MFMA block): a row/column permutation
+// the epilogue undoes.  A ring of RING slabs keeps AHEAD slabs of loads in
+// flight ahead of the MFMAs; past the last slab the load address stops
+// advancing (the extra loads re-read the last slab and are never used).
+// Requires K % (4 * RING) == 0, K >= 4 * (AHEAD + 1), 16-byte aligned rows.
+template <int RING, int AHEAD, int ROWMAP>
+__global__ __launch_bounds__(NT, 2) void k_update_f32_dtv(LpArgs a) {
+    static_assert((RING & (RING - 1)) == 0 && AHEAD < RING, "ring");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    long long L = xcd_remap(blockIdx.x, a.nblk);
+    int mbi, nbi;
+    tri_decode(L, a.mb, a.nbk, mbi, nbi);
+    const int m0 = mbi * BM, n0 = nbi * BM;
+    const int r = lane & 15, kq = lane >> 4;
+    const float *X = (const float *)a.X;
+    const float *pa = X + (long long)kq * a.ldx + m0 + wm * 64 + 4 * r;
+    const float *pb = X + (long long)kq * a.ldx + n0 + wn * 64 + 4 * r;
+    const long long step = 4 * a.ldx;
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+    f4 ra[RING], rb[RING];
+    const int ns = a.K >> 2;
+#pragma unroll
+    for (int q = 0; q < AHEAD; ++q) {
+        ra[q] = *reinterpret_cast<const f4 *>(pa);
+        rb[q] = *reinterpret_cast<const f4 *>(pb);
+        pa += step;
+        pb += step;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int s0 = 0; s0 < ns; s0 += RING) {
+#pragma unroll
+        for (int q = 0; q < RING; ++q) {
+            const int nx = (q + AHEAD) & (RING - 1);
+            ra[nx] = *reinterpret_cast<const f4 *>(pa);
+            rb[nx] = *reinterpret_cast<const f4 *>(pb);
+            const bool go = s0 + q + AHEAD + 1 < ns;
+            pa += go ? step : 0;
+            pb += go ? step : 0;
+            __builtin_amdgcn_sched_barrier(0);
+            const float fa[4] = {ra[q].x, ra[q].y, ra[q].z, ra[q].w};
+            const float fb[4] = {rb[q].x, rb[q].y, rb[q].z, rb[q].w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    // acc[t][u][e] is the MFMA output element e of block (t, u): MFMA row
+    // i = (ROWMAP ? kq + 4e : 4kq + e), column r, i.e. wave-block row 4i + t,
+    // column 4r + u.  For fixed (u, e) the four t are consecutive rows.
+    const long long per = (long long)a.nb * a.nb;
+    const int gr0 = a.r0 + m0, gc0 = a.r0 + n0;
+    const int ti = gr0 / a.nb, tk = gc0 / a.nb;
+    double *cb = a.C + tix(ti, tk, a.c_ym, 1) * per + (gr0 - ti * a.nb) +
+                 (long long)(gc0 - tk * a.nb) * a.nb;
+    const bool diag = m0 == n0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int cl = wn * 64 + 4 * r + u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int rl = wm * 64 + 4 * (ROWMAP ? kq + 4 * e : 4 * kq + e);
+            double *c = cb + rl + (long long)cl * a.nb;
+            if (!diag) {
+                d2 *c2 = reinterpret_cast<d2 *>(c);
+                d2 w0 = c2[0], w1 = c2[1];
+                w0.x -= (double)acc[0][u][e];
+                w0.y -= (double)acc[1][u][e];
+                w1.x -= (double)acc[2][u][e];
+                w1.y -= (double)acc[3][u][e];
+                c2[0] = w0;
+                c2[1] = w1;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (rl + t >= cl) c[t] -= (double)acc[t][u][e];
+            }
+        }
+    }
+}
+
 constexpr int BK16 = 64;            // bf16 per K step
 constexpr int S16 = BK16 + 8;       // LDS row stride (bf16), rows are m
 
@@ -393,6 +493,8 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
 
 int g_rowmap[2] = {-1, -1};
 std::mutex g_probe_mu;
+// FP32 update kernel: 0 operands straight to MFMA registers, 1 LDS-staged
+std::atomic<int> g_f32_variant{0};
 
 }  // namespace
 
@@ -469,7 +571,13 @@ hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const
     for (int c = 0; c < a.nbk; ++c) n += a.mb - c;
     a.nblk = n;
     a.rowmap = rowmap;
-    if (mode == 0)
+    const bool direct = g_f32_variant.load() == 0 && K % 32 == 0 && K >= 28 &&
+                        ((uintptr_t)X & 15) == 0;
+    if (mode == 0 && direct && rowmap == 0)
+        hipLaunchKernelGGL((k_update_f32_dtv<8, 6, 0>), dim3((unsigned)n), dim3(NT), 0, s, a);
+    else if (mode == 0 && direct)
+        hipLaunchKernelGGL((k_update_f32_dtv<8, 6, 1>), dim3((unsigned)n), dim3(NT), 0, s, a);
+    else if (mode == 0)
         hipLaunchKernelGGL(k_update_f32, dim3((unsigned)n), dim3(NT), 0, s, a);
     else
         hipLaunchKernelGGL(k_update_bf16, dim3((unsigned)n), dim3(NT), 0, s, a);
@@ -477,3 +585,11 @@ hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const
 }
 
 }  // namespace ooc
+
+// Chooses the FP32 low-precision update kernel (test and tuning hook):
+// 0 = operands straight into MFMA registers (default), 1 = LDS-staged.
+extern "C" int oocdag_mp_set_update_variant(int variant) {
+    if (variant != 0 && variant != 1) return OOCDAG_EARG;
+    ooc::g_f32_variant.store(variant);
+    return 2;
+}

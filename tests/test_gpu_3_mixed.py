@@ -71,7 +71,35 @@ def test_mpir_matches_fp64_oracle(gpu, prec, mu, n, nb):
     ferr = np.abs(X - Xref).max() / np.abs(Xref).max()
     assert ferr <= max(1e2 * kappa * EPS, 1e-13)
     if prec == "fp32" and mu == 1e-2:
+        # kappa ~ 4e2: an FP32 factor contracts the error by ~kappa * 2^-24
+        # per step, so a wrong update (which refinement alone would hide
+        # behind many steps or the FP64 fallback) shows here
         assert st["fell_back"] == 0 and st["precision_used"] == 1
+        assert st["iterations"] <= 3
+
+
+def test_mpir_fp32_update_kernels_agree(gpu):
+    """Both FP32 update kernels accumulate every element over the same
+    4-wide K slabs in the same order, so the factorizations, the refinement
+    and the solution are bit-identical."""
+    ooc = gpu
+    n, nb, nrhs = 6144, 128, 4
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=5, stream_id=2)
+    out = []
+    try:
+        for variant in (1, 0):
+            ooc.mp_set_update_variant(variant)
+            ooc.gram_tiles(buf, n, nb, 1e-2, seed=5)
+            dX = ooc.DeviceBuffer(n * nrhs * 8)
+            st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, "fp32", tol=8 * EPS, max_iter=30)
+            assert st["fell_back"] == 0 and st["iterations"] <= 3, st
+            out.append((st["iterations"], dX.to_array((n, nrhs))))
+    finally:
+        ooc.mp_set_update_variant(0)
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
 
 
 def test_mpir_non_spd_reports_info(gpu):
