@@ -4,14 +4,19 @@
 // The reference runs Cholesky() (oocdag.cpp:18-112) as a QUARK task DAG on
 // four host threads with cuBLAS on the legacy NULL stream and pageable
 // synchronous copies (oocquark.cpp:16-63, oocdag.cpp:279-289).  Here one host
-// thread drives three HIP streams:
+// thread queues all kernels on four HIP streams:
 //   h2d  : pinned-slot -> HBM copies of Y panels and X tile columns,
-//   comp : all kernels (updates, diagonal factorization, solves),
-//   d2h  : factored tile columns back to pinned slots,
+//   comp : updates and the diagonal factorization (critical path),
+//   bulk : the look-ahead bulk update of each panel,
+//   d2h  : factored tile columns back to the host,
 // ordered by events: an X buffer is refilled only after the update that read
 // it (double-buffered X), a factored tile column is copied out as soon as it
-// is final while later columns are still being computed, and host packing of
-// the next X column overlaps the update running on the current one.
+// is final while later columns are still being computed, host packing of the
+// next X column overlaps the update running on the current one, and the next
+// panel is copied into the device tiles of finished columns while the current
+// panel is factored.  Two helper threads keep host staging off the queueing
+// thread: the loader packs and queues the next panel's copies, the unpacker
+// empties device->host slots into the caller's array.
 #include "ooc_internal.h"
 #include "../../include/oocdag.h"
 
@@ -19,6 +24,11 @@
 #include <atomic>
 #include <chrono>
 #include <climits>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -180,11 +190,13 @@ bool make_plan(int N, int NB, size_t memsize, Plan &pl, int force_nx = 0) {
 
 // Packs tiles (ti0 .. bb-1, tj) of A into dst (consecutive NB^2 tiles),
 // zero-padding past N and putting 1 on the padded diagonal of tile (tj,tj).
-void pack_column(const double *A, int N, int NB, int LDA, int tj, int ti0, double *dst) {
+void pack_column(const double *A, int N, int NB, int LDA, int tj, int ti0, double *dst,
+                 int nth = 0) {
     const int bb = (N + NB - 1) / NB;
     const int ntile = bb - ti0;
     const size_t nb2 = (size_t)NB * NB;
-#pragma omp parallel for schedule(static) collapse(2)
+    if (nth <= 0) nth = omp_get_max_threads();
+#pragma omp parallel for schedule(static) collapse(2) num_threads(nth)
     for (int t = 0; t < ntile; ++t) {
         for (int c = 0; c < NB; ++c) {
             const int ti = ti0 + t;
@@ -203,11 +215,13 @@ void pack_column(const double *A, int N, int NB, int LDA, int tj, int ti0, doubl
 
 // Writes tiles (ti0 .. bb-1, tj) from src back into A: valid rows/columns
 // only, and only the lower triangle of the diagonal tile.
-void unpack_column(double *A, int N, int NB, int LDA, int tj, int ti0, const double *src) {
+void unpack_column(double *A, int N, int NB, int LDA, int tj, int ti0, const double *src,
+                   int nth = 0) {
     const int bb = (N + NB - 1) / NB;
     const int ntile = bb - ti0;
     const size_t nb2 = (size_t)NB * NB;
-#pragma omp parallel for schedule(static) collapse(2)
+    if (nth <= 0) nth = omp_get_max_threads();
+#pragma omp parallel for schedule(static) collapse(2) num_threads(nth)
     for (int t = 0; t < ntile; ++t) {
         for (int c = 0; c < NB; ++c) {
             const int ti = ti0 + t;
@@ -224,15 +238,113 @@ void unpack_column(double *A, int N, int NB, int LDA, int tj, int ti0, const dou
 
 // Host copy split across the OpenMP threads (staging of host tiles that are
 // not page-locked).
-void par_memcpy(double *dst, const double *src, size_t n) {
+void par_memcpy(double *dst, const double *src, size_t n, int nth = 0) {
     const size_t chunk = (size_t)1 << 20;   // doubles
     const long long nch = (long long)((n + chunk - 1) / chunk);
-#pragma omp parallel for schedule(static)
+    if (nth <= 0) nth = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nth)
     for (long long c = 0; c < nch; ++c) {
         const size_t o = (size_t)c * chunk;
         memcpy(dst + o, src + o, std::min(chunk, n - o) * sizeof(double));
     }
 }
+
+// A host thread that runs posted tasks in order (staging work that must not
+// block the thread queueing kernels).  post() returns the task's sequence
+// number; wait_done(q) returns once tasks 1..q have run; drain() waits for
+// all of them and returns the first failure (its message is copied into this
+// thread's error string).  After a failure, and for tasks still queued when
+// the worker is destroyed, the remaining tasks are skipped.
+class Worker {
+  public:
+    Worker() : th_([this] { loop(); }) {}
+    ~Worker() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    long long post(std::function<int()> f) {
+        std::lock_guard<std::mutex> lk(mu_);
+        q_.push_back(std::move(f));
+        cv_.notify_all();
+        return ++posted_;
+    }
+    void wait_done(long long q) {
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return finished_ >= q; });
+    }
+    int drain() {
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return finished_ == posted_; });
+        if (rc_) g_err = err_;
+        return rc_;
+    }
+
+  private:
+    void loop() {
+        for (;;) {
+            std::function<int()> f;
+            bool skip;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;
+                f = std::move(q_.front());
+                q_.pop_front();
+                skip = rc_ != 0 || stop_;   // after a failure or at teardown
+            }
+            const int r = skip ? 0 : f();
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (r && !rc_) {
+                    rc_ = r;
+                    err_ = g_err;   // this thread's message
+                }
+                ++finished_;
+            }
+            done_cv_.notify_all();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::deque<std::function<int()>> q_;
+    long long posted_ = 0, finished_ = 0;
+    int rc_ = 0;
+    std::string err_;
+    bool stop_ = false;
+    std::thread th_;   // last: starts after the members above exist
+};
+
+// A monotone counter one thread raises and another waits on; abort() releases
+// every waiter (wait() then returns false).
+struct Gate {
+    std::mutex mu;
+    std::condition_variable cv;
+    int value = -1;
+    bool aborted = false;
+    void set(int v) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            value = v;
+        }
+        cv.notify_all();
+    }
+    bool wait(int v) {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return aborted || value >= v; });
+        return !aborted;
+    }
+    void abort() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            aborted = true;
+        }
+        cv.notify_all();
+    }
+};
 
 // True when p is page-locked host memory known to HIP.
 bool host_pinned(const void *p) {
@@ -277,7 +389,8 @@ struct Fp64Updater : PanelUpdater {
 // gcol0 = global column of panel column 0.  on_column(j) is called after
 // column j's factor kernels were queued on s0.
 int factor_panel(double *dY, int NB, int Ym, int Yn, int gcol0, double *dinv, int *dinfo,
-                 hipStream_t s0, hipStream_t s1, const std::function<int(int)> &on_column) {
+                 hipStream_t s0, hipStream_t s1, const std::function<int(int)> &on_column,
+                 const std::function<int(int)> *after_updates = nullptr) {
     Fp64Updater up;
     up.P = panel(dY, NB, Ym, 1);
     up.NB = NB;
@@ -303,7 +416,8 @@ int factor_panel(double *dY, int NB, int Ym, int Yn, int gcol0, double *dinv, in
         }
         return OOCDAG_OK;
     };
-    int rc = factor_panel_la(up.P, NB, Ym, Yn, gcol0, dinv, 0, dinfo, s0, s1, up, progress);
+    int rc = factor_panel_la(up.P, NB, Ym, Yn, gcol0, dinv, 0, dinfo, s0, s1, up, progress,
+                             after_updates);
     for (hipEvent_t e : marks) (void)hipEventDestroy(e);
     if (rc == -4) {
         set_err("HIP error in the panel factorization: %s", hipGetErrorString(hipGetLastError()));
@@ -366,10 +480,36 @@ struct Scratch {
     }
 };
 
+// Device tiles that tile column c of the next panel (Ym2 tile rows, packed
+// from tile 0) occupies overlap tile columns 0..k of the current panel (Ym
+// tile rows, Yn columns); returns k (-1: none), so the next panel's column
+// may be copied in once those columns of the current panel are free.
+int last_overlapping_column(int Ym, int Yn, int Ym2, int c) {
+    const long long end = tile_index(c, c, Ym2, 1) + (Ym2 - c);
+    int k = -1;
+    for (int q = 0; q < Yn && tile_index(q, q, Ym, 1) < end; ++q) k = q;
+    return k;
+}
+
 // Out-of-core factorization.  Host side: the column-major array A (LDA) of
 // the reference entry point, packed through pinned slots, or -- tiles_host --
 // A already holds the packed lower tiles (the device layout; pinned memory
 // for full-speed DMA), so every tile-column copy is one contiguous DMA.
+//
+// Per panel (widths from the planner):
+//   1. its tile columns are in HBM: copied by this thread for the first
+//      panel, prefetched during the previous panel otherwise (step 4);
+//   2. left-looking update from every prior tile column, each re-read from
+//      the host into one of the X buffers (A2X + ooc_syrk, oocquark.cpp:
+//      301-356), the copy of the next X column overlapping the current update;
+//   3. in-core factorization with look-ahead (ooc_incore, oocquark.cpp:
+//      358-421); every finished tile column is copied back to the host at once
+//      (into a pinned slot that the unpack thread empties into A, or straight
+//      into pinned host tiles), so this thread never waits on a copy;
+//   4. meanwhile the loader thread copies the next panel's tile columns into
+//      the device tiles this panel no longer needs: a column's copy waits (on
+//      the GPU) for the events that mark the overlapped columns of this panel
+//      final, read by every update and copied out.
 int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, double *secs,
                  bool tiles_host = false) {
     Plan pl;
@@ -380,6 +520,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     const int bb = (N + NB - 1) / NB;
     const size_t nb2 = (size_t)NB * NB;
     const size_t col_bytes = (size_t)bb * nb2 * sizeof(double);
+    constexpr int ND = 3;   // device->host slots
 
     Streams st;
     HIPTRY(hipStreamCreateWithFlags(&st.h2d, hipStreamNonBlocking));
@@ -403,66 +544,98 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     int rc = sc.init(NB, st.comp);
     if (rc) return rc;
 
-    // pinned slots: 2 for host->device, 2 for device->host
-    double *slot[4] = {nullptr, nullptr, nullptr, nullptr};
-    struct FreeHost {
-        double **s;
-        ~FreeHost() {
-            for (int i = 0; i < 4; ++i)
-                if (s[i]) (void)hipHostFree(s[i]);
+    // pinned slots: 2 for host->device, ND for device->host
+    double *hslot_buf[2] = {nullptr, nullptr}, *dslot_buf[ND] = {};
+    std::vector<hipEvent_t> events;   // every event made below, destroyed on return
+    struct Cleanup {
+        double **h, **d;
+        std::vector<hipEvent_t> *ev;
+        ~Cleanup() {
+            for (int i = 0; i < 2; ++i)
+                if (h[i]) (void)hipHostFree(h[i]);
+            for (int i = 0; i < ND; ++i)
+                if (d[i]) (void)hipHostFree(d[i]);
+            for (hipEvent_t e : *ev) (void)hipEventDestroy(e);
         }
-    } fh{slot};
-    for (int i = 0; i < 4 && !direct; ++i)
-        if (hipHostMalloc(&slot[i], col_bytes, hipHostMallocDefault) != hipSuccess) {
+    } cleanup{hslot_buf, dslot_buf, &events};
+    for (int i = 0; i < 2 + ND && !direct; ++i) {
+        double **p = i < 2 ? &hslot_buf[i] : &dslot_buf[i - 2];
+        if (hipHostMalloc(p, col_bytes, hipHostMallocDefault) != hipSuccess) {
             set_err("pinned host allocation of %zu bytes failed", col_bytes);
             return OOCDAG_ENOMEM;
         }
-    hipEvent_t ev_slot[4], ev_xready[2], ev_xfree[2], ev_y, ev_col, ev_t0, ev_t1;
-    for (auto &e : ev_slot) HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    for (auto &e : ev_xready) HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    for (auto &e : ev_xfree) HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    HIPTRY(hipEventCreateWithFlags(&ev_y, hipEventDisableTiming));
-    HIPTRY(hipEventCreateWithFlags(&ev_col, hipEventDisableTiming));
-    HIPTRY(hipEventCreate(&ev_t0));
-    HIPTRY(hipEventCreate(&ev_t1));
-    bool slot_used[4] = {false, false, false, false};
+    }
+    auto new_event = [&](hipEvent_t *e) -> hipError_t {
+        hipError_t r = hipEventCreateWithFlags(e, hipEventDisableTiming);
+        if (r == hipSuccess) events.push_back(*e);
+        return r;
+    };
+    hipEvent_t ev_hslot[2], ev_xready[2], ev_xfree[2], ev_y, ev_xdone;
+    for (auto &e : ev_hslot) HIPTRY(new_event(&e));
+    for (auto &e : ev_xready) HIPTRY(new_event(&e));
+    for (auto &e : ev_xfree) HIPTRY(new_event(&e));
+    HIPTRY(new_event(&ev_y));
+    HIPTRY(new_event(&ev_xdone));
+    bool hslot_used[2] = {false, false};
+    int hslot = 0;
+    long long dslot_task[ND] = {};   // unpack task that last used each slot
     HIPTRY(hipStreamSynchronize(st.comp));
 
+    const int nth = std::max(1, omp_get_max_threads());
+    const int nth_half = std::max(1, nth / 2);
     double h2d_bytes = 0, d2h_bytes = 0;
-    CopyTimer t_h2d, t_d2h;
-    const double t0 = now();
-    int hslot = 0;
-    int nxb = pl.nx;
+    CopyTimer t_h2d, t_h2d_pre, t_d2h;
+    Gate freed;   // highest tile column of the current panel whose free-events exist
 
-    auto take_slot = [&](int s) -> int {
-        if (slot_used[s]) HIPTRY(hipEventSynchronize(ev_slot[s]));
-        slot_used[s] = true;
+    // One tile column (rows ti0.. of tile column tj) host -> device through
+    // H2D slot hslot (packed first, with `threads` threads) or straight from
+    // pinned host tiles.  before_copy() runs just before the copy is queued
+    // (it adds the stream waits the copy must follow).
+    auto h2d_column = [&](double *dst, int tj, int ti0, int threads, CopyTimer &timer,
+                          const std::function<int()> &before_copy) -> int {
+        const size_t bytes = (size_t)(bb - ti0) * nb2 * sizeof(double);
+        const double *src = tiles_host ? A + (size_t)tile_index(ti0, tj, bb, 1) * nb2 : nullptr;
+        if (!direct) {
+            if (hslot_used[hslot]) HIPTRY(hipEventSynchronize(ev_hslot[hslot]));
+            hslot_used[hslot] = true;
+            if (tiles_host)
+                par_memcpy(hslot_buf[hslot], src, bytes / sizeof(double), threads);
+            else
+                pack_column(A, N, NB, LDA, tj, ti0, hslot_buf[hslot], threads);
+            src = hslot_buf[hslot];
+        }
+        int r = before_copy();
+        if (r) return r;
+        HIPTRY(timer.copy(dst, src, bytes, hipMemcpyHostToDevice, st.h2d));
+        if (!direct) {
+            HIPTRY(hipEventRecord(ev_hslot[hslot], st.h2d));
+            hslot ^= 1;
+        }
+        h2d_bytes += bytes;
         return OOCDAG_OK;
     };
+    const std::function<int()> no_wait = [] { return OOCDAG_OK; };
+    // declared after everything their tasks use, so they are joined first;
+    // the gate is released before that (an early return leaves the loader
+    // waiting for columns that will not be factored)
+    Worker unpacker, loader;
+    struct AbortGate {
+        Gate *g;
+        ~AbortGate() { g->abort(); }
+    } abort_gate{&freed};
 
+    const double t0 = now();
     for (size_t p = 0, jb = 0; p < pl.width.size(); jb += pl.width[p], ++p) {
         const int Yn = pl.width[p];
         const int Ym = bb - (int)jb;
         // --- A2Y (oocquark.cpp:253-275): the panel's tile columns -> Y
-        for (int jj = 0; jj < Yn; ++jj) {
-            const int tj = (int)jb + jj;
-            size_t bytes = (size_t)(bb - tj) * nb2 * sizeof(double);
-            double *dst = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
-            if (direct) {
-                HIPTRY(t_h2d.copy(dst, A + (size_t)tile_index(tj, tj, bb, 1) * nb2, bytes,
-                                  hipMemcpyHostToDevice, st.h2d));
-            } else {
-                if ((rc = take_slot(hslot))) return rc;
-                if (tiles_host)
-                    par_memcpy(slot[hslot], A + (size_t)tile_index(tj, tj, bb, 1) * nb2,
-                               bytes / sizeof(double));
-                else
-                    pack_column(A, N, NB, LDA, tj, tj, slot[hslot]);
-                HIPTRY(t_h2d.copy(dst, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
-                HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
-                hslot ^= 1;
-            }
-            h2d_bytes += bytes;
+        if (p == 0) {
+            for (int jj = 0; jj < Yn; ++jj)
+                if ((rc = h2d_column(dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2, (int)jb + jj,
+                                     (int)jb + jj, nth, t_h2d, no_wait)))
+                    return rc;
+        } else if ((rc = loader.drain())) {   // prefetched while the previous panel ran
+            return rc;
         }
         HIPTRY(hipEventRecord(ev_y, st.h2d));
         HIPTRY(hipStreamWaitEvent(st.comp, ev_y, 0));
@@ -470,27 +643,12 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
         // --- left-looking updates from every prior tile column (oocdag.cpp:73-77)
         const long long ytiles = trap_tiles(Ym, Yn);
         for (int jjb = 0; jjb < (int)jb; ++jjb) {
-            const int b = jjb % nxb;
+            const int b = jjb % pl.nx;
             double *X = dY + (size_t)(ytiles + (long long)b * Ym) * nb2;
-            if (jjb >= nxb) HIPTRY(hipStreamWaitEvent(st.h2d, ev_xfree[b], 0));
+            if (jjb >= pl.nx) HIPTRY(hipStreamWaitEvent(st.h2d, ev_xfree[b], 0));
             // A2X (oocquark.cpp:301-319): rows jb.. of tile column jjb
-            size_t bytes = (size_t)Ym * nb2 * sizeof(double);
-            if (direct) {
-                HIPTRY(t_h2d.copy(X, A + (size_t)tile_index((int)jb, jjb, bb, 1) * nb2, bytes,
-                                  hipMemcpyHostToDevice, st.h2d));
-            } else {
-                if ((rc = take_slot(hslot))) return rc;
-                if (tiles_host)
-                    par_memcpy(slot[hslot], A + (size_t)tile_index((int)jb, jjb, bb, 1) * nb2,
-                               bytes / sizeof(double));
-                else
-                    pack_column(A, N, NB, LDA, jjb, (int)jb, slot[hslot]);
-                HIPTRY(t_h2d.copy(X, slot[hslot], bytes, hipMemcpyHostToDevice, st.h2d));
-                HIPTRY(hipEventRecord(ev_slot[hslot], st.h2d));
-                hslot ^= 1;
-            }
+            if ((rc = h2d_column(X, jjb, (int)jb, nth, t_h2d, no_wait))) return rc;
             HIPTRY(hipEventRecord(ev_xready[b], st.h2d));
-            h2d_bytes += bytes;
             HIPTRY(hipStreamWaitEvent(st.comp, ev_xready[b], 0));
             // ooc_syrk(X, Y, YM, YN, NB) (oocquark.cpp:321-356) as one launch
             Gemm g;
@@ -506,62 +664,97 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             HIPTRY(launch_update(g, st.comp));
             HIPTRY(hipEventRecord(ev_xfree[b], st.comp));
         }
+        HIPTRY(hipEventRecord(ev_xdone, st.comp));
+
+        // per tile column of this panel: copied out (evd), no longer read on
+        // s0 / s1 (ev0 / ev1)
+        std::vector<hipEvent_t> evc(Yn), evd(Yn), ev0(Yn), ev1(Yn);
+        for (int jj = 0; jj < Yn; ++jj) {
+            HIPTRY(new_event(&evc[jj]));
+            HIPTRY(new_event(&evd[jj]));
+            HIPTRY(new_event(&ev0[jj]));
+            HIPTRY(new_event(&ev1[jj]));
+        }
+        freed.set(-1);
+
+        // --- the next panel's tile columns, queued to the loader thread
+        if (p + 1 < pl.width.size()) {
+            const int jb2 = (int)jb + Yn, Ym2 = bb - jb2, Yn2 = pl.width[p + 1];
+            for (int c = 0; c < Yn2; ++c) {
+                const int need = last_overlapping_column(Ym, Yn, Ym2, c);
+                double *dst = dY + (size_t)tile_index(c, c, Ym2, 1) * nb2;
+                const hipEvent_t w0 = need >= 0 ? ev0[need] : nullptr;
+                const hipEvent_t w1 = need >= 0 ? ev1[need] : nullptr;
+                const hipEvent_t wd = need >= 0 ? evd[need] : nullptr;
+                loader.post([&, dst, c, need, jb2, w0, w1, wd]() -> int {
+                    return h2d_column(dst, jb2 + c, jb2 + c, nth_half, t_h2d_pre, [&]() -> int {
+                        if (need >= 0) {
+                            if (!freed.wait(need)) {
+                                set_err("out-of-core factorization aborted");
+                                return OOCDAG_EHIP;
+                            }
+                            HIPTRY(hipStreamWaitEvent(st.h2d, w0, 0));
+                            HIPTRY(hipStreamWaitEvent(st.h2d, w1, 0));
+                            HIPTRY(hipStreamWaitEvent(st.h2d, wd, 0));
+                        }
+                        HIPTRY(hipStreamWaitEvent(st.h2d, ev_xdone, 0));
+                        return OOCDAG_OK;
+                    });
+                });
+            }
+        }
 
         // --- in-core factorization of the panel, each finished tile column
         // copied back while the next ones are computed (oocquark.cpp:389-419)
-        std::vector<int> pending;
-        auto drain_one = [&]() -> int {
-            // oldest pending column: wait for its copy and unpack it into A
-            int jj = pending.front();
-            pending.erase(pending.begin());
-            int s = 2 + (jj & 1);
-            HIPTRY(hipEventSynchronize(ev_slot[s]));
-            if (tiles_host)
-                par_memcpy(A + (size_t)tile_index((int)jb + jj, (int)jb + jj, bb, 1) * nb2, slot[s],
-                           (size_t)(Ym - jj) * nb2);
-            else
-                unpack_column(A, N, NB, LDA, (int)jb + jj, (int)jb + jj, slot[s]);
-            slot_used[s] = false;
+        const std::function<int(int)> on_column = [&](int jj) -> int {
+            const int tj = (int)jb + jj;
+            const size_t bytes = (size_t)(Ym - jj) * nb2 * sizeof(double);
+            const double *src = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
+            const int s = jj % ND;
+            double *dst;
+            if (direct) {
+                dst = A + (size_t)tile_index(tj, tj, bb, 1) * nb2;
+            } else {
+                if (dslot_task[s]) unpacker.wait_done(dslot_task[s]);
+                dst = dslot_buf[s];
+            }
+            HIPTRY(hipEventRecord(evc[jj], st.comp));
+            HIPTRY(hipStreamWaitEvent(st.d2h, evc[jj], 0));
+            HIPTRY(t_d2h.copy(dst, src, bytes, hipMemcpyDeviceToHost, st.d2h));
+            HIPTRY(hipEventRecord(evd[jj], st.d2h));
+            d2h_bytes += bytes;
+            if (!direct) {
+                const hipEvent_t done = evd[jj];
+                const double *from = dslot_buf[s];
+                const size_t elems = (size_t)(Ym - jj) * nb2;
+                dslot_task[s] = unpacker.post([&, done, from, tj, elems]() -> int {
+                    HIPTRY(hipEventSynchronize(done));
+                    if (tiles_host)
+                        par_memcpy(A + (size_t)tile_index(tj, tj, bb, 1) * nb2, from, elems,
+                                   nth_half);
+                    else
+                        unpack_column(A, N, NB, LDA, tj, tj, from, nth_half);
+                    return OOCDAG_OK;
+                });
+            }
+            return OOCDAG_OK;
+        };
+        const std::function<int(int)> after_updates = [&](int jj) -> int {
+            HIPTRY(hipEventRecord(ev0[jj], st.comp));
+            HIPTRY(hipEventRecord(ev1[jj], st.bulk));
+            freed.set(jj);
             return OOCDAG_OK;
         };
         rc = factor_panel(dY, NB, Ym, Yn, (int)jb * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
-                          [&](int jj) -> int {
-                              if (direct) {
-                                  // finished column straight back into the host tiles
-                                  HIPTRY(hipEventRecord(ev_col, st.comp));
-                                  HIPTRY(hipStreamWaitEvent(st.d2h, ev_col, 0));
-                                  const int tj = (int)jb + jj;
-                                  size_t bytes = (size_t)(Ym - jj) * nb2 * sizeof(double);
-                                  const double *src = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
-                                  HIPTRY(t_d2h.copy(A + (size_t)tile_index(tj, tj, bb, 1) * nb2, src,
-                                                    bytes, hipMemcpyDeviceToHost, st.d2h));
-                                  d2h_bytes += bytes;
-                                  return OOCDAG_OK;
-                              }
-                              int s = 2 + (jj & 1);
-                              if (slot_used[s]) {
-                                  int r = drain_one();
-                                  if (r) return r;
-                              }
-                              HIPTRY(hipEventRecord(ev_col, st.comp));
-                              HIPTRY(hipStreamWaitEvent(st.d2h, ev_col, 0));
-                              size_t bytes = (size_t)(Ym - jj) * nb2 * sizeof(double);
-                              const double *src = dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2;
-                              HIPTRY(t_d2h.copy(slot[s], src, bytes, hipMemcpyDeviceToHost,
-                                                st.d2h));
-                              HIPTRY(hipEventRecord(ev_slot[s], st.d2h));
-                              slot_used[s] = true;
-                              pending.push_back(jj);
-                              d2h_bytes += bytes;
-                              return OOCDAG_OK;
-                          });
+                          on_column, &after_updates);
         if (rc) return rc;
-        while (!pending.empty())
-            if ((rc = drain_one())) return rc;
-        // the next panel overwrites Y: everything on this panel must be done
+        // the next panel's X buffers may overwrite this panel's tiles and its
+        // X columns are read from A: this panel must be finished and unpacked
+        if ((rc = unpacker.drain())) return rc;
         HIPTRY(hipStreamSynchronize(st.comp));
         HIPTRY(hipStreamSynchronize(st.d2h));
     }
+    if ((rc = loader.drain())) return rc;
     HIPTRY(hipStreamSynchronize(st.h2d));
     rc = read_info(sc.dinfo, st.comp, N, info);
     *secs = now() - t0;
@@ -569,15 +762,8 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     g_stat[0] = *secs;
     g_stat[1] = h2d_bytes;
     g_stat[2] = d2h_bytes;
-    g_stat[3] = t_h2d.seconds();
+    g_stat[3] = t_h2d.seconds() + t_h2d_pre.seconds();
     g_stat[4] = t_d2h.seconds();
-    for (auto &e : ev_slot) (void)hipEventDestroy(e);
-    for (auto &e : ev_xready) (void)hipEventDestroy(e);
-    for (auto &e : ev_xfree) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(ev_y);
-    (void)hipEventDestroy(ev_col);
-    (void)hipEventDestroy(ev_t0);
-    (void)hipEventDestroy(ev_t1);
     return rc;
 }
 
