@@ -321,6 +321,7 @@ void oocdag_server::run_small(std::vector<Job> &jobs, hipStream_t s, double t_st
             rc = OOCDAG_EHIP;
     }
     if (rc == OOCDAG_OK && hipStreamSynchronize(s) != hipSuccess) rc = OOCDAG_EHIP;
+    const double t_input = now() - t_start;
     // small systems: FP64 factor (FP32 + refinement does not pay at these
     // orders); the batched kernel still refines when the error is above tol
     int prec = 0;
@@ -376,6 +377,7 @@ void oocdag_server::run_small(std::vector<Job> &jobs, hipStream_t s, double t_st
             }
         d.r.x_checksum = cs;
         fill_common(d.r, jobs[i], t_start, t_end);
+        d.r.input_seconds = t_input;
         finish(jobs[i].ticket, std::move(d));
     }
 }
@@ -395,12 +397,15 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
         rc = OOCDAG_ENOMEM;
     if (rc == OOCDAG_OK && normal_block(B, n, q.nrhs, ldb, q.rhs_seed, 2, s) != hipSuccess) rc = OOCDAG_EHIP;
     oocdag_mpir_stats st{};
+    double t_input = 0;
     if (rc == OOCDAG_OK && !hit) {
         // materialize A, factor (mixed precision, FP64 fallback), refine, keep the factor
         if (hipMallocAsync((void **)&ce->A, tiles_bytes(n, nb), s) != hipSuccess)
             rc = OOCDAG_ENOMEM;
-        else if (gram_tiles(ce->A, n, nb, q.mu, q.seed, s) != hipSuccess)
+        else if (gram_tiles(ce->A, n, nb, q.mu, q.seed, s) != hipSuccess ||
+                 hipStreamSynchronize(s) != hipSuccess)
             rc = OOCDAG_EHIP;
+        t_input = now() - t_start;
         if (rc == OOCDAG_OK) {
             const int prec = n >= MP_MIN_N ? q.precision : 0;
             rc = mp_factor_solve(ce->A, n, nb, B, q.nrhs, ldb, X, ldb, prec, q.tol, 30, s, s1, &st,
@@ -453,6 +458,7 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
     for (double v : d.x) cs += v;
     d.r.x_checksum = cs;
     fill_common(d.r, j, t_start, now());
+    d.r.input_seconds = t_input;
     finish(j.ticket, std::move(d));
 }
 
@@ -510,6 +516,7 @@ void oocdag_server::run_ooc(Job &j, hipStream_t s, double t_start) {
             for (int r = c + 1; r < n; ++r) Ah[(size_t)r * n + c] = Ah[(size_t)c * n + r];
         }
     }
+    const double t_input = now() - t_start;
     // 2. FP64 out-of-core factorization under this server's budget
     if (rc == OOCDAG_OK) {
         int info = 0;
