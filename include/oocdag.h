@@ -44,7 +44,10 @@ extern "C" {
  * negative value on error.  *info: 0 success; k > 0 the leading minor of
  * order k is not positive definite (LAPACK dpotrf convention; the reference
  * only printed it, oocquark.cpp:95); < 0 an OOCDAG_E* code.  NB must be a
- * multiple of 64.
+ * multiple of 64.  The out-of-core path drives four HIP streams; run the
+ * process with GPU_MAX_HW_QUEUES >= 5 (HIP's default is 4) so that none of
+ * them shares a hardware queue -- shared, a finished column's copy to the
+ * host holds back the next update (about 4 % at n = 196608).
  * ------------------------------------------------------------------------- */
 double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *info);
 
@@ -221,11 +224,14 @@ int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long lon
  * tiles, and the left-looking MFMA kernel (T = 64, RB = 0). */
 int oocdag_batch_set_variant(int precision, int variant);
 
-/* Chooses the kernel of the FP32 trailing update in the mixed-precision
- * factorization (oocdag_mpir_solve_tiles and the server): 0 = operands
- * loaded straight into MFMA registers (default), 1 = LDS-staged.  A test and
- * tuning hook with no reference counterpart; returns the number of variants
- * (2) or OOCDAG_EARG. */
+/* Chooses how the mixed-precision factorization (oocdag_mpir_solve_tiles and
+ * the server) runs its low-precision trailing updates, as bits: 1 = the
+ * LDS-staged FP32 update kernel instead of the one that loads operands
+ * straight into MFMA registers; 2 = each super-column's bulk update runs
+ * after the next super-column's FP64 factorization instead of beside it on a
+ * second stream.  Every variant gives bit-identical results.  A test and
+ * tuning hook with no reference counterpart; 0 restores the default; returns
+ * the number of variants (4) or OOCDAG_EARG. */
 int oocdag_mp_set_update_variant(int variant);
 
 /* ---------------------------------------------------------------------------

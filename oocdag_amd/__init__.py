@@ -20,6 +20,15 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboocdag.so")
 
+# The out-of-core factorization drives four HIP streams (host->device,
+# critical path, bulk update, device->host) beside HIP's null stream.  HIP
+# maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default); with
+# fewer queues than busy streams two of them share one and the device->host
+# copy of a finished column then holds back the next bulk update.  8 queues
+# keep every stream on its own.  Read once, when HIP initializes: a value the
+# caller set, or one a process that already initialized HIP is using, stays.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 OK, EARG, EBUDGET, ENOMEM, EHIP, ENODEV = 0, -1, -2, -3, -4, -5
 
 _dp = C.POINTER(C.c_double)
@@ -407,8 +416,10 @@ def mpir_solve_tiles(A: DeviceBuffer, n: int, nb: int, B: DeviceBuffer, nrhs: in
 
 
 def mp_set_update_variant(variant: int) -> int:
-    """Choose the FP32 trailing-update kernel of the mixed-precision factor
-    (0: operands straight into MFMA registers, the default; 1: LDS-staged)."""
+    """Choose how the mixed-precision factor runs its low-precision updates,
+    as bits: 1 = LDS-staged FP32 kernel (else operands straight into MFMA
+    registers), 2 = bulk update serialized after the next super-column (else
+    overlapped on a second stream); 0 is the default."""
     rc = lib().oocdag_mp_set_update_variant(variant)
     if rc < 0:
         raise OocdagError(f"mp_set_update_variant({variant}) failed")

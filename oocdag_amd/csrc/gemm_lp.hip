@@ -493,8 +493,10 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
 
 int g_rowmap[2] = {-1, -1};
 std::mutex g_probe_mu;
-// FP32 update kernel: 0 operands straight to MFMA registers, 1 LDS-staged
-std::atomic<int> g_f32_variant{0};
+// Mixed-precision variant bits: 1 = LDS-staged FP32 update kernel (else
+// operands straight to MFMA registers); 2 = bulk update on the critical-path
+// stream (else overlapped on the second stream).
+std::atomic<int> g_mp_variant{0};
 
 }  // namespace
 
@@ -571,7 +573,7 @@ hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const
     for (int c = 0; c < a.nbk; ++c) n += a.mb - c;
     a.nblk = n;
     a.rowmap = rowmap;
-    const bool direct = g_f32_variant.load() == 0 && K % 32 == 0 && K >= 28 &&
+    const bool direct = (g_mp_variant.load() & 1) == 0 && K % 32 == 0 && K >= 28 &&
                         ((uintptr_t)X & 15) == 0;
     if (mode == 0 && direct && rowmap == 0)
         hipLaunchKernelGGL((k_update_f32_dtv<8, 6, 0>), dim3((unsigned)n), dim3(NT), 0, s, a);
@@ -586,10 +588,14 @@ hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const
 
 }  // namespace ooc
 
-// Chooses the FP32 low-precision update kernel (test and tuning hook):
-// 0 = operands straight into MFMA registers (default), 1 = LDS-staged.
+namespace ooc {
+int mp_variant() { return g_mp_variant.load(); }
+}  // namespace ooc
+
+// Mixed-precision factorization variant (test and tuning hook), bits:
+// 1 = LDS-staged FP32 update kernel, 2 = no overlap of the bulk update.
 extern "C" int oocdag_mp_set_update_variant(int variant) {
-    if (variant != 0 && variant != 1) return OOCDAG_EARG;
-    ooc::g_f32_variant.store(variant);
-    return 2;
+    if (variant < 0 || variant > 3) return OOCDAG_EARG;
+    ooc::g_mp_variant.store(variant);
+    return 4;
 }

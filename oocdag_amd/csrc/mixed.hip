@@ -326,6 +326,9 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
         return rc ? -2 : 0;
     }
     const int mode = precision - 1;
+    // overlap: the bulk update of super-column J runs on s1 beside the FP64
+    // factorization of super-column J+1 (else after it on s)
+    const bool overlap = (mp_variant() & 2) == 0;
     const int rowmap = mfma_rowmap(mode, s);
     if (rowmap < 0) return -1;
     const int sw = mp_super_width(NB, bb);
@@ -383,12 +386,15 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
             const void *Xb = mode == 0 ? (const void *)((const float *)lp[J & 1] + roff)
                                        : (const void *)((const unsigned short *)lp[J & 1] +
                                                         roff * ldx);
-            MP_TRY(hipEventRecord(ev_fact, s));
-            MP_TRY(hipStreamWaitEvent(s1, ev_fact, 0));
+            if (overlap) {
+                MP_TRY(hipEventRecord(ev_fact, s));
+                MP_TRY(hipStreamWaitEvent(s1, ev_fact, 0));
+            }
+            hipStream_t sb = overlap ? s1 : s;
             MP_TRY(launch_update_lp(mode, W, bb, NB, k1 * NB, Xb, ldx, K, sc[J & 1] + (k1 - j1),
-                                    (bb - k1) * NB, (bb - k1) * NB, rowmap, s1));
-            MP_TRY(hipEventRecord(ev_bulk[J & 1], s1));
-            any_bulk = true;
+                                    (bb - k1) * NB, (bb - k1) * NB, rowmap, sb));
+            MP_TRY(hipEventRecord(ev_bulk[J & 1], sb));
+            any_bulk = overlap;
         }
     }
     if (any_bulk) {

@@ -83,6 +83,8 @@ hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s);
 // 16x16x32), read once from a probe: 0 row = 4*(lane/16) + r, 1 row =
 // lane/16 + 4*r; -1 if the layouts differ from both (gemm_lp.hip).
 int mfma_rowmap(int kind, hipStream_t s);
+// Mixed-precision variant bits (oocdag_mp_set_update_variant).
+int mp_variant();
 
 // Largest block edge of the GEMM kernels: operands must provide this many
 // addressable rows past any block start they cover.
@@ -120,10 +122,14 @@ struct PanelUpdater {
 // receives column j's 64-block inverses (stride 0: reuse one buffer).
 // on_column(j) runs right after column j's kernels were queued on s0 (the
 // column is final once s0 reaches that point); a nonzero return aborts.
-// Returns an OOCDAG-style code (0 ok, -4 HIP error, other: on_column's).
+// after_updates(j), when given, runs once every kernel that reads tile column
+// j has been queued (the update of column j+1 on s0 and the bulk update from
+// column j on s1): work queued on s0 and s1 after that point never reads it.
+// Returns an OOCDAG-style code (0 ok, -4 HIP error, other: a callback's).
 int factor_panel_la(const Opnd &P, int NB, int Ym, int Yn, int gcol0, double *dinv,
                     size_t dinv_stride, int *dinfo, hipStream_t s0, hipStream_t s1,
-                    PanelUpdater &up, const std::function<int(int)> &on_column);
+                    PanelUpdater &up, const std::function<int(int)> &on_column,
+                    const std::function<int(int)> *after_updates = nullptr);
 
 // Make a staged tile whose valid part is vr x vc behave like an NB x NB
 // tile of the zero-padded problem: entries outside the valid part become 0,

@@ -36,6 +36,9 @@ static void matprint(const double *A, int N, int LDA, char AL) {
 }
 
 int main(int argc, char **argv) {
+    // four busy HIP streams plus the null stream: give each its own hardware
+    // queue (HIP reads this at its first call; an explicit setting stays)
+    setenv("GPU_MAX_HW_QUEUES", "8", 0);
     if (argc < 4) {
         printf("invalid arguments N NB memsize(M)\n");
         return 1;

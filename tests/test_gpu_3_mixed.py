@@ -78,10 +78,13 @@ def test_mpir_matches_fp64_oracle(gpu, prec, mu, n, nb):
         assert st["iterations"] <= 3
 
 
-def test_mpir_fp32_update_kernels_agree(gpu):
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_mpir_update_variants_agree(gpu, prec):
     """Both FP32 update kernels accumulate every element over the same
-    4-wide K slabs in the same order, so the factorizations, the refinement
-    and the solution are bit-identical."""
+    4-wide K slabs in the same order, and the bulk update touches other
+    tiles than the next super-column's factorization whether or not the two
+    overlap, so every variant gives a bit-identical factorization,
+    refinement and solution."""
     ooc = gpu
     n, nb, nrhs = 6144, 128, 4
     buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
@@ -89,17 +92,19 @@ def test_mpir_fp32_update_kernels_agree(gpu):
     ooc.normal_fill(dB, n, nrhs, n, seed=5, stream_id=2)
     out = []
     try:
-        for variant in (1, 0):
-            ooc.mp_set_update_variant(variant)
+        for variant in (3, 2, 1, 0):
+            assert ooc.mp_set_update_variant(variant) == 4
             ooc.gram_tiles(buf, n, nb, 1e-2, seed=5)
             dX = ooc.DeviceBuffer(n * nrhs * 8)
-            st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, "fp32", tol=8 * EPS, max_iter=30)
-            assert st["fell_back"] == 0 and st["iterations"] <= 3, st
+            st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, prec, tol=8 * EPS, max_iter=30)
+            if prec == "fp32":
+                assert st["fell_back"] == 0 and st["iterations"] <= 3, st
             out.append((st["iterations"], dX.to_array((n, nrhs))))
     finally:
         ooc.mp_set_update_variant(0)
-    assert out[0][0] == out[1][0]
-    assert np.array_equal(out[0][1], out[1][1])
+    for it, x in out[1:]:
+        assert it == out[0][0]
+        assert np.array_equal(x, out[0][1])
 
 
 def test_mpir_non_spd_reports_info(gpu):
