@@ -354,6 +354,26 @@ bool host_pinned(const void *p) {
     return e == hipSuccess && at.type == hipMemoryTypeHost;
 }
 
+// A stream on a hardware queue of its own.  HIP shares its pool of
+// GPU_MAX_HW_QUEUES queues among streams, and two busy streams on one queue
+// run in submission order (a device->host copy then holds back the bulk
+// update queued after it).  A stream with a CU mask -- a property of the
+// queue -- gets a queue of its own; the mask enables every CU.  Falls back to
+// a plain non-blocking stream if the runtime refuses the mask.
+hipError_t make_stream(hipStream_t *s) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        ncu > 0) {
+        std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0xffffffffu);
+        if (ncu % 32) mask.back() = (1u << (ncu % 32)) - 1u;
+        if (hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data()) == hipSuccess)
+            return hipSuccess;
+        (void)hipGetLastError();
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
 struct Streams {
     hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr, bulk = nullptr;
     bool own_comp = true;
@@ -497,8 +517,10 @@ int last_overlapping_column(int Ym, int Yn, int Ym2, int c) {
 // for full-speed DMA), so every tile-column copy is one contiguous DMA.
 //
 // Per panel (widths from the planner):
-//   1. its tile columns are in HBM: copied by this thread for the first
-//      panel, prefetched during the previous panel otherwise (step 4);
+//   1. its tile columns are in HBM: prefetched during the previous panel
+//      (step 4), or, for the first panel, copied by this thread -- its first
+//      w0 tile columns are factored while the others arrive, each of which is
+//      updated from those w0 columns on arrival;
 //   2. left-looking update from every prior tile column, each re-read from
 //      the host into one of the X buffers (A2X + ooc_syrk, oocquark.cpp:
 //      301-356), the copy of the next X column overlapping the current update;
@@ -523,10 +545,10 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     constexpr int ND = 3;   // device->host slots
 
     Streams st;
-    HIPTRY(hipStreamCreateWithFlags(&st.h2d, hipStreamNonBlocking));
-    HIPTRY(hipStreamCreateWithFlags(&st.comp, hipStreamNonBlocking));
-    HIPTRY(hipStreamCreateWithFlags(&st.d2h, hipStreamNonBlocking));
-    HIPTRY(hipStreamCreateWithFlags(&st.bulk, hipStreamNonBlocking));
+    HIPTRY(make_stream(&st.h2d));
+    HIPTRY(make_stream(&st.comp));
+    HIPTRY(make_stream(&st.d2h));
+    HIPTRY(make_stream(&st.bulk));
     // host tiles in page-locked memory are copied directly; otherwise (and
     // for the column-major array) through pinned slots
     const bool direct = tiles_host && host_pinned(A);
@@ -628,9 +650,13 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     for (size_t p = 0, jb = 0; p < pl.width.size(); jb += pl.width[p], ++p) {
         const int Yn = pl.width[p];
         const int Ym = bb - (int)jb;
+        // the first panel is split: its first w0 tile columns are factored
+        // while the others are still being copied in (nothing to overlap the
+        // copy of a whole first panel with otherwise)
+        const int w0 = (p == 0 && Yn >= 4) ? std::min(8, Yn / 4) : 0;
         // --- A2Y (oocquark.cpp:253-275): the panel's tile columns -> Y
         if (p == 0) {
-            for (int jj = 0; jj < Yn; ++jj)
+            for (int jj = 0; jj < (w0 > 0 ? w0 : Yn); ++jj)
                 if ((rc = h2d_column(dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2, (int)jb + jj,
                                      (int)jb + jj, nth, t_h2d, no_wait)))
                     return rc;
@@ -678,31 +704,34 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
         freed.set(-1);
 
         // --- the next panel's tile columns, queued to the loader thread
-        if (p + 1 < pl.width.size()) {
+        // (which shares the host->device slots: only while this thread queues
+        // no copies of its own)
+        auto post_prefetch = [&]() {
+            if (p + 1 >= pl.width.size()) return;
             const int jb2 = (int)jb + Yn, Ym2 = bb - jb2, Yn2 = pl.width[p + 1];
             for (int c = 0; c < Yn2; ++c) {
                 const int need = last_overlapping_column(Ym, Yn, Ym2, c);
                 double *dst = dY + (size_t)tile_index(c, c, Ym2, 1) * nb2;
-                const hipEvent_t w0 = need >= 0 ? ev0[need] : nullptr;
-                const hipEvent_t w1 = need >= 0 ? ev1[need] : nullptr;
-                const hipEvent_t wd = need >= 0 ? evd[need] : nullptr;
-                loader.post([&, dst, c, need, jb2, w0, w1, wd]() -> int {
+                const hipEvent_t e0 = need >= 0 ? ev0[need] : nullptr;
+                const hipEvent_t e1 = need >= 0 ? ev1[need] : nullptr;
+                const hipEvent_t ed = need >= 0 ? evd[need] : nullptr;
+                loader.post([&, dst, c, need, jb2, e0, e1, ed]() -> int {
                     return h2d_column(dst, jb2 + c, jb2 + c, nth_half, t_h2d_pre, [&]() -> int {
                         if (need >= 0) {
                             if (!freed.wait(need)) {
                                 set_err("out-of-core factorization aborted");
                                 return OOCDAG_EHIP;
                             }
-                            HIPTRY(hipStreamWaitEvent(st.h2d, w0, 0));
-                            HIPTRY(hipStreamWaitEvent(st.h2d, w1, 0));
-                            HIPTRY(hipStreamWaitEvent(st.h2d, wd, 0));
+                            HIPTRY(hipStreamWaitEvent(st.h2d, e0, 0));
+                            HIPTRY(hipStreamWaitEvent(st.h2d, e1, 0));
+                            HIPTRY(hipStreamWaitEvent(st.h2d, ed, 0));
                         }
                         HIPTRY(hipStreamWaitEvent(st.h2d, ev_xdone, 0));
                         return OOCDAG_OK;
                     });
                 });
             }
-        }
+        };
 
         // --- in-core factorization of the panel, each finished tile column
         // copied back while the next ones are computed (oocquark.cpp:389-419)
@@ -745,9 +774,54 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             freed.set(jj);
             return OOCDAG_OK;
         };
-        rc = factor_panel(dY, NB, Ym, Yn, (int)jb * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
-                          on_column, &after_updates);
-        if (rc) return rc;
+        if (w0 == 0) {
+            post_prefetch();
+            rc = factor_panel(dY, NB, Ym, Yn, (int)jb * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
+                              on_column, &after_updates);
+            if (rc) return rc;
+        } else {
+            // columns [0, w0), then each later column's update from them
+            // (one K = w0*NB launch) as soon as that column is in HBM, then
+            // columns [w0, Yn): the trailing part of a packed panel is itself
+            // a packed panel of Ym - w0 tile rows at tile_index(w0, w0, Ym)
+            rc = factor_panel(dY, NB, Ym, w0, (int)jb * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
+                              on_column);
+            if (rc) return rc;
+            for (int c = w0; c < Yn; ++c) {
+                if ((rc = h2d_column(dY + (size_t)tile_index(c, c, Ym, 1) * nb2, (int)jb + c,
+                                     (int)jb + c, nth, t_h2d, no_wait)))
+                    return rc;
+                HIPTRY(hipEventRecord(ev_y, st.h2d));
+                HIPTRY(hipStreamWaitEvent(st.comp, ev_y, 0));
+                const Opnd P = panel(dY, NB, Ym, 1);
+                Gemm g;
+                g.A = P; g.A.r0 = c * NB; g.A.c0 = 0;
+                g.B = g.A;
+                g.C = P; g.C.r0 = c * NB; g.C.c0 = c * NB;
+                g.M = (Ym - c) * NB;
+                g.N = NB;
+                g.K = w0 * NB;
+                g.alpha = -1.0;
+                g.beta = 1.0;
+                g.lower = 1;
+                HIPTRY(launch_update(g, st.comp));
+            }
+            // those updates were the last readers of columns [0, w0)
+            for (int jj = 0; jj < w0; ++jj) {
+                HIPTRY(hipEventRecord(ev0[jj], st.comp));
+                HIPTRY(hipEventRecord(ev1[jj], st.bulk));
+            }
+            freed.set(w0 - 1);
+            post_prefetch();
+            const std::function<int(int)> on_column_b = [&](int jj) { return on_column(jj + w0); };
+            const std::function<int(int)> after_updates_b = [&](int jj) {
+                return after_updates(jj + w0);
+            };
+            rc = factor_panel(dY + (size_t)tile_index(w0, w0, Ym, 1) * nb2, NB, Ym - w0, Yn - w0,
+                              ((int)jb + w0) * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
+                              on_column_b, &after_updates_b);
+            if (rc) return rc;
+        }
         // the next panel's X buffers may overwrite this panel's tiles and its
         // X columns are read from A: this panel must be finished and unpacked
         if ((rc = unpacker.drain())) return rc;

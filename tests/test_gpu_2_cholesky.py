@@ -23,6 +23,7 @@ def rel(a, b):
     (1000, 64, 96),      # ragged edge, many panels
     (2048, 256, 60),     # whole lower triangle fits next to the X columns: one panel
     (1536, 128, 40),
+    (3000, 64, 1200),    # panels [40, 7]: the first panel split (8 columns first), ragged
 ])
 def test_cholesky_host_ooc_matches_oracle(gpu, ora, n, nb, budget_tiles):
     ooc = gpu
@@ -62,6 +63,7 @@ def test_host_tiles_generator_matches_device(gpu, n, nb):
     (4096, 256, 56),     # 4 panels
     (1000, 64, 96),      # ragged edge, many panels
     (1536, 128, 40),
+    (3000, 64, 1200),    # first panel split, next panel prefetched into its tiles
 ])
 def test_cholesky_host_tiles_ooc(gpu, ora, n, nb, budget_tiles, pinned):
     """Out-of-core factorization of host packed tiles (page-locked: direct
