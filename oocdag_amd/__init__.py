@@ -22,12 +22,15 @@ LIB_PATH = os.path.join(HERE, "liboocdag.so")
 
 # The out-of-core factorization drives four HIP streams (host->device,
 # critical path, bulk update, device->host) beside HIP's null stream.  HIP
-# maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default); with
-# fewer queues than busy streams two of them share one and the device->host
-# copy of a finished column then holds back the next bulk update.  8 queues
-# keep every stream on its own.  Read once, when HIP initializes: a value the
-# caller set, or one a process that already initialized HIP is using, stays.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# maps streams onto a pool of GPU_MAX_HW_QUEUES hardware queues (4 by
+# default); with fewer queues than busy streams two of them share one and the
+# device->host copy of a finished column then holds back the next bulk
+# update.  Unset or below 5, the pool is raised to 8 here.  HIP reads the
+# variable once, when it initializes: a process that already initialized HIP
+# keeps its pool.
+_q = os.environ.get("GPU_MAX_HW_QUEUES", "")
+if not _q.isdigit() or int(_q) < 5:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 OK, EARG, EBUDGET, ENOMEM, EHIP, ENODEV = 0, -1, -2, -3, -4, -5
 

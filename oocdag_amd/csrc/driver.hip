@@ -354,26 +354,6 @@ bool host_pinned(const void *p) {
     return e == hipSuccess && at.type == hipMemoryTypeHost;
 }
 
-// A stream on a hardware queue of its own.  HIP shares its pool of
-// GPU_MAX_HW_QUEUES queues among streams, and two busy streams on one queue
-// run in submission order (a device->host copy then holds back the bulk
-// update queued after it).  A stream with a CU mask -- a property of the
-// queue -- gets a queue of its own; the mask enables every CU.  Falls back to
-// a plain non-blocking stream if the runtime refuses the mask.
-hipError_t make_stream(hipStream_t *s) {
-    int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        ncu > 0) {
-        std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0xffffffffu);
-        if (ncu % 32) mask.back() = (1u << (ncu % 32)) - 1u;
-        if (hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data()) == hipSuccess)
-            return hipSuccess;
-        (void)hipGetLastError();
-    }
-    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
-}
-
 struct Streams {
     hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr, bulk = nullptr;
     bool own_comp = true;
@@ -545,10 +525,10 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     constexpr int ND = 3;   // device->host slots
 
     Streams st;
-    HIPTRY(make_stream(&st.h2d));
-    HIPTRY(make_stream(&st.comp));
-    HIPTRY(make_stream(&st.d2h));
-    HIPTRY(make_stream(&st.bulk));
+    HIPTRY(hipStreamCreateWithFlags(&st.h2d, hipStreamNonBlocking));
+    HIPTRY(hipStreamCreateWithFlags(&st.comp, hipStreamNonBlocking));
+    HIPTRY(hipStreamCreateWithFlags(&st.d2h, hipStreamNonBlocking));
+    HIPTRY(hipStreamCreateWithFlags(&st.bulk, hipStreamNonBlocking));
     // host tiles in page-locked memory are copied directly; otherwise (and
     // for the column-major array) through pinned slots
     const bool direct = tiles_host && host_pinned(A);

@@ -36,9 +36,10 @@ static void matprint(const double *A, int N, int LDA, char AL) {
 }
 
 int main(int argc, char **argv) {
-    // four busy HIP streams plus the null stream: give each its own hardware
-    // queue (HIP reads this at its first call; an explicit setting stays)
-    setenv("GPU_MAX_HW_QUEUES", "8", 0);
+    // four busy HIP streams plus the null stream: a pool of at least 5
+    // hardware queues keeps them apart (HIP reads this at its first call)
+    const char *q = getenv("GPU_MAX_HW_QUEUES");
+    if (!q || atoi(q) < 5) setenv("GPU_MAX_HW_QUEUES", "8", 1);
     if (argc < 4) {
         printf("invalid arguments N NB memsize(M)\n");
         return 1;
