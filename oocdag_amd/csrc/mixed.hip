@@ -81,19 +81,45 @@ __global__ void k_finish_diag(double *Y, int N, int nb, int bb, double mu) {
 }
 
 // Row sums of |A| over the full symmetric matrix from its packed lower tiles
-// (diagonal tiles stored full).
-__global__ void k_tile_rowabs(const double *Y, int nb, int bb, double *rowsum) {
-    const int ti = blockIdx.y, tj = blockIdx.z;
-    if (tj > ti) return;
-    const long long base = (ti + (long long)tj * bb - ((long long)tj * (tj + 1)) / 2) * nb * nb;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nb) return;
-    double rs = 0.0, cs = 0.0;
-    for (int c = 0; c < nb; ++c) rs += fabs(Y[base + r + (long long)c * nb]);
-    atomicAdd(&rowsum[ti * nb + r], rs);
-    if (ti != tj) {
-        for (int rr = 0; rr < nb; ++rr) cs += fabs(Y[base + rr + (long long)r * nb]);
-        atomicAdd(&rowsum[tj * nb + r], cs);
+// (diagonal tiles stored full).  One block per 64 rows r0..r0+63 of tile row
+// ti reads exactly those rows of A: the row part from tiles (ti, 0..ti), a
+// wave per tile column with lane = row (512 contiguous bytes per load), and
+// the column part from tiles (ti+1.., ti) -- rows of A above their diagonal
+// are columns of the lower tiles below it -- a wave per column, summed by a
+// wave reduction.  The four waves' partial sums are combined in a fixed
+// order: no atomics, the same result on every run.
+__global__ __launch_bounds__(256) void k_tile_rowabs(const double *Y, int nb, int bb,
+                                                     double *rowsum) {
+    const int ti = blockIdx.y, r0 = blockIdx.x * 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long per = (long long)nb * nb;
+    auto tile = [&](int i, int j) {
+        return Y + (i + (long long)j * bb - ((long long)j * (j + 1)) / 2) * per;
+    };
+    double acc = 0.0;   // this lane's row r0 + lane (row part)
+    for (int tj = 0; tj <= ti; ++tj) {
+        const double *T = tile(ti, tj) + r0 + lane;
+        for (int c = w; c < nb; c += 4) acc += fabs(T[(long long)c * nb]);
+    }
+    __shared__ double part[4][64];
+    __shared__ double colsum[64];
+    if (threadIdx.x < 64) colsum[threadIdx.x] = 0.0;
+    __syncthreads();
+    for (int cc = w; cc < 64; cc += 4) {   // column r0 + cc of the tiles below
+        double cs = 0.0;
+        for (int tk = ti + 1; tk < bb; ++tk) {
+            const double *col = tile(tk, ti) + (long long)(r0 + cc) * nb;
+            for (int r = lane; r < nb; r += 64) cs += fabs(col[r]);
+        }
+        for (int o = 32; o > 0; o >>= 1) cs += __shfl_down(cs, o);
+        if (lane == 0) colsum[cc] = cs;
+    }
+    part[w][lane] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int t = threadIdx.x;
+        rowsum[(long long)ti * nb + r0 + t] =
+            ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t])) + colsum[t];
     }
 }
 
@@ -231,32 +257,31 @@ hipError_t tile_inverses(const double *dY, int N, int NB, const double *dinv_all
     return e;
 }
 
-// X^T <- B^T A^-1 with the diagonal tiles' inverses Z (one GEMM per tile per
-// sweep; T: nrhs x NB scratch, ld ldx).
+// X^T <- B^T A^-1 with the diagonal tiles' inverses Z, one GEMM per tile per
+// sweep.  The forward sweep writes Y^T into Yt (nrhs x N, ld ldx) while it
+// updates the rest of B^T in Xt; the backward sweep writes X^T back into Xt
+// while it updates the rest of Y^T in Yt: each diagonal product lands in the
+// other buffer, so no copy follows it.
 hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double *Xt, int nrhs,
-                       long long ldx, double *T, hipStream_t s) {
+                       long long ldx, double *Yt, hipStream_t s) {
     const int bb = (N + NB - 1) / NB;
     const Opnd Lp = panel((double *)dY, NB, bb, 1);
     const size_t per = (size_t)NB * NB;
     hipError_t e;
-    auto diag = [&](int k, GemmKind kind) -> hipError_t {
+    auto diag = [&](const double *src, double *dst, int k, GemmKind kind) -> hipError_t {
         Gemm g;
-        g.A = plain(Xt, ldx, 0, k * NB);
+        g.A = plain((double *)src, ldx, 0, k * NB);
         g.B = plain((double *)Z + k * per, NB);
-        g.C = plain(T, ldx);
+        g.C = plain(dst, ldx, 0, k * NB);
         g.M = nrhs; g.N = NB; g.K = NB;
         g.alpha = 1.0; g.beta = 0.0; g.lower = 0;
-        hipError_t r = launch_gemm(g, kind, s);
-        if (r != hipSuccess) return r;
-        return hipMemcpy2DAsync(Xt + (size_t)k * NB * ldx, ldx * sizeof(double), T,
-                                ldx * sizeof(double), nrhs * sizeof(double), NB,
-                                hipMemcpyDeviceToDevice, s);
+        return launch_gemm(g, kind, s);
     };
     for (int k = 0; k < bb; ++k) {  // Y^T_k = B^T_k L_kk^-T = B^T_k Z_k
-        if ((e = diag(k, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB)) != hipSuccess) return e;
+        if ((e = diag(Xt, Yt, k, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB)) != hipSuccess) return e;
         if (k + 1 < bb) {
             Gemm g;
-            g.A = plain(Xt, ldx, 0, k * NB);
+            g.A = plain(Yt, ldx, 0, k * NB);
             g.B = Lp; g.B.r0 = (k + 1) * NB; g.B.c0 = k * NB;
             g.C = plain(Xt, ldx, 0, (k + 1) * NB);
             g.M = nrhs; g.N = (bb - k - 1) * NB; g.K = NB;
@@ -265,12 +290,12 @@ hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double 
         }
     }
     for (int k = bb - 1; k >= 0; --k) {  // X^T_k = Y^T_k L_kk^-1 = Y^T_k Z_k^T
-        if ((e = diag(k, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL)) != hipSuccess) return e;
+        if ((e = diag(Yt, Xt, k, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL)) != hipSuccess) return e;
         if (k > 0) {
             Gemm g;
             g.A = plain(Xt, ldx, 0, k * NB);
-            g.B = Lp; g.B.r0 = k * NB; g.B.c0 = 0;
-            g.C = plain(Xt, ldx, 0, 0);
+            g.B = Lp; g.B.r0 = k * NB; g.B.c0 = 0;  // L[k, :k], read transposed
+            g.C = plain(Yt, ldx, 0, 0);
             g.M = nrhs; g.N = k * NB; g.K = NB;
             g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
             if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
@@ -465,9 +490,7 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
     int rc = OOCDAG_OK;
     do {
         // ||A||_inf
-        if (!ok(hipMemsetAsync(anorm, 0, (size_t)Np * 8, s0))) { rc = OOCDAG_EHIP; break; }
-        hipLaunchKernelGGL(k_tile_rowabs, dim3((NB + 255) / 256, bb, bb), dim3(256), 0, s0, dA, NB,
-                           bb, anorm);
+        hipLaunchKernelGGL(k_tile_rowabs, dim3(NB / 64, bb), dim3(256), 0, s0, dA, NB, bb, anorm);
         std::vector<double> hrow(Np);
         int big = INT_MAX, hinfo = INT_MAX;
         if (!ok(hipMemcpyAsync(hrow.data(), anorm, Np * 8, hipMemcpyDeviceToHost, s0)) ||
@@ -505,7 +528,7 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
     auto ok = [](hipError_t e) { return e == hipSuccess; };
     const size_t vb = (size_t)(ldt * Np) * 8;
     if (!ok(salloc(&Bt, vb, s)) || !ok(salloc(&Xt, vb, s)) || !ok(salloc(&Rt, vb, s)) ||
-        !ok(salloc(&T, (size_t)ldt * NB * 8, s)) || !ok(salloc(&nrm, (size_t)2 * ldt * 8, s)))
+        !ok(salloc(&T, vb, s)) || !ok(salloc(&nrm, (size_t)2 * ldt * 8, s)))
         return OOCDAG_ENOMEM;
     int rc = OOCDAG_OK;
     int it = 0;
