@@ -162,6 +162,37 @@ def test_potrf_tiles_repeatable(gpu):
     assert np.array_equal(outs[0], outs[1])
 
 
+def test_large_factor_out_of_core_matches_hbm(gpu):
+    """n = 32768, NB = 1024 (528 tiles, 4.4 GB): the HBM-resident factor has a
+    LIN scaled residual <= 10 (formed on the GPU), and the out-of-core factor
+    of the same matrix from pageable host tiles under a 2 GiB budget (panels
+    [6, 9, 17]: first panel split, later panels prefetched into finished
+    columns' tiles) agrees with it to 1e-12 and has its own residual <= 10."""
+    ooc = gpu
+    n, nb, mem = 32768, 1024, 2 << 30
+    plan = ooc.panel_plan(n, nb, mem)
+    assert len(plan) >= 3 and plan[0] >= 4, plan
+    elems = ooc.tiles_elems(n, nb)
+    buf = ooc.DeviceBuffer(elems * 8)
+    ooc.pdmatgen_tiles(buf, n, nb)
+    assert ooc.potrf_tiles(buf, n, nb) == 0
+    assert ooc.llt_resid_tiles(buf, n, nb) <= 10.0
+    L_hbm = buf.to_array((elems,))
+    t = ooc.HostTiles(n, nb, pinned=False)
+    try:
+        t.generate()
+        secs, info = ooc.cholesky_host_tiles(t, mem)
+        assert info == 0 and secs > 0
+        L_ooc = t.array().copy()
+    finally:
+        t.free()
+    scale = np.max(np.abs(L_hbm))
+    assert np.max(np.abs(L_ooc - L_hbm)) <= 1e-12 * scale
+    ooc.lib().oocdag_memcpy_h2d(buf.ptr, L_ooc.ctypes.data, elems * 8)
+    assert ooc.llt_resid_tiles(buf, n, nb) <= 10.0
+    buf.free()
+
+
 # ---------------------------------------------------------------- solve path
 
 @pytest.mark.parametrize("n,nrhs", [(1000, 5), (4096, 16), (300, 1)])
