@@ -697,7 +697,68 @@ __global__ __launch_bounds__(256) void k_rhs(RArgs a) {
     d4 acc = {0.0, 0.0, 0.0, 0.0};
     const int nk = a.K / 4;
     int t = 0;
-    if (nk >= U) {
+    if (TB == 1 && (a.K & 15) == 0 && a.K >= 128) {
+        // Transposed B: K runs down a stored column, so a lane loads 4
+        // consecutive K values of its column (rows 4kq..4kq+3 of each 16-wide
+        // K group, two 16-byte loads) and MFMA j of the group takes element j:
+        // MFMA j covers K = 16g + 4kq + j over the 4 lane groups kq.  A is
+        // loaded to match (A[m][16g + 4kq + j]).  Two batches of 4 groups in
+        // flight; the loads of a column use 64 contiguous bytes per
+        // instruction instead of 32.
+        const double *qa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + 4 * kl);
+        const double *qb = a.B.p + elem_off(a.B, a.B.r0 + 4 * kl, a.B.c0 + ni);
+        const long long lda = a.A.ld;
+        constexpr int G = 4;   // 16-wide K groups per batch
+        const int ng = a.K / 16;
+        double av[G][4];
+        d2 bv[G][2];
+        auto load = [&](int g0, double (&x)[G][4], d2 (&y)[G][2]) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const long long k = 16LL * (g0 + g);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[g][j] = qa[(k + j) * lda];
+                y[g][0] = *reinterpret_cast<const d2 *>(qb + k);
+                y[g][1] = *reinterpret_cast<const d2 *>(qb + k + 2);
+            }
+        };
+        int g0 = 0;
+        load(0, av, bv);
+        for (; g0 + G <= ng; g0 += G) {
+            double an[G][4];
+            d2 bn[G][2];
+            const bool more = g0 + 2 * G <= ng;
+            if (more) load(g0 + G, an, bn);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][0], bv[g][0].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][1], bv[g][0].y, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][2], bv[g][1].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][3], bv[g][1].y, acc, 0, 0, 0);
+            }
+            if (more) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) av[g][j] = an[g][j];
+                    bv[g][0] = bn[g][0];
+                    bv[g][1] = bn[g][1];
+                }
+            }
+        }
+        // remaining groups (fewer than G), one at a time
+        for (; g0 < ng; ++g0) {
+            const long long k = 16LL * g0;
+            const d2 b0 = *reinterpret_cast<const d2 *>(qb + k);
+            const d2 b1 = *reinterpret_cast<const d2 *>(qb + k + 2);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[k * lda], b0.x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 1) * lda], b0.y, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 2) * lda], b1.x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 3) * lda], b1.y, acc, 0, 0, 0);
+        }
+        t = nk;
+    }
+    if (nk >= U && t == 0) {
         double av[U], bv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
