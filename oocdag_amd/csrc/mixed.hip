@@ -230,31 +230,106 @@ hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt,
     return hipSuccess;
 }
 
-__global__ void k_identity(double *Z, int nb, int count) {
-    const long long total = (long long)nb * nb * count;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        long long w = idx % ((long long)nb * nb);
-        Z[idx] = ((w % nb) == (w / nb)) ? 1.0 : 0.0;
+// Step s of the blocked inverse X = L_kk^-1 of every diagonal tile k at once
+// (64 x 64 blocks, P = NB/64 per tile row): X_jj = D_j, the inverse of the
+// 64 x 64 diagonal block that the factorization left in dinv, and for
+// i = j + s > j
+//     X_ij = -D_i * sum_{m=j}^{i-1} L_im X_mj ,
+// which needs only blocks of earlier steps.  Z_k = X^T is what the solves
+// use, so X_ij is stored as Z block (j, i), X_mj read back from Z block
+// (j, m).  Step 0 writes the diagonal blocks and zeros below them.  Block
+// (x, y) = (tile k, j); 256 threads, thread t owns row t % 64 and columns
+// t / 64 + 4u of the block, operands staged in LDS.
+constexpr int TI = 64, TIS = 65;
+__global__ __launch_bounds__(256) void k_tile_inv_step(const double *W, int NB, int bb,
+                                                       const double *dinv_all,
+                                                       long long inv_per_tile, double *Z,
+                                                       int s) {
+    const int k = blockIdx.x, j = blockIdx.y, i = j + s;
+    const int P = NB / TI;
+    if (i >= P) return;
+    const int t = threadIdx.x, r = t & 63, c0 = t >> 6;
+    const double *Lk = W + (k + (long long)k * bb - ((long long)k * (k + 1)) / 2) *
+                               (long long)NB * NB;
+    const double *D = dinv_all + (long long)k * inv_per_tile;
+    double *Zk = Z + (long long)k * NB * NB;
+    __shared__ double La[TI * TIS], Xb[TI * TIS];
+    if (s == 0) {
+        // Z block (j, j) = D_j^T; Z blocks (a, j), a > j, = 0
+        const double *Dj = D + (long long)j * TI * TI;
+        for (int u = 0; u < 16; ++u) {
+            const int cc = c0 + 4 * u;
+            // Z(j*64 + r, j*64 + cc) = D_j(cc, r)
+            Zk[(long long)(j * TI + r) + (long long)(j * TI + cc) * NB] = Dj[cc + r * TI];
+            for (int a = j + 1; a < P; ++a)
+                Zk[(long long)(a * TI + r) + (long long)(j * TI + cc) * NB] = 0.0;
+        }
+        return;
+    }
+    double acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+    for (int m = j; m < i; ++m) {
+        // La = L_im (rows i*64.., cols m*64.. of the tile), Xb = X_mj
+        for (int u = 0; u < 16; ++u) {
+            const int cc = c0 + 4 * u;
+            La[r + cc * TIS] = Lk[(long long)(i * TI + r) + (long long)(m * TI + cc) * NB];
+            // X_mj(cc, r) = Z(j*64 + r, m*64 + cc): consecutive r, consecutive addresses
+            if (m == j)
+                Xb[r + cc * TIS] = D[(long long)j * TI * TI + r + cc * TI];
+            else
+                Xb[cc + r * TIS] = Zk[(long long)(j * TI + r) + (long long)(m * TI + cc) * NB];
+        }
+        __syncthreads();
+        for (int q = 0; q < TI; ++q) {
+            const double l = La[r + q * TIS];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) acc[u] += l * Xb[q + (c0 + 4 * u) * TIS];
+        }
+        __syncthreads();
+    }
+    // X_ij = -D_i * T
+    const double *Di = D + (long long)i * TI * TI;
+    for (int u = 0; u < 16; ++u) {
+        const int cc = c0 + 4 * u;
+        Xb[r + cc * TIS] = acc[u];
+        La[r + cc * TIS] = Di[r + cc * TI];
+    }
+    __syncthreads();
+    double x[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) x[u] = 0.0;
+    for (int q = 0; q < TI; ++q) {
+        const double d = La[r + q * TIS];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) x[u] -= d * Xb[q + (c0 + 4 * u) * TIS];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) Xb[r + (c0 + 4 * u) * TIS] = x[u];
+    __syncthreads();
+    // Z(j*64 + c, i*64 + rr) = X_ij(rr, c): consecutive threads, consecutive c
+    for (int u = 0; u < 16; ++u) {
+        const int rr = c0 + 4 * u;
+        Zk[(long long)(j * TI + r) + (long long)(i * TI + rr) * NB] = Xb[rr + r * TIS];
     }
 }
 
 // Z_k = L_kk^-T for every diagonal tile (NB x NB each, ld NB), from the
-// 64-block inverses the factorization left in dinv_all.
+// 64-block inverses the factorization left in dinv_all: P = NB/64 launches,
+// each over all tiles.
 hipError_t tile_inverses(const double *dY, int N, int NB, const double *dinv_all, double *Z,
                          hipStream_t s) {
-    const int bb = (N + NB - 1) / NB;
-    const Opnd Lp = panel((double *)dY, NB, bb, 1);
-    const size_t per = (size_t)NB * NB, inv_per_tile = (size_t)(NB / 64) * 4096;
-    hipLaunchKernelGGL(k_identity, dim3(4096), dim3(256), 0, s, Z, NB, bb);
-    hipError_t e = hipGetLastError();
-    for (int k = 0; k < bb && e == hipSuccess; ++k) {
-        Opnd Lk = Lp;
-        Lk.r0 = k * NB;
-        Lk.c0 = k * NB;
-        e = trsm_right_lower_t(plain(Z + k * per, NB), NB, NB, Lk, dinv_all + k * inv_per_tile, s);
+    const int bb = (N + NB - 1) / NB, P = NB / TI;
+    if (NB % TI) return hipErrorInvalidValue;
+    const long long inv_per_tile = (long long)P * TI * TI;
+    for (int st = 0; st < P; ++st) {
+        hipLaunchKernelGGL(k_tile_inv_step, dim3(bb, P - st), dim3(256), 0, s, dY, NB, bb,
+                           dinv_all, inv_per_tile, Z, st);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-    return e;
+    return hipSuccess;
 }
 
 // X^T <- B^T A^-1 with the diagonal tiles' inverses Z, one GEMM per tile per

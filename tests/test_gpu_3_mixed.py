@@ -70,12 +70,36 @@ def test_mpir_matches_fp64_oracle(gpu, prec, mu, n, nb):
     kappa = np.linalg.cond(A, 1)
     ferr = np.abs(X - Xref).max() / np.abs(Xref).max()
     assert ferr <= max(1e2 * kappa * EPS, 1e-13)
+    if prec == "fp64":
+        # the FP64 factor's solve alone meets tol or nearly so; a wrong
+        # diagonal-tile inverse would show as many steps or a fallback
+        assert st["fell_back"] == 0 and st["iterations"] <= 2, st
     if prec == "fp32" and mu == 1e-2:
         # kappa ~ 4e2: an FP32 factor contracts the error by ~kappa * 2^-24
         # per step, so a wrong update (which refinement alone would hide
         # behind many steps or the FP64 fallback) shows here
         assert st["fell_back"] == 0 and st["precision_used"] == 1
         assert st["iterations"] <= 3
+
+
+def test_fp64_solve_tile_inverses_nb512_ragged(gpu):
+    """The solves use Z_k = L_kk^-T of every diagonal tile, built blockwise
+    from the 64 x 64 diagonal inverses (8 steps at NB = 512); a partial last
+    tile (n = 2500) is padded with an identity diagonal."""
+    ooc = gpu
+    n, nb, nrhs = 2500, 512, 3
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, 1e-2, seed=11)
+    A = gram_host(ooc, buf, n, nb)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=11, stream_id=2)
+    B = dB.to_array((n, nrhs))
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, "fp64", tol=8 * EPS, max_iter=30)
+    X = dX.to_array((n, nrhs))
+    assert st["fell_back"] == 0 and st["iterations"] <= 2, st
+    Xref = np.linalg.solve(A, B)
+    assert np.abs(X - Xref).max() / np.abs(Xref).max() <= 1e2 * np.linalg.cond(A, 1) * EPS
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
