@@ -672,13 +672,100 @@ __device__ __forceinline__ long long elem_off(const Opnd &o, int gr, int gc) {
     return tix(ti, tk, o.ym, o.tri) * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
 }
 
-template <int TB>
+// B not transposed, N a multiple of 32: one wave owns 16 rows x 32 columns.
+// Lane (il, kl) loads B rows n0 + 2il and n0 + 2il + 1 of K column kl (16
+// contiguous bytes) and feeds them to two MFMAs, one over the wave's even
+// output columns and one over its odd ones, so one A load serves two MFMAs
+// and a B load instruction covers 4 x 256 contiguous bytes.
+__global__ __launch_bounds__(256) void k_rhs_w2(RArgs a) {
+    constexpr int U = 8;   // K steps per batch; two batches in flight
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n0 = (blockIdx.x * 4 + wave) * 32;
+    const int m0 = blockIdx.y * 16;
+    if (n0 >= a.N) return;   // whole wave
+    const int il = lane & 15, kl = lane >> 4;
+    const int mi = min(m0 + il, a.M - 1);
+    const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kl);
+    const long long astep = 4 * a.A.ld;
+    const double *pb = a.B.p + elem_off(a.B, a.B.r0 + n0 + 2 * il, a.B.c0 + kl);
+    const long long bstep = 4 * a.B.ld;
+    d4 ev = {0.0, 0.0, 0.0, 0.0}, od = {0.0, 0.0, 0.0, 0.0};
+    const int nk = a.K / 4;
+    int t = 0;
+    if (nk >= U) {
+        double av[U];
+        d2 bv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            av[u] = pa[u * astep];
+            bv[u] = *reinterpret_cast<const d2 *>(pb + u * bstep);
+        }
+        for (; t + U <= nk; t += U) {
+            double an[U];
+            d2 bn[U];
+            const bool more = t + 2 * U <= nk;
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    an[u] = pa[(t + U + u) * astep];
+                    bn[u] = *reinterpret_cast<const d2 *>(pb + (t + U + u) * bstep);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ev = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u].x, ev, 0, 0, 0);
+                od = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u].y, od, 0, 0, 0);
+            }
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    av[u] = an[u];
+                    bv[u] = bn[u];
+                }
+            }
+        }
+    }
+    for (; t < nk; ++t) {
+        const double x = pa[t * astep];
+        const d2 y = *reinterpret_cast<const d2 *>(pb + t * bstep);
+        ev = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.x, ev, 0, 0, 0);
+        od = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.y, od, 0, 0, 0);
+    }
+    // element r of a lane: row m0 + kl + 4r, columns n0 + 2il (ev), + 1 (od)
+    double *ce = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 2 * il);
+    double *co = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 2 * il + 1);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (m0 + kl + 4 * r >= a.M) break;
+        double v = a.alpha * ev[r], w = a.alpha * od[r];
+        if (a.beta != 0.0) {
+            v += a.beta * ce[4 * r];
+            w += a.beta * co[4 * r];
+        }
+        ce[4 * r] = v;
+        co[4 * r] = w;
+    }
+}
+
+//
+// KS = 4 (split K): the block's four waves share one 16 x 16 output and take
+// a quarter of K each, reduced through LDS in a fixed order.  Each wave's
+// serial MFMA and load chain is a quarter as long, which is what bounds the
+// small launches (a diagonal-tile product: 8 blocks) and a large launch's
+// tail.
+template <int TB, int KS>
 __global__ __launch_bounds__(256) void k_rhs(RArgs a) {
     constexpr int U = 16;   // K steps per batch; two batches in flight
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n0 = (blockIdx.x * 4 + wave) * 16;
+    const int n0 = (KS == 1 ? blockIdx.x * 4 + wave : blockIdx.x) * 16;
     const int m0 = blockIdx.y * 16;
-    if (n0 >= a.N) return;   // whole wave
+    if (n0 >= a.N) return;   // whole wave (whole block for KS = 4)
+    if (KS > 1) {
+        a.K /= KS;
+        const int kb = wave * a.K;
+        a.A.c0 += kb;
+        if (TB == 0) a.B.c0 += kb; else a.B.r0 += kb;
+    }
     const int il = lane & 15, kl = lane >> 4;
     // lanes past M or N read a valid row/column instead: they only feed
     // output rows/columns that are never stored
@@ -787,6 +874,20 @@ __global__ __launch_bounds__(256) void k_rhs(RArgs a) {
         }
     }
     for (; t < nk; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t * astep], pb[t * bstep], acc, 0, 0, 0);
+    if constexpr (KS > 1) {
+        __shared__ d4 part[KS - 1][64];
+        if (wave > 0) part[wave - 1][lane] = acc;
+        __syncthreads();
+        if (wave > 0) return;
+#pragma unroll
+        for (int w = 0; w < KS - 1; ++w) {
+            const d4 q = part[w][lane];
+            acc[0] += q[0];
+            acc[1] += q[1];
+            acc[2] += q[2];
+            acc[3] += q[3];
+        }
+    }
     // element r of a lane sits at row lane/16 + 4r, column lane%16
     if (n0 + il >= a.N) return;
     double *c = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + il);
@@ -873,11 +974,29 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
     a.K = g.K;
     a.alpha = g.alpha;
     a.beta = g.beta;
-    dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 15) / 16));
-    if (tb)
-        hipLaunchKernelGGL(k_rhs<1>, grid, dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_rhs<0>, grid, dim3(256), 0, s, a);
+    // split K when the unsplit grid would not give every CU a block (a
+    // large grid streams B faster unsplit) and each quarter keeps whole
+    // 16-wide K groups, at least 64
+    const long long blocks1 = (long long)((g.N + 63) / 64) * ((g.M + 15) / 16);
+    if (blocks1 < 256 && g.K % 64 == 0 && g.K >= 256) {
+        dim3 grid((unsigned)((g.N + 15) / 16), (unsigned)((g.M + 15) / 16));
+        if (tb)
+            hipLaunchKernelGGL((k_rhs<1, 4>), grid, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_rhs<0, 4>), grid, dim3(256), 0, s, a);
+    } else if (!tb && g.N % 32 == 0 && (B.ts > 0 ? B.ts % 32 == 0 && B.r0 % 32 == 0
+                                                  : B.ld % 2 == 0 && B.r0 % 2 == 0) &&
+               ((uintptr_t)B.p & 15) == 0) {
+        // 32-column groups inside one tile, 16-byte aligned pairs of B rows
+        dim3 grid((unsigned)((g.N + 127) / 128), (unsigned)((g.M + 15) / 16));
+        hipLaunchKernelGGL(k_rhs_w2, grid, dim3(256), 0, s, a);
+    } else {
+        dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 15) / 16));
+        if (tb)
+            hipLaunchKernelGGL((k_rhs<1, 1>), grid, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_rhs<0, 1>), grid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -102,6 +102,31 @@ def test_fp64_solve_tile_inverses_nb512_ragged(gpu):
     assert np.abs(X - Xref).max() / np.abs(Xref).max() <= 1e2 * np.linalg.cond(A, 1) * EPS
 
 
+def test_fp64_solve_large_grids(gpu):
+    """n = 20480, NB = 512: the early steps of each sweep and of the residual
+    have 16384+ columns, which take the solve kernels' large-grid forms
+    (unsplit K, two B rows per lane); the later steps the split-K form.
+    Backward error of the returned X recomputed on the host."""
+    ooc = gpu
+    n, nb, nrhs = 20480, 512, 2
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, 1e-2, seed=13)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=13, stream_id=2)
+    B = dB.to_array((n, nrhs))
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, "fp64", tol=8 * EPS, max_iter=30)
+    X = dX.to_array((n, nrhs))
+    assert st["fell_back"] == 0 and st["iterations"] <= 2, st
+    A = gram_host(ooc, buf, n, nb)
+    R = B - A @ X
+    anorm = np.abs(A).sum(1).max()
+    del A
+    eta = max(np.abs(R[:, k]).max() / (anorm * np.abs(X[:, k]).max() + np.abs(B[:, k]).max())
+              for k in range(nrhs))
+    assert eta <= 2 * 8 * EPS
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
 def test_mpir_update_variants_agree(gpu, prec):
     """Both FP32 update kernels accumulate every element over the same
