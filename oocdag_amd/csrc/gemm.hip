@@ -672,180 +672,20 @@ __device__ __forceinline__ long long elem_off(const Opnd &o, int gr, int gc) {
     return tix(ti, tk, o.ym, o.tri) * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
 }
 
-// B not transposed, N a multiple of 32: one wave owns 16 rows x 32 columns.
-// Lane (il, kl) loads B rows n0 + 2il and n0 + 2il + 1 of K column kl (16
-// contiguous bytes) and feeds them to two MFMAs, one over the wave's even
-// output columns and one over its odd ones, so one A load serves two MFMAs
-// and a B load instruction covers 4 x 256 contiguous bytes.
-__global__ __launch_bounds__(256) void k_rhs_w2(RArgs a) {
-    constexpr int U = 8;   // K steps per batch; two batches in flight
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n0 = (blockIdx.x * 4 + wave) * 32;
-    const int m0 = blockIdx.y * 16;
-    if (n0 >= a.N) return;   // whole wave
-    const int il = lane & 15, kl = lane >> 4;
-    const int mi = min(m0 + il, a.M - 1);
-    const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kl);
-    const long long astep = 4 * a.A.ld;
-    const double *pb = a.B.p + elem_off(a.B, a.B.r0 + n0 + 2 * il, a.B.c0 + kl);
-    const long long bstep = 4 * a.B.ld;
-    d4 ev = {0.0, 0.0, 0.0, 0.0}, od = {0.0, 0.0, 0.0, 0.0};
-    const int nk = a.K / 4;
-    int t = 0;
-    if (nk >= U) {
-        double av[U];
-        d2 bv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            av[u] = pa[u * astep];
-            bv[u] = *reinterpret_cast<const d2 *>(pb + u * bstep);
-        }
-        for (; t + U <= nk; t += U) {
-            double an[U];
-            d2 bn[U];
-            const bool more = t + 2 * U <= nk;
-            if (more) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    an[u] = pa[(t + U + u) * astep];
-                    bn[u] = *reinterpret_cast<const d2 *>(pb + (t + U + u) * bstep);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                ev = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u].x, ev, 0, 0, 0);
-                od = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u].y, od, 0, 0, 0);
-            }
-            if (more) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    av[u] = an[u];
-                    bv[u] = bn[u];
-                }
-            }
-        }
-    }
-    for (; t < nk; ++t) {
-        const double x = pa[t * astep];
-        const d2 y = *reinterpret_cast<const d2 *>(pb + t * bstep);
-        ev = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.x, ev, 0, 0, 0);
-        od = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.y, od, 0, 0, 0);
-    }
-    // element r of a lane: row m0 + kl + 4r, columns n0 + 2il (ev), + 1 (od)
-    double *ce = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 2 * il);
-    double *co = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 2 * il + 1);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        if (m0 + kl + 4 * r >= a.M) break;
-        double v = a.alpha * ev[r], w = a.alpha * od[r];
-        if (a.beta != 0.0) {
-            v += a.beta * ce[4 * r];
-            w += a.beta * co[4 * r];
-        }
-        ce[4 * r] = v;
-        co[4 * r] = w;
-    }
+// K may span several tiles of a packed B (the solves' bulk updates take 8
+// tile columns or rows at once): the kernels walk K in chunks that end at
+// tile edges, with the operand pointers recomputed per chunk.
+__device__ __forceinline__ int rhs_chunk(const Opnd &B, int gk, int left) {
+    return B.ts > 0 ? min(left, B.ts - gk % B.ts) : left;
 }
 
-//
-// KS = 4 (split K): the block's four waves share one 16 x 16 output and take
-// a quarter of K each, reduced through LDS in a fixed order.  Each wave's
-// serial MFMA and load chain is a quarter as long, which is what bounds the
-// small launches (a diagonal-tile product: 8 blocks) and a large launch's
-// tail.
-template <int TB, int KS>
-__global__ __launch_bounds__(256) void k_rhs(RArgs a) {
-    constexpr int U = 16;   // K steps per batch; two batches in flight
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n0 = (KS == 1 ? blockIdx.x * 4 + wave : blockIdx.x) * 16;
-    const int m0 = blockIdx.y * 16;
-    if (n0 >= a.N) return;   // whole wave (whole block for KS = 4)
-    if (KS > 1) {
-        a.K /= KS;
-        const int kb = wave * a.K;
-        a.A.c0 += kb;
-        if (TB == 0) a.B.c0 += kb; else a.B.r0 += kb;
-    }
-    const int il = lane & 15, kl = lane >> 4;
-    // lanes past M or N read a valid row/column instead: they only feed
-    // output rows/columns that are never stored
-    const int mi = min(m0 + il, a.M - 1), ni = min(n0 + il, a.N - 1);
-    const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kl);
-    const long long astep = 4 * a.A.ld;
-    const double *pb;
-    long long bstep;
-    if (TB == 0) {
-        pb = a.B.p + elem_off(a.B, a.B.r0 + ni, a.B.c0 + kl);
-        bstep = 4 * a.B.ld;
-    } else {
-        pb = a.B.p + elem_off(a.B, a.B.r0 + kl, a.B.c0 + ni);
-        bstep = 4;
-    }
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    const int nk = a.K / 4;
+// acc += A(m, 4t + kl) * B(n, 4t + kl) over t < nk: lane (il, kl) of A at
+// pa + t*astep and of B at pb + t*bstep; two batches of U loads in flight.
+template <int U>
+__device__ __forceinline__ void rhs_run(const double *pa, long long astep, const double *pb,
+                                        long long bstep, int nk, d4 &acc) {
     int t = 0;
-    if (TB == 1 && (a.K & 15) == 0 && a.K >= 128) {
-        // Transposed B: K runs down a stored column, so a lane loads 4
-        // consecutive K values of its column (rows 4kq..4kq+3 of each 16-wide
-        // K group, two 16-byte loads) and MFMA j of the group takes element j:
-        // MFMA j covers K = 16g + 4kq + j over the 4 lane groups kq.  A is
-        // loaded to match (A[m][16g + 4kq + j]).  Two batches of 4 groups in
-        // flight; the loads of a column use 64 contiguous bytes per
-        // instruction instead of 32.
-        const double *qa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + 4 * kl);
-        const double *qb = a.B.p + elem_off(a.B, a.B.r0 + 4 * kl, a.B.c0 + ni);
-        const long long lda = a.A.ld;
-        constexpr int G = 4;   // 16-wide K groups per batch
-        const int ng = a.K / 16;
-        double av[G][4];
-        d2 bv[G][2];
-        auto load = [&](int g0, double (&x)[G][4], d2 (&y)[G][2]) {
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const long long k = 16LL * (g0 + g);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[g][j] = qa[(k + j) * lda];
-                y[g][0] = *reinterpret_cast<const d2 *>(qb + k);
-                y[g][1] = *reinterpret_cast<const d2 *>(qb + k + 2);
-            }
-        };
-        int g0 = 0;
-        load(0, av, bv);
-        for (; g0 + G <= ng; g0 += G) {
-            double an[G][4];
-            d2 bn[G][2];
-            const bool more = g0 + 2 * G <= ng;
-            if (more) load(g0 + G, an, bn);
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][0], bv[g][0].x, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][1], bv[g][0].y, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][2], bv[g][1].x, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][3], bv[g][1].y, acc, 0, 0, 0);
-            }
-            if (more) {
-#pragma unroll
-                for (int g = 0; g < G; ++g) {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) av[g][j] = an[g][j];
-                    bv[g][0] = bn[g][0];
-                    bv[g][1] = bn[g][1];
-                }
-            }
-        }
-        // remaining groups (fewer than G), one at a time
-        for (; g0 < ng; ++g0) {
-            const long long k = 16LL * g0;
-            const d2 b0 = *reinterpret_cast<const d2 *>(qb + k);
-            const d2 b1 = *reinterpret_cast<const d2 *>(qb + k + 2);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[k * lda], b0.x, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 1) * lda], b0.y, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 2) * lda], b1.x, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 3) * lda], b1.y, acc, 0, 0, 0);
-        }
-        t = nk;
-    }
-    if (nk >= U && t == 0) {
+    if (nk >= U) {
         double av[U], bv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -874,6 +714,288 @@ __global__ __launch_bounds__(256) void k_rhs(RArgs a) {
         }
     }
     for (; t < nk; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t * astep], pb[t * bstep], acc, 0, 0, 0);
+}
+
+// Transposed B with K a multiple of 16: K runs down a stored column, so a
+// lane loads 4 consecutive K values of its column (rows 4kq..4kq+3 of each
+// 16-wide K group, two 16-byte loads) and MFMA j of the group takes element
+// j: MFMA j covers K = 16g + 4kq + j over the 4 lane groups kq.  A is loaded
+// to match (qa = A[m][4kq], element j of group g at qa + (16g + j)*lda).
+// Two batches of 4 groups in flight.
+__device__ __forceinline__ void rhs_run_tb16(const double *qa, long long lda, const double *qb,
+                                             int ng, d4 &acc) {
+    constexpr int G = 4;
+    double av[G][4];
+    d2 bv[G][2];
+    auto load = [&](int g0, double (&x)[G][4], d2 (&y)[G][2]) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const long long k = 16LL * (g0 + g);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[g][j] = qa[(k + j) * lda];
+            y[g][0] = *reinterpret_cast<const d2 *>(qb + k);
+            y[g][1] = *reinterpret_cast<const d2 *>(qb + k + 2);
+        }
+    };
+    int g0 = 0;
+    if (ng >= G) {
+        load(0, av, bv);
+        for (; g0 + G <= ng; g0 += G) {
+            double an[G][4];
+            d2 bn[G][2];
+            const bool more = g0 + 2 * G <= ng;
+            if (more) load(g0 + G, an, bn);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][0], bv[g][0].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][1], bv[g][0].y, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][2], bv[g][1].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][3], bv[g][1].y, acc, 0, 0, 0);
+            }
+            if (more) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) av[g][j] = an[g][j];
+                    bv[g][0] = bn[g][0];
+                    bv[g][1] = bn[g][1];
+                }
+            }
+        }
+    }
+    // remaining groups (fewer than G), one at a time
+    for (; g0 < ng; ++g0) {
+        const long long k = 16LL * g0;
+        const d2 b0 = *reinterpret_cast<const d2 *>(qb + k);
+        const d2 b1 = *reinterpret_cast<const d2 *>(qb + k + 2);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[k * lda], b0.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 1) * lda], b0.y, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 2) * lda], b1.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 3) * lda], b1.y, acc, 0, 0, 0);
+    }
+}
+
+// B not transposed, N a multiple of 32: one wave owns 16 rows x 32 columns.
+// Lane (il, kl) loads B rows n0 + 2il and n0 + 2il + 1 of K column kl (16
+// contiguous bytes) and feeds them to two MFMAs, one over the wave's even
+// output columns and one over its odd ones, so one A load serves two MFMAs
+// and a B load instruction covers 4 x 256 contiguous bytes.  One-wave
+// workgroups: the dispatcher hands out work in units of one wave, so a grid
+// a little over a multiple of the CU count does not leave a few CUs a whole
+// extra round.
+__global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
+    constexpr int U = 8;   // K steps per batch; two batches in flight
+    const int lane = threadIdx.x;
+    const int n0 = blockIdx.x * 32;
+    const int m0 = blockIdx.y * 16;
+    if (n0 >= a.N) return;   // whole wave
+    const int il = lane & 15, kl = lane >> 4;
+    const int mi = min(m0 + il, a.M - 1);
+    const long long astep = 4 * a.A.ld, bstep = 4 * a.B.ld;
+    d4 ev = {0.0, 0.0, 0.0, 0.0}, od = {0.0, 0.0, 0.0, 0.0};
+    for (int kd = 0; kd < a.K;) {
+        const int len = rhs_chunk(a.B, a.B.c0 + kd, a.K - kd);
+        const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + kl);
+        const double *pb = a.B.p + elem_off(a.B, a.B.r0 + n0 + 2 * il, a.B.c0 + kd + kl);
+        const int nk = len / 4;
+        int t = 0;
+        if (nk >= U) {
+            double av[U];
+            d2 bv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                av[u] = pa[u * astep];
+                bv[u] = *reinterpret_cast<const d2 *>(pb + u * bstep);
+            }
+            for (; t + U <= nk; t += U) {
+                double an[U];
+                d2 bn[U];
+                const bool more = t + 2 * U <= nk;
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        an[u] = pa[(t + U + u) * astep];
+                        bn[u] = *reinterpret_cast<const d2 *>(pb + (t + U + u) * bstep);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ev = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u].x, ev, 0, 0, 0);
+                    od = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u].y, od, 0, 0, 0);
+                }
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        av[u] = an[u];
+                        bv[u] = bn[u];
+                    }
+                }
+            }
+        }
+        for (; t < nk; ++t) {
+            const double x = pa[t * astep];
+            const d2 y = *reinterpret_cast<const d2 *>(pb + t * bstep);
+            ev = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.x, ev, 0, 0, 0);
+            od = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.y, od, 0, 0, 0);
+        }
+        kd += len;
+    }
+    // element r of a lane: row m0 + kl + 4r, columns n0 + 2il (ev), + 1 (od)
+    double *ce = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 2 * il);
+    double *co = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 2 * il + 1);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (m0 + kl + 4 * r >= a.M) break;
+        double v = a.alpha * ev[r], w = a.alpha * od[r];
+        if (a.beta != 0.0) {
+            v += a.beta * ce[4 * r];
+            w += a.beta * co[4 * r];
+        }
+        ce[4 * r] = v;
+        co[4 * r] = w;
+    }
+}
+
+// B transposed (K down a stored column), N a multiple of 32, K chunks of
+// whole 16-wide groups: one wave owns 16 rows x 32 columns, the two 16-column
+// halves taking the same A registers (rhs_run_tb16's K order, 4 consecutive
+// K per lane), so A costs one load per two MFMAs.  One-wave workgroups.
+__global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
+    constexpr int G = 2;   // 16-wide K groups per batch; two batches in flight
+    const int lane = threadIdx.x;
+    const int n0 = blockIdx.x * 32;
+    const int m0 = blockIdx.y * 16;
+    if (n0 >= a.N) return;
+    const int il = lane & 15, kl = lane >> 4;
+    const int mi = min(m0 + il, a.M - 1);
+    const long long lda = a.A.ld;
+    d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+    for (int kd = 0; kd < a.K;) {
+        const int len = rhs_chunk(a.B, a.B.r0 + kd, a.K - kd);
+        const double *qa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + 4 * kl);
+        const double *qb0 = a.B.p + elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + il);
+        const double *qb1 = a.B.p + elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + 16 + il);
+        const int ng = len / 16;
+        double av[G][4];
+        d2 bv[G][4];
+        auto load = [&](int g0, double (&x)[G][4], d2 (&y)[G][4]) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const long long k = 16LL * (g0 + g);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[g][j] = qa[(k + j) * lda];
+                y[g][0] = *reinterpret_cast<const d2 *>(qb0 + k);
+                y[g][1] = *reinterpret_cast<const d2 *>(qb0 + k + 2);
+                y[g][2] = *reinterpret_cast<const d2 *>(qb1 + k);
+                y[g][3] = *reinterpret_cast<const d2 *>(qb1 + k + 2);
+            }
+        };
+        auto mma = [&](const double (&x)[4], const d2 (&y)[4]) {
+            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[0], y[0].x, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[0], y[2].x, c1, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[1], y[0].y, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[1], y[2].y, c1, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[2], y[1].x, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[2], y[3].x, c1, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[3], y[1].y, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[3], y[3].y, c1, 0, 0, 0);
+        };
+        int g0 = 0;
+        if (ng >= G) {
+            load(0, av, bv);
+            for (; g0 + G <= ng; g0 += G) {
+                double an[G][4];
+                d2 bn[G][4];
+                const bool more = g0 + 2 * G <= ng;
+                if (more) load(g0 + G, an, bn);
+#pragma unroll
+                for (int g = 0; g < G; ++g) mma(av[g], bv[g]);
+                if (more) {
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            av[g][j] = an[g][j];
+                            bv[g][j] = bn[g][j];
+                        }
+                    }
+                }
+            }
+        }
+        for (; g0 < ng; ++g0) {
+            double x[1][4];
+            d2 y[1][4];
+            const long long k = 16LL * g0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[0][j] = qa[(k + j) * lda];
+            y[0][0] = *reinterpret_cast<const d2 *>(qb0 + k);
+            y[0][1] = *reinterpret_cast<const d2 *>(qb0 + k + 2);
+            y[0][2] = *reinterpret_cast<const d2 *>(qb1 + k);
+            y[0][3] = *reinterpret_cast<const d2 *>(qb1 + k + 2);
+            mma(x[0], y[0]);
+        }
+        kd += len;
+    }
+    // element r of a lane: row m0 + kl + 4r, columns n0 + il (c0), + 16 (c1)
+    double *ca = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + il);
+    double *cb = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 16 + il);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (m0 + kl + 4 * r >= a.M) break;
+        double v = a.alpha * c0[r], w = a.alpha * c1[r];
+        if (a.beta != 0.0) {
+            v += a.beta * ca[4 * r];
+            w += a.beta * cb[4 * r];
+        }
+        ca[4 * r] = v;
+        cb[4 * r] = w;
+    }
+}
+
+//
+// KS = 4 (split K): the block's four waves share one 16 x 16 output and take
+// a quarter of K each, reduced through LDS in a fixed order.  Each wave's
+// serial MFMA and load chain is a quarter as long, which is what bounds the
+// small launches (a diagonal-tile product: 8 blocks) and a large launch's
+// tail.
+template <int TB, int KS>
+__global__ __launch_bounds__(256) void k_rhs(RArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n0 = (KS == 1 ? blockIdx.x * 4 + wave : blockIdx.x) * 16;
+    const int m0 = blockIdx.y * 16;
+    if (n0 >= a.N) return;   // whole wave (whole block for KS = 4)
+    if (KS > 1) {
+        a.K /= KS;
+        const int kb = wave * a.K;
+        a.A.c0 += kb;
+        if (TB == 0) a.B.c0 += kb; else a.B.r0 += kb;
+    }
+    const int il = lane & 15, kl = lane >> 4;
+    // lanes past M or N read a valid row/column instead: they only feed
+    // output rows/columns that are never stored
+    const int mi = min(m0 + il, a.M - 1), ni = min(n0 + il, a.N - 1);
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int kd = 0; kd < a.K;) {
+        const int len = rhs_chunk(a.B, (TB == 0 ? a.B.c0 : a.B.r0) + kd, a.K - kd);
+        if (TB == 1 && (len & 15) == 0 && len >= 128) {
+            const double *qa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + 4 * kl);
+            const double *qb = a.B.p + elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + ni);
+            rhs_run_tb16(qa, a.A.ld, qb, len / 16, acc);
+        } else {
+            const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + kl);
+            const double *pb;
+            long long bstep;
+            if (TB == 0) {
+                pb = a.B.p + elem_off(a.B, a.B.r0 + ni, a.B.c0 + kd + kl);
+                bstep = 4 * a.B.ld;
+            } else {
+                pb = a.B.p + elem_off(a.B, a.B.r0 + kd + kl, a.B.c0 + ni);
+                bstep = 4;
+            }
+            rhs_run<16>(pa, 4 * a.A.ld, pb, bstep, len / 4, acc);
+        }
+        kd += len;
+    }
     if constexpr (KS > 1) {
         __shared__ d4 part[KS - 1][64];
         if (wave > 0) part[wave - 1][lane] = acc;
@@ -960,10 +1082,11 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
     if (g.A.ts > 0 || g.C.ts > 0 || (g.K & 3)) return hipErrorInvalidValue;
     const Opnd &B = g.B;
     if (B.ts > 0) {
-        // each 16-column group and the whole K range inside one tile
+        // each 16-column group inside one tile; K chunks (split at tile
+        // edges) of whole 4-wide steps
         if (B.ts % 16) return hipErrorInvalidValue;
-        if (!tb && ((B.r0 % 16) || (B.c0 % B.ts) + g.K > B.ts)) return hipErrorInvalidValue;
-        if (tb && ((B.c0 % 16) || (B.r0 % B.ts) + g.K > B.ts)) return hipErrorInvalidValue;
+        if (!tb && ((B.r0 % 16) || (B.c0 % 4))) return hipErrorInvalidValue;
+        if (tb && ((B.c0 % 16) || (B.r0 % 4))) return hipErrorInvalidValue;
     }
     RArgs a;
     a.A = g.A;
@@ -988,8 +1111,16 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
                                                   : B.ld % 2 == 0 && B.r0 % 2 == 0) &&
                ((uintptr_t)B.p & 15) == 0) {
         // 32-column groups inside one tile, 16-byte aligned pairs of B rows
-        dim3 grid((unsigned)((g.N + 127) / 128), (unsigned)((g.M + 15) / 16));
-        hipLaunchKernelGGL(k_rhs_w2, grid, dim3(256), 0, s, a);
+        dim3 grid((unsigned)(g.N / 32), (unsigned)((g.M + 15) / 16));
+        hipLaunchKernelGGL(k_rhs_w2, grid, dim3(64), 0, s, a);
+    } else if (tb && g.N % 32 == 0 && g.K % 16 == 0 &&
+               (B.ts > 0 ? B.ts % 32 == 0 && B.c0 % 32 == 0 && B.r0 % 16 == 0
+                         : B.ld % 2 == 0 && B.r0 % 2 == 0) &&
+               ((uintptr_t)B.p & 15) == 0) {
+        // 32-column groups inside one tile, K chunks of whole 16-wide groups
+        // starting on 16-byte boundaries
+        dim3 grid((unsigned)(g.N / 32), (unsigned)((g.M + 15) / 16));
+        hipLaunchKernelGGL(k_rhs_tw2, grid, dim3(64), 0, s, a);
     } else {
         dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 15) / 16));
         if (tb)

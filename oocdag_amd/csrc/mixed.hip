@@ -190,14 +190,24 @@ hipError_t normal_block(double *dB, int rows, int cols, long long ld, unsigned l
     return hipGetLastError();
 }
 
+// Tile columns per super-step of the solves and the residual: the updates
+// reaching past a super-step go out as one launch with K = W*NB (~4096).
+int solve_step_width(int NB) { return std::max(1, 4096 / NB); }
+
 // Y^T (nrhs x Np) += alpha * X^T A for symmetric A in packed lower tiles
-// with full diagonal tiles, as 2*bb launches with K = NB each:
-//   tile row i   : Y^T[:, :(i+1)NB] += X^T[:, i] A[i, :(i+1)NB]   (stored, read transposed)
-//   tile column j: Y^T[:, rows > j]  += X^T[:, j] A[rows > j, j]^T
+// with full diagonal tiles.  Tile row i of the stored triangle gives
+//   Y^T[:, j] += X^T[:, i] A[i, j]     (j <= i, the tile read transposed)
+// and tile column j gives
+//   Y^T[:, i] += X^T[:, j] A[i, j]^T   (i > j).
+// In super-blocks of W tile rows [s0, s1): one launch for the rows' parts
+// left of the block (K = (s1-s0)*NB across the W tile rows), one for the
+// columns' parts below it, and per tile the parts inside the block.
 hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt, long long ld,
                   int nrhs, double alpha, double beta, hipStream_t s) {
-    const int bb = (N + NB - 1) / NB;
+    const int bb = (N + NB - 1) / NB, W = solve_step_width(NB);
     const Opnd Ap = panel((double *)dA, NB, bb, 1);
+    const GemmKind nt = nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL;
+    const GemmKind tt = nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB;
     hipError_t e;
     if (beta != 1.0) {
         // scale Y^T by beta via a zero-K GEMM
@@ -209,22 +219,27 @@ hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt,
         g.alpha = 0.0; g.beta = beta; g.lower = 0;
         if ((e = launch_gemm(g, GEMM_GENERAL, s)) != hipSuccess) return e;
     }
-    for (int i = 0; i < bb; ++i) {
+    // Y^T[:, c0 .. c0+n*NB) += alpha X^T[:, a0 .. a0+k*NB) op(A block at (br, bc))
+    auto product = [&](int a0, int c0, int br, int bc, int n, int k, GemmKind kind) -> hipError_t {
         Gemm g;
-        g.A = plain((double *)Xt, ld, 0, i * NB);
-        g.B = Ap; g.B.r0 = i * NB; g.B.c0 = 0;  // operand (n, k) = A(i*NB + k, n)
-        g.C = plain(Yt, ld, 0, 0);
-        g.M = nrhs; g.N = (i + 1) * NB; g.K = NB;
+        g.A = plain((double *)Xt, ld, 0, a0 * NB);
+        g.B = Ap; g.B.r0 = br * NB; g.B.c0 = bc * NB;
+        g.C = plain(Yt, ld, 0, c0 * NB);
+        g.M = nrhs; g.N = n * NB; g.K = k * NB;
         g.alpha = alpha; g.beta = 1.0; g.lower = 0;
-        if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
-        if (i + 1 < bb) {
-            Gemm h;
-            h.A = plain((double *)Xt, ld, 0, i * NB);
-            h.B = Ap; h.B.r0 = (i + 1) * NB; h.B.c0 = i * NB;
-            h.C = plain(Yt, ld, 0, (i + 1) * NB);
-            h.M = nrhs; h.N = (bb - i - 1) * NB; h.K = NB;
-            h.alpha = alpha; h.beta = 1.0; h.lower = 0;
-            if ((e = launch_gemm(h, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL, s)) != hipSuccess) return e;
+        return launch_gemm(g, kind, s);
+    };
+    for (int s0 = 0; s0 < bb; s0 += W) {
+        const int s1 = std::min(bb, s0 + W);
+        // rows s0..s1-1, columns j < s0
+        if (s0 > 0 && (e = product(s0, 0, s0, 0, s0, s1 - s0, tt)) != hipSuccess) return e;
+        // columns s0..s1-1, rows i >= s1
+        if (s1 < bb && (e = product(s0, s1, s1, s0, bb - s1, s1 - s0, nt)) != hipSuccess) return e;
+        for (int i = s0; i < s1; ++i) {
+            // row i, columns s0..i; column i, rows i+1..s1-1
+            if ((e = product(i, s0, i, s0, i + 1 - s0, 1, tt)) != hipSuccess) return e;
+            if (i + 1 < s1 && (e = product(i, i + 1, i + 1, i, s1 - i - 1, 1, nt)) != hipSuccess)
+                return e;
         }
     }
     return hipSuccess;
@@ -332,16 +347,21 @@ hipError_t tile_inverses(const double *dY, int N, int NB, const double *dinv_all
     return hipSuccess;
 }
 
-// X^T <- B^T A^-1 with the diagonal tiles' inverses Z, one GEMM per tile per
-// sweep.  The forward sweep writes Y^T into Yt (nrhs x N, ld ldx) while it
-// updates the rest of B^T in Xt; the backward sweep writes X^T back into Xt
-// while it updates the rest of Y^T in Yt: each diagonal product lands in the
-// other buffer, so no copy follows it.
+// X^T <- B^T A^-1 with the diagonal tiles' inverses Z.  The forward sweep
+// writes Y^T into Yt (nrhs x N, ld ldx) while it updates the rest of B^T in
+// Xt; the backward sweep writes X^T back into Xt while it updates the rest
+// of Y^T in Yt: each diagonal product lands in the other buffer, so no copy
+// follows it.  Each sweep goes in super-steps of W tiles: inside one, a
+// diagonal product per tile and a narrow update of the super-step's later
+// tiles; after it, one update of everything beyond with K = W*NB (the
+// factor's W tile columns, or tile rows, at once).
 hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double *Xt, int nrhs,
                        long long ldx, double *Yt, hipStream_t s) {
-    const int bb = (N + NB - 1) / NB;
+    const int bb = (N + NB - 1) / NB, W = solve_step_width(NB);
     const Opnd Lp = panel((double *)dY, NB, bb, 1);
     const size_t per = (size_t)NB * NB;
+    const GemmKind nt = nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL;
+    const GemmKind tt = nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB;
     hipError_t e;
     auto diag = [&](const double *src, double *dst, int k, GemmKind kind) -> hipError_t {
         Gemm g;
@@ -352,29 +372,38 @@ hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double 
         g.alpha = 1.0; g.beta = 0.0; g.lower = 0;
         return launch_gemm(g, kind, s);
     };
-    for (int k = 0; k < bb; ++k) {  // Y^T_k = B^T_k L_kk^-T = B^T_k Z_k
-        if ((e = diag(Xt, Yt, k, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB)) != hipSuccess) return e;
-        if (k + 1 < bb) {
-            Gemm g;
-            g.A = plain(Yt, ldx, 0, k * NB);
-            g.B = Lp; g.B.r0 = (k + 1) * NB; g.B.c0 = k * NB;
-            g.C = plain(Xt, ldx, 0, (k + 1) * NB);
-            g.M = nrhs; g.N = (bb - k - 1) * NB; g.K = NB;
-            g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
-            if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL, s)) != hipSuccess) return e;
+    // C[:, c0 .. c0+n*NB) -= src[:, a0 .. a0+k*NB) * op(L block at (br, bc))
+    auto update = [&](const double *src, int a0, double *dst, int c0, int br, int bc, int n,
+                      int k, GemmKind kind) -> hipError_t {
+        Gemm g;
+        g.A = plain((double *)src, ldx, 0, a0 * NB);
+        g.B = Lp; g.B.r0 = br * NB; g.B.c0 = bc * NB;
+        g.C = plain(dst, ldx, 0, c0 * NB);
+        g.M = nrhs; g.N = n * NB; g.K = k * NB;
+        g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
+        return launch_gemm(g, kind, s);
+    };
+    for (int s0 = 0; s0 < bb; s0 += W) {   // Y^T_k = B^T_k L_kk^-T = B^T_k Z_k
+        const int s1 = std::min(bb, s0 + W);
+        for (int k = s0; k < s1; ++k) {
+            if ((e = diag(Xt, Yt, k, tt)) != hipSuccess) return e;
+            // B^T_i -= Y^T_k L_ik^T, k < i < s1
+            if (k + 1 < s1 && (e = update(Yt, k, Xt, k + 1, k + 1, k, s1 - k - 1, 1, nt)) != hipSuccess)
+                return e;
         }
+        // B^T_i -= sum_{s0 <= k < s1} Y^T_k L_ik^T, i >= s1
+        if (s1 < bb && (e = update(Yt, s0, Xt, s1, s1, s0, bb - s1, s1 - s0, nt)) != hipSuccess)
+            return e;
     }
-    for (int k = bb - 1; k >= 0; --k) {  // X^T_k = Y^T_k L_kk^-1 = Y^T_k Z_k^T
-        if ((e = diag(Yt, Xt, k, nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL)) != hipSuccess) return e;
-        if (k > 0) {
-            Gemm g;
-            g.A = plain(Xt, ldx, 0, k * NB);
-            g.B = Lp; g.B.r0 = k * NB; g.B.c0 = 0;  // L[k, :k], read transposed
-            g.C = plain(Yt, ldx, 0, 0);
-            g.M = nrhs; g.N = k * NB; g.K = NB;
-            g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
-            if ((e = launch_gemm(g, nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB, s)) != hipSuccess) return e;
+    for (int s1 = bb; s1 > 0; s1 -= W) {   // X^T_k = Y^T_k L_kk^-1 = Y^T_k Z_k^T
+        const int s0 = std::max(0, s1 - W);
+        for (int k = s1 - 1; k >= s0; --k) {
+            if ((e = diag(Yt, Xt, k, nt)) != hipSuccess) return e;
+            // Y^T_j -= X^T_k L_kj, s0 <= j < k (tile row k read transposed)
+            if (k > s0 && (e = update(Xt, k, Yt, s0, k, s0, k - s0, 1, tt)) != hipSuccess) return e;
         }
+        // Y^T_j -= sum_{s0 <= k < s1} X^T_k L_kj, j < s0
+        if (s0 > 0 && (e = update(Xt, s0, Yt, 0, s0, 0, s0, s1 - s0, tt)) != hipSuccess) return e;
     }
     return hipSuccess;
 }

@@ -71,8 +71,9 @@ enum GemmKind {
     GEMM_SKINNY = 7,
     GEMM_SKINNY_TB = 8,
     // right-hand-side panels: A and C plain, any M (16-row groups), one
-    // wave per 16 columns, operands streamed straight into MFMA registers;
-    // a tiled B must keep each 16-column group and the K range in one tile
+    // wave per 16 or 32 columns, operands streamed straight into MFMA
+    // registers; a tiled B keeps each 16-column group in one tile, K may
+    // span tiles
     GEMM_RHS = 9,
     GEMM_RHS_TB = 10
 };
