@@ -672,6 +672,29 @@ __device__ __forceinline__ long long elem_off(const Opnd &o, int gr, int gc) {
     return tix(ti, tk, o.ym, o.tri) * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
 }
 
+// B operand elements are double, or float (a factor kept in FP32 for the
+// refinement's correction solves) widened to double on load: the MFMAs and
+// the accumulation stay FP64.
+template <typename BT> __device__ __forceinline__ const BT *bptr(const Opnd &B, long long off) {
+    return reinterpret_cast<const BT *>(B.p) + off;
+}
+// two consecutive elements (16-byte aligned for double, 8 for float)
+__device__ __forceinline__ d2 ld2(const double *q) { return *reinterpret_cast<const d2 *>(q); }
+__device__ __forceinline__ d2 ld2(const float *q) {
+    const float2 v = *reinterpret_cast<const float2 *>(q);
+    return d2{(double)v.x, (double)v.y};
+}
+// four consecutive elements as two pairs (16-byte aligned)
+__device__ __forceinline__ void ld4(const double *q, d2 &lo, d2 &hi) {
+    lo = *reinterpret_cast<const d2 *>(q);
+    hi = *reinterpret_cast<const d2 *>(q + 2);
+}
+__device__ __forceinline__ void ld4(const float *q, d2 &lo, d2 &hi) {
+    const float4 v = *reinterpret_cast<const float4 *>(q);
+    lo = d2{(double)v.x, (double)v.y};
+    hi = d2{(double)v.z, (double)v.w};
+}
+
 // K may span several tiles of a packed B (the solves' bulk updates take 8
 // tile columns or rows at once): the kernels walk K in chunks that end at
 // tile edges, with the operand pointers recomputed per chunk.
@@ -681,8 +704,8 @@ __device__ __forceinline__ int rhs_chunk(const Opnd &B, int gk, int left) {
 
 // acc += A(m, 4t + kl) * B(n, 4t + kl) over t < nk: lane (il, kl) of A at
 // pa + t*astep and of B at pb + t*bstep; two batches of U loads in flight.
-template <int U>
-__device__ __forceinline__ void rhs_run(const double *pa, long long astep, const double *pb,
+template <int U, typename BT>
+__device__ __forceinline__ void rhs_run(const double *pa, long long astep, const BT *pb,
                                         long long bstep, int nk, d4 &acc) {
     int t = 0;
     if (nk >= U) {
@@ -690,7 +713,7 @@ __device__ __forceinline__ void rhs_run(const double *pa, long long astep, const
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             av[u] = pa[u * astep];
-            bv[u] = pb[u * bstep];
+            bv[u] = (double)pb[u * bstep];
         }
         for (; t + U <= nk; t += U) {
             double an[U], bq[U];
@@ -699,7 +722,7 @@ __device__ __forceinline__ void rhs_run(const double *pa, long long astep, const
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     an[u] = pa[(t + U + u) * astep];
-                    bq[u] = pb[(t + U + u) * bstep];
+                    bq[u] = (double)pb[(t + U + u) * bstep];
                 }
             }
 #pragma unroll
@@ -713,7 +736,8 @@ __device__ __forceinline__ void rhs_run(const double *pa, long long astep, const
             }
         }
     }
-    for (; t < nk; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t * astep], pb[t * bstep], acc, 0, 0, 0);
+    for (; t < nk; ++t)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t * astep], (double)pb[t * bstep], acc, 0, 0, 0);
 }
 
 // Transposed B with K a multiple of 16: K runs down a stored column, so a
@@ -722,7 +746,8 @@ __device__ __forceinline__ void rhs_run(const double *pa, long long astep, const
 // j: MFMA j covers K = 16g + 4kq + j over the 4 lane groups kq.  A is loaded
 // to match (qa = A[m][4kq], element j of group g at qa + (16g + j)*lda).
 // Two batches of 4 groups in flight.
-__device__ __forceinline__ void rhs_run_tb16(const double *qa, long long lda, const double *qb,
+template <typename BT>
+__device__ __forceinline__ void rhs_run_tb16(const double *qa, long long lda, const BT *qb,
                                              int ng, d4 &acc) {
     constexpr int G = 4;
     double av[G][4];
@@ -733,8 +758,7 @@ __device__ __forceinline__ void rhs_run_tb16(const double *qa, long long lda, co
             const long long k = 16LL * (g0 + g);
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[g][j] = qa[(k + j) * lda];
-            y[g][0] = *reinterpret_cast<const d2 *>(qb + k);
-            y[g][1] = *reinterpret_cast<const d2 *>(qb + k + 2);
+            ld4(qb + k, y[g][0], y[g][1]);
         }
     };
     int g0 = 0;
@@ -766,8 +790,8 @@ __device__ __forceinline__ void rhs_run_tb16(const double *qa, long long lda, co
     // remaining groups (fewer than G), one at a time
     for (; g0 < ng; ++g0) {
         const long long k = 16LL * g0;
-        const d2 b0 = *reinterpret_cast<const d2 *>(qb + k);
-        const d2 b1 = *reinterpret_cast<const d2 *>(qb + k + 2);
+        d2 b0, b1;
+        ld4(qb + k, b0, b1);
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[k * lda], b0.x, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 1) * lda], b0.y, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 2) * lda], b1.x, acc, 0, 0, 0);
@@ -783,6 +807,7 @@ __device__ __forceinline__ void rhs_run_tb16(const double *qa, long long lda, co
 // workgroups: the dispatcher hands out work in units of one wave, so a grid
 // a little over a multiple of the CU count does not leave a few CUs a whole
 // extra round.
+template <typename BT>
 __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
     constexpr int U = 8;   // K steps per batch; two batches in flight
     const int lane = threadIdx.x;
@@ -796,7 +821,7 @@ __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
     for (int kd = 0; kd < a.K;) {
         const int len = rhs_chunk(a.B, a.B.c0 + kd, a.K - kd);
         const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + kl);
-        const double *pb = a.B.p + elem_off(a.B, a.B.r0 + n0 + 2 * il, a.B.c0 + kd + kl);
+        const BT *pb = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + n0 + 2 * il, a.B.c0 + kd + kl));
         const int nk = len / 4;
         int t = 0;
         if (nk >= U) {
@@ -805,7 +830,7 @@ __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 av[u] = pa[u * astep];
-                bv[u] = *reinterpret_cast<const d2 *>(pb + u * bstep);
+                bv[u] = ld2(pb + u * bstep);
             }
             for (; t + U <= nk; t += U) {
                 double an[U];
@@ -815,7 +840,7 @@ __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         an[u] = pa[(t + U + u) * astep];
-                        bn[u] = *reinterpret_cast<const d2 *>(pb + (t + U + u) * bstep);
+                        bn[u] = ld2(pb + (t + U + u) * bstep);
                     }
                 }
 #pragma unroll
@@ -834,7 +859,7 @@ __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
         }
         for (; t < nk; ++t) {
             const double x = pa[t * astep];
-            const d2 y = *reinterpret_cast<const d2 *>(pb + t * bstep);
+            const d2 y = ld2(pb + t * bstep);
             ev = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.x, ev, 0, 0, 0);
             od = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.y, od, 0, 0, 0);
         }
@@ -860,6 +885,7 @@ __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
 // whole 16-wide groups: one wave owns 16 rows x 32 columns, the two 16-column
 // halves taking the same A registers (rhs_run_tb16's K order, 4 consecutive
 // K per lane), so A costs one load per two MFMAs.  One-wave workgroups.
+template <typename BT>
 __global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
     constexpr int G = 2;   // 16-wide K groups per batch; two batches in flight
     const int lane = threadIdx.x;
@@ -873,8 +899,8 @@ __global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
     for (int kd = 0; kd < a.K;) {
         const int len = rhs_chunk(a.B, a.B.r0 + kd, a.K - kd);
         const double *qa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + 4 * kl);
-        const double *qb0 = a.B.p + elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + il);
-        const double *qb1 = a.B.p + elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + 16 + il);
+        const BT *qb0 = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + il));
+        const BT *qb1 = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + 16 + il));
         const int ng = len / 16;
         double av[G][4];
         d2 bv[G][4];
@@ -884,10 +910,8 @@ __global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
                 const long long k = 16LL * (g0 + g);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) x[g][j] = qa[(k + j) * lda];
-                y[g][0] = *reinterpret_cast<const d2 *>(qb0 + k);
-                y[g][1] = *reinterpret_cast<const d2 *>(qb0 + k + 2);
-                y[g][2] = *reinterpret_cast<const d2 *>(qb1 + k);
-                y[g][3] = *reinterpret_cast<const d2 *>(qb1 + k + 2);
+                ld4(qb0 + k, y[g][0], y[g][1]);
+                ld4(qb1 + k, y[g][2], y[g][3]);
             }
         };
         auto mma = [&](const double (&x)[4], const d2 (&y)[4]) {
@@ -928,10 +952,8 @@ __global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
             const long long k = 16LL * g0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[0][j] = qa[(k + j) * lda];
-            y[0][0] = *reinterpret_cast<const d2 *>(qb0 + k);
-            y[0][1] = *reinterpret_cast<const d2 *>(qb0 + k + 2);
-            y[0][2] = *reinterpret_cast<const d2 *>(qb1 + k);
-            y[0][3] = *reinterpret_cast<const d2 *>(qb1 + k + 2);
+            ld4(qb0 + k, y[0][0], y[0][1]);
+            ld4(qb1 + k, y[0][2], y[0][3]);
             mma(x[0], y[0]);
         }
         kd += len;
@@ -958,7 +980,7 @@ __global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
 // serial MFMA and load chain is a quarter as long, which is what bounds the
 // small launches (a diagonal-tile product: 8 blocks) and a large launch's
 // tail.
-template <int TB, int KS>
+template <int TB, int KS, typename BT>
 __global__ __launch_bounds__(256) void k_rhs(RArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n0 = (KS == 1 ? blockIdx.x * 4 + wave : blockIdx.x) * 16;
@@ -979,17 +1001,17 @@ __global__ __launch_bounds__(256) void k_rhs(RArgs a) {
         const int len = rhs_chunk(a.B, (TB == 0 ? a.B.c0 : a.B.r0) + kd, a.K - kd);
         if (TB == 1 && (len & 15) == 0 && len >= 128) {
             const double *qa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + 4 * kl);
-            const double *qb = a.B.p + elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + ni);
+            const BT *qb = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + ni));
             rhs_run_tb16(qa, a.A.ld, qb, len / 16, acc);
         } else {
             const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + kl);
-            const double *pb;
+            const BT *pb;
             long long bstep;
             if (TB == 0) {
-                pb = a.B.p + elem_off(a.B, a.B.r0 + ni, a.B.c0 + kd + kl);
+                pb = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + ni, a.B.c0 + kd + kl));
                 bstep = 4 * a.B.ld;
             } else {
-                pb = a.B.p + elem_off(a.B, a.B.r0 + kd + kl, a.B.c0 + ni);
+                pb = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + kd + kl, a.B.c0 + ni));
                 bstep = 4;
             }
             rhs_run<16>(pa, 4 * a.A.ld, pb, bstep, len / 4, acc);
@@ -1078,6 +1100,42 @@ hipError_t launch_one(const Gemm &g, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename BT>
+void launch_rhs_kernels(const Gemm &g, const RArgs &a, int tb, hipStream_t s) {
+    const Opnd &B = g.B;
+    // split K when the unsplit grid would not give every CU a block (a
+    // large grid streams B faster unsplit) and each quarter keeps whole
+    // 16-wide K groups, at least 64
+    const long long blocks1 = (long long)((g.N + 63) / 64) * ((g.M + 15) / 16);
+    if (blocks1 < 256 && g.K % 64 == 0 && g.K >= 256) {
+        dim3 grid((unsigned)((g.N + 15) / 16), (unsigned)((g.M + 15) / 16));
+        if (tb)
+            hipLaunchKernelGGL((k_rhs<1, 4, BT>), grid, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_rhs<0, 4, BT>), grid, dim3(256), 0, s, a);
+    } else if (!tb && g.N % 32 == 0 && (B.ts > 0 ? B.ts % 32 == 0 && B.r0 % 32 == 0
+                                                  : B.ld % 2 == 0 && B.r0 % 2 == 0) &&
+               ((uintptr_t)B.p & 15) == 0) {
+        // 32-column groups inside one tile, aligned pairs of B rows
+        dim3 grid((unsigned)(g.N / 32), (unsigned)((g.M + 15) / 16));
+        hipLaunchKernelGGL(k_rhs_w2<BT>, grid, dim3(64), 0, s, a);
+    } else if (tb && g.N % 32 == 0 && g.K % 16 == 0 &&
+               (B.ts > 0 ? B.ts % 32 == 0 && B.c0 % 32 == 0 && B.r0 % 16 == 0
+                         : B.ld % 4 == 0 && B.r0 % 4 == 0) &&
+               ((uintptr_t)B.p & 15) == 0) {
+        // 32-column groups inside one tile, K chunks of whole 16-wide groups
+        // starting on 16-byte boundaries
+        dim3 grid((unsigned)(g.N / 32), (unsigned)((g.M + 15) / 16));
+        hipLaunchKernelGGL(k_rhs_tw2<BT>, grid, dim3(64), 0, s, a);
+    } else {
+        dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 15) / 16));
+        if (tb)
+            hipLaunchKernelGGL((k_rhs<1, 1, BT>), grid, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_rhs<0, 1, BT>), grid, dim3(256), 0, s, a);
+    }
+}
+
 hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
     if (g.A.ts > 0 || g.C.ts > 0 || (g.K & 3)) return hipErrorInvalidValue;
     const Opnd &B = g.B;
@@ -1088,6 +1146,10 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
         if (!tb && ((B.r0 % 16) || (B.c0 % 4))) return hipErrorInvalidValue;
         if (tb && ((B.c0 % 16) || (B.r0 % 4))) return hipErrorInvalidValue;
     }
+    // float B: 4 consecutive elements of a stored column on 16-byte
+    // boundaries (the transposed loads), pairs on 8-byte boundaries
+    if (g.b32 && ((B.ts <= 0 && (B.ld % 4 || B.r0 % 4)) || ((uintptr_t)B.p & 15)))
+        return hipErrorInvalidValue;
     RArgs a;
     a.A = g.A;
     a.B = g.B;
@@ -1097,37 +1159,10 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
     a.K = g.K;
     a.alpha = g.alpha;
     a.beta = g.beta;
-    // split K when the unsplit grid would not give every CU a block (a
-    // large grid streams B faster unsplit) and each quarter keeps whole
-    // 16-wide K groups, at least 64
-    const long long blocks1 = (long long)((g.N + 63) / 64) * ((g.M + 15) / 16);
-    if (blocks1 < 256 && g.K % 64 == 0 && g.K >= 256) {
-        dim3 grid((unsigned)((g.N + 15) / 16), (unsigned)((g.M + 15) / 16));
-        if (tb)
-            hipLaunchKernelGGL((k_rhs<1, 4>), grid, dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_rhs<0, 4>), grid, dim3(256), 0, s, a);
-    } else if (!tb && g.N % 32 == 0 && (B.ts > 0 ? B.ts % 32 == 0 && B.r0 % 32 == 0
-                                                  : B.ld % 2 == 0 && B.r0 % 2 == 0) &&
-               ((uintptr_t)B.p & 15) == 0) {
-        // 32-column groups inside one tile, 16-byte aligned pairs of B rows
-        dim3 grid((unsigned)(g.N / 32), (unsigned)((g.M + 15) / 16));
-        hipLaunchKernelGGL(k_rhs_w2, grid, dim3(64), 0, s, a);
-    } else if (tb && g.N % 32 == 0 && g.K % 16 == 0 &&
-               (B.ts > 0 ? B.ts % 32 == 0 && B.c0 % 32 == 0 && B.r0 % 16 == 0
-                         : B.ld % 2 == 0 && B.r0 % 2 == 0) &&
-               ((uintptr_t)B.p & 15) == 0) {
-        // 32-column groups inside one tile, K chunks of whole 16-wide groups
-        // starting on 16-byte boundaries
-        dim3 grid((unsigned)(g.N / 32), (unsigned)((g.M + 15) / 16));
-        hipLaunchKernelGGL(k_rhs_tw2, grid, dim3(64), 0, s, a);
-    } else {
-        dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 15) / 16));
-        if (tb)
-            hipLaunchKernelGGL((k_rhs<1, 1>), grid, dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_rhs<0, 1>), grid, dim3(256), 0, s, a);
-    }
+    if (g.b32)
+        launch_rhs_kernels<float>(g, a, tb, s);
+    else
+        launch_rhs_kernels<double>(g, a, tb, s);
     return hipGetLastError();
 }
 
@@ -1135,6 +1170,7 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
 
 hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
+    if (g.b32 && kind != GEMM_RHS && kind != GEMM_RHS_TB) return hipErrorInvalidValue;
     // Operand constraints of the streaming loads (16-byte row-pair loads,
     // 16-wide K steps that never straddle a tile).
     const Opnd *ops[3] = {&g.A, &g.B, &g.C};

@@ -355,21 +355,27 @@ hipError_t tile_inverses(const double *dY, int N, int NB, const double *dinv_all
 // diagonal product per tile and a narrow update of the super-step's later
 // tiles; after it, one update of everything beyond with K = W*NB (the
 // factor's W tile columns, or tile rows, at once).
-hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double *Xt, int nrhs,
-                       long long ldx, double *Yt, hipStream_t s) {
+//
+// f32 != 0: dY and Z hold float elements (the FP32 copy of a low-precision
+// factor), widened to double on load.
+hipError_t potrs_inv_t(const void *dY, int N, int NB, const void *Z, int f32, double *Xt,
+                       int nrhs, long long ldx, double *Yt, hipStream_t s) {
     const int bb = (N + NB - 1) / NB, W = solve_step_width(NB);
     const Opnd Lp = panel((double *)dY, NB, bb, 1);
     const size_t per = (size_t)NB * NB;
+    // the FP32 factor only feeds the right-hand-side kernels (nrhs <= 64)
+    if (f32 && nrhs > 64) return hipErrorInvalidValue;
     const GemmKind nt = nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL;
     const GemmKind tt = nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB;
     hipError_t e;
     auto diag = [&](const double *src, double *dst, int k, GemmKind kind) -> hipError_t {
         Gemm g;
         g.A = plain((double *)src, ldx, 0, k * NB);
-        g.B = plain((double *)Z + k * per, NB);
+        g.B = plain(f32 ? (double *)((const float *)Z + k * per) : (double *)Z + k * per, NB);
         g.C = plain(dst, ldx, 0, k * NB);
         g.M = nrhs; g.N = NB; g.K = NB;
         g.alpha = 1.0; g.beta = 0.0; g.lower = 0;
+        g.b32 = f32;
         return launch_gemm(g, kind, s);
     };
     // C[:, c0 .. c0+n*NB) -= src[:, a0 .. a0+k*NB) * op(L block at (br, bc))
@@ -381,6 +387,7 @@ hipError_t potrs_inv_t(const double *dY, int N, int NB, const double *Z, double 
         g.C = plain(dst, ldx, 0, c0 * NB);
         g.M = nrhs; g.N = n * NB; g.K = k * NB;
         g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
+        g.b32 = f32;
         return launch_gemm(g, kind, s);
     };
     for (int s0 = 0; s0 < bb; s0 += W) {   // Y^T_k = B^T_k L_kk^-T = B^T_k Z_k
@@ -550,18 +557,30 @@ hipError_t salloc(T **p, size_t bytes, hipStream_t s) {
 }
 }  // namespace
 
+__global__ void k_to_f32(const double *x, float *y, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = (float)x[i];
+}
+
+// W, Z, the 64-block inverses, their FP32 copies (low-precision factors)
+// and the two low-precision super-column buffers
 size_t mp_factor_bytes(int N, int NB) {
     const size_t bb = (size_t)((N + NB - 1) / NB);
     const size_t tiles = bb * (bb + 1) / 2 * NB * NB;
     const size_t sw = (size_t)mp_super_width(NB, (int)bb);
-    return (tiles + bb * NB * NB + bb * NB * 64) * 8 + 2 * bb * NB * sw * NB * 4;
+    return (tiles + bb * NB * NB + bb * NB * 64) * 8 + (tiles + bb * NB * NB) * 4 +
+           2 * bb * NB * sw * NB * 4;
 }
 
 void mp_release(MpFactor &f, hipStream_t s) {
     if (f.W) (void)hipFreeAsync(f.W, s);
     if (f.Z) (void)hipFreeAsync(f.Z, s);
     if (f.dinv) (void)hipFreeAsync(f.dinv, s);
+    if (f.W32) (void)hipFreeAsync(f.W32, s);
+    if (f.Z32) (void)hipFreeAsync(f.Z32, s);
     f.W = f.Z = f.dinv = nullptr;
+    f.W32 = f.Z32 = nullptr;
 }
 
 int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipStream_t s0,
@@ -605,7 +624,24 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
         }
         int r = factor_mixed(f.W, N, NB, precision, f.dinv, info, lp, scale, amax, s0, s1);
         if (r) { rc = OOCDAG_EHIP; break; }
-        if (!ok(tile_inverses(f.W, N, NB, f.dinv, f.Z, s0)) ||
+        if (!ok(tile_inverses(f.W, N, NB, f.dinv, f.Z, s0))) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        if (precision > 0) {
+            // the correction solves read the factor in FP32: its error is at
+            // the low precision's level already, and the sweeps stream half
+            // the bytes
+            const size_t zn = (size_t)bb * NB * NB;
+            if ((!f.W32 && !ok(salloc(&f.W32, tiles * 4, s0))) ||
+                (!f.Z32 && !ok(salloc(&f.Z32, zn * 4, s0)))) {
+                rc = OOCDAG_ENOMEM;
+                break;
+            }
+            hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s0, f.W, f.W32, (long long)tiles);
+            hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s0, f.Z, f.Z32, (long long)zn);
+        }
+        if (!ok(hipGetLastError()) ||
             !ok(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s0)) ||
             !ok(hipStreamSynchronize(s0))) {
             rc = OOCDAG_EHIP;
@@ -637,6 +673,10 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
     int rc = OOCDAG_OK;
     int it = 0;
     double eta = INFINITY, prev = INFINITY;
+    // a low-precision factor's solves read its FP32 copy (few right-hand sides)
+    const int lp32 = f.precision > 0 && f.W32 && f.Z32 && nrhs <= 64;
+    const void *LW = lp32 ? (const void *)f.W32 : (const void *)f.W;
+    const void *LZ = lp32 ? (const void *)f.Z32 : (const void *)f.Z;
     std::vector<double> bnorm(nrhs), hx(nrhs), hr(nrhs);
     do {
         if (!ok(hipMemsetAsync(Bt, 0, vb, s)) || !ok(launch_transpose(dB, ldb, Bt, ldt, N, nrhs, s))) {
@@ -646,7 +686,7 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
         hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Bt, ldt, nrhs, N, nrm);
         if (!ok(hipMemcpyAsync(bnorm.data(), nrm, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
             !ok(hipMemcpyAsync(Xt, Bt, vb, hipMemcpyDeviceToDevice, s)) ||
-            !ok(potrs_inv_t(f.W, N, NB, f.Z, Xt, nrhs, ldt, T, s))) {
+            !ok(potrs_inv_t(LW, N, NB, LZ, lp32, Xt, nrhs, ldt, T, s))) {
             rc = OOCDAG_EHIP;
             break;
         }
@@ -673,7 +713,7 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
                 rc = 1;  // stalled
                 break;
             }
-            if (!ok(potrs_inv_t(f.W, N, NB, f.Z, Rt, nrhs, ldt, T, s))) {
+            if (!ok(potrs_inv_t(LW, N, NB, LZ, lp32, Rt, nrhs, ldt, T, s))) {
                 rc = OOCDAG_EHIP;
                 break;
             }

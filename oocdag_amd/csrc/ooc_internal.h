@@ -53,6 +53,9 @@ struct Gemm {
     int M, N, K;
     double alpha, beta;
     int lower;
+    // B.p points to float elements (same element offsets), widened to
+    // double on load; GEMM_RHS / GEMM_RHS_TB only
+    int b32 = 0;
 };
 
 // Kernel families (distinct kernel symbols, so profiles separate them).
@@ -178,6 +181,9 @@ hipError_t resid_upper_block(const double *U, long long ldu, int N, int NB, int 
 struct MpFactor {
     const double *A = nullptr;
     double *W = nullptr, *Z = nullptr, *dinv = nullptr;
+    // FP32 copies of W and Z for the correction solves of a low-precision
+    // factor (precision > 0)
+    float *W32 = nullptr, *Z32 = nullptr;
     int N = 0, NB = 0, precision = 0, info = 0;
     double Anorm = 0, factor_seconds = 0;
     size_t bytes = 0;

@@ -425,6 +425,7 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
         if (r == 1 && ce->f.precision != 0) {
             MpFactor f64 = ce->f;
             f64.W = f64.Z = f64.dinv = nullptr;
+            f64.W32 = f64.Z32 = nullptr;
             rc = mp_factor(f64, ce->A, n, nb, 0, s, s1);
             if (rc == OOCDAG_OK) {
                 mp_release(ce->f, s);
