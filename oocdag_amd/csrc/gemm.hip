@@ -655,15 +655,23 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_dtv(KArgs a) {
 // ---- few right-hand sides -------------------------------------------------
 // C (M x N) = beta*C + alpha * A * B^T for the solves' right-hand-side
 // panels: A and C plain, B a packed (or plain) operand streamed once.  One
-// wave owns 16 rows m x 16 columns n; A and B go from global memory straight
-// into v_mfma_f64_16x16x4_f64 operand registers (A lane (m = l%16, k = l/16),
-// B lane (n = l%16, k = l/16)), no LDS and no barriers, and the K loop is
-// unrolled so each wave keeps many loads in flight: the kernel streams B
-// (the factor's tile column or tile row) at memory speed.
+// wave owns 16 rows m x 16 (or 32) columns n; A and B go from global memory
+// straight into MFMA operand registers (A lane (m = l%16, k = l/16), B lane
+// (n = l%16, k = l/16)), no LDS and no barriers, and the K loop is unrolled
+// so each wave keeps many loads in flight: the kernel streams B (the
+// factor's tile column or tile row) at memory speed.
+//
+// Element type T of A, B and C: double (v_mfma_f64_16x16x4_f64, result
+// element r of a lane at row lane/16 + 4r) or float for the correction
+// solves of a low-precision factor (v_mfma_f32_16x16x4_f32, same operand
+// layout, result rows as probed on the device: rowmap 0: 4*(lane/16) + r,
+// 1: lane/16 + 4r).  FP32 halves the bytes of the factor and of the
+// right-hand-side operand that every wave re-reads.
 struct RArgs {
     Opnd A, B, C;
     int M, N, K;
     double alpha, beta;
+    int rowmap;
 };
 
 __device__ __forceinline__ long long elem_off(const Opnd &o, int gr, int gc) {
@@ -672,27 +680,57 @@ __device__ __forceinline__ long long elem_off(const Opnd &o, int gr, int gc) {
     return tix(ti, tk, o.ym, o.tri) * o.te + (gr - ti * o.ts) + (long long)(gc - tk * o.ts) * o.ld;
 }
 
-// B operand elements are double, or float (a factor kept in FP32 for the
-// refinement's correction solves) widened to double on load: the MFMAs and
-// the accumulation stay FP64.
-template <typename BT> __device__ __forceinline__ const BT *bptr(const Opnd &B, long long off) {
-    return reinterpret_cast<const BT *>(B.p) + off;
+typedef float f4r __attribute__((ext_vector_type(4)));
+typedef float f2r __attribute__((ext_vector_type(2)));
+
+template <typename T> struct Rk;
+template <> struct Rk<double> {
+    typedef d4 acc;
+    typedef d2 pair;
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r, int) { return (lane >> 4) + 4 * r; }
+    // four consecutive elements (16-byte aligned) as two pairs
+    static __device__ __forceinline__ void ld4(const double *q, pair &lo, pair &hi) {
+        lo = *reinterpret_cast<const d2 *>(q);
+        hi = *reinterpret_cast<const d2 *>(q + 2);
+    }
+};
+template <> struct Rk<float> {
+    typedef f4r acc;
+    typedef f2r pair;
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r, int rowmap) {
+        return rowmap ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+    }
+    static __device__ __forceinline__ void ld4(const float *q, pair &lo, pair &hi) {
+        const f4r v = *reinterpret_cast<const f4r *>(q);
+        lo = f2r{v.x, v.y};
+        hi = f2r{v.z, v.w};
+    }
+};
+
+template <typename T> __device__ __forceinline__ T *optr(const Opnd &o, long long off) {
+    return reinterpret_cast<T *>(o.p) + off;
 }
-// two consecutive elements (16-byte aligned for double, 8 for float)
-__device__ __forceinline__ d2 ld2(const double *q) { return *reinterpret_cast<const d2 *>(q); }
-__device__ __forceinline__ d2 ld2(const float *q) {
-    const float2 v = *reinterpret_cast<const float2 *>(q);
-    return d2{(double)v.x, (double)v.y};
-}
-// four consecutive elements as two pairs (16-byte aligned)
-__device__ __forceinline__ void ld4(const double *q, d2 &lo, d2 &hi) {
-    lo = *reinterpret_cast<const d2 *>(q);
-    hi = *reinterpret_cast<const d2 *>(q + 2);
-}
-__device__ __forceinline__ void ld4(const float *q, d2 &lo, d2 &hi) {
-    const float4 v = *reinterpret_cast<const float4 *>(q);
-    lo = d2{(double)v.x, (double)v.y};
-    hi = d2{(double)v.z, (double)v.w};
+
+// C[rows m0 + row(r), column n] = alpha * acc[r] + beta * C (rows < M)
+template <typename T>
+__device__ __forceinline__ void rhs_store(const RArgs &a, int m0, int n, int lane,
+                                          const typename Rk<T>::acc &acc) {
+    const T alpha = (T)a.alpha, beta = (T)a.beta;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int m = m0 + Rk<T>::row(lane, r, a.rowmap);
+        if (m >= a.M) continue;
+        T *c = optr<T>(a.C, elem_off(a.C, a.C.r0 + m, a.C.c0 + n));
+        T v = alpha * acc[r];
+        if (a.beta != 0.0) v += beta * *c;
+        *c = v;
+    }
 }
 
 // K may span several tiles of a packed B (the solves' bulk updates take 8
@@ -704,29 +742,29 @@ __device__ __forceinline__ int rhs_chunk(const Opnd &B, int gk, int left) {
 
 // acc += A(m, 4t + kl) * B(n, 4t + kl) over t < nk: lane (il, kl) of A at
 // pa + t*astep and of B at pb + t*bstep; two batches of U loads in flight.
-template <int U, typename BT>
-__device__ __forceinline__ void rhs_run(const double *pa, long long astep, const BT *pb,
-                                        long long bstep, int nk, d4 &acc) {
+template <int U, typename T>
+__device__ __forceinline__ void rhs_run(const T *pa, long long astep, const T *pb,
+                                        long long bstep, int nk, typename Rk<T>::acc &acc) {
     int t = 0;
     if (nk >= U) {
-        double av[U], bv[U];
+        T av[U], bv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             av[u] = pa[u * astep];
-            bv[u] = (double)pb[u * bstep];
+            bv[u] = pb[u * bstep];
         }
         for (; t + U <= nk; t += U) {
-            double an[U], bq[U];
+            T an[U], bq[U];
             const bool more = t + 2 * U <= nk;
             if (more) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     an[u] = pa[(t + U + u) * astep];
-                    bq[u] = (double)pb[(t + U + u) * bstep];
+                    bq[u] = pb[(t + U + u) * bstep];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+            for (int u = 0; u < U; ++u) acc = Rk<T>::mma(av[u], bv[u], acc);
             if (more) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -736,45 +774,45 @@ __device__ __forceinline__ void rhs_run(const double *pa, long long astep, const
             }
         }
     }
-    for (; t < nk; ++t)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t * astep], (double)pb[t * bstep], acc, 0, 0, 0);
+    for (; t < nk; ++t) acc = Rk<T>::mma(pa[t * astep], pb[t * bstep], acc);
 }
 
 // Transposed B with K a multiple of 16: K runs down a stored column, so a
 // lane loads 4 consecutive K values of its column (rows 4kq..4kq+3 of each
-// 16-wide K group, two 16-byte loads) and MFMA j of the group takes element
-// j: MFMA j covers K = 16g + 4kq + j over the 4 lane groups kq.  A is loaded
-// to match (qa = A[m][4kq], element j of group g at qa + (16g + j)*lda).
-// Two batches of 4 groups in flight.
-template <typename BT>
-__device__ __forceinline__ void rhs_run_tb16(const double *qa, long long lda, const BT *qb,
-                                             int ng, d4 &acc) {
+// 16-wide K group, 16-byte loads) and MFMA j of the group takes element j:
+// MFMA j covers K = 16g + 4kq + j over the 4 lane groups kq.  A is loaded to
+// match (qa = A[m][4kq], element j of group g at qa + (16g + j)*lda).  Two
+// batches of 4 groups in flight.
+template <typename T>
+__device__ __forceinline__ void rhs_run_tb16(const T *qa, long long lda, const T *qb, int ng,
+                                             typename Rk<T>::acc &acc) {
+    typedef typename Rk<T>::pair P2;
     constexpr int G = 4;
-    double av[G][4];
-    d2 bv[G][2];
-    auto load = [&](int g0, double (&x)[G][4], d2 (&y)[G][2]) {
+    T av[G][4];
+    P2 bv[G][2];
+    auto load = [&](int g0, T (&x)[G][4], P2 (&y)[G][2]) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const long long k = 16LL * (g0 + g);
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[g][j] = qa[(k + j) * lda];
-            ld4(qb + k, y[g][0], y[g][1]);
+            Rk<T>::ld4(qb + k, y[g][0], y[g][1]);
         }
     };
     int g0 = 0;
     if (ng >= G) {
         load(0, av, bv);
         for (; g0 + G <= ng; g0 += G) {
-            double an[G][4];
-            d2 bn[G][2];
+            T an[G][4];
+            P2 bn[G][2];
             const bool more = g0 + 2 * G <= ng;
             if (more) load(g0 + G, an, bn);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][0], bv[g][0].x, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][1], bv[g][0].y, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][2], bv[g][1].x, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][3], bv[g][1].y, acc, 0, 0, 0);
+                acc = Rk<T>::mma(av[g][0], bv[g][0].x, acc);
+                acc = Rk<T>::mma(av[g][1], bv[g][0].y, acc);
+                acc = Rk<T>::mma(av[g][2], bv[g][1].x, acc);
+                acc = Rk<T>::mma(av[g][3], bv[g][1].y, acc);
             }
             if (more) {
 #pragma unroll
@@ -790,25 +828,26 @@ __device__ __forceinline__ void rhs_run_tb16(const double *qa, long long lda, co
     // remaining groups (fewer than G), one at a time
     for (; g0 < ng; ++g0) {
         const long long k = 16LL * g0;
-        d2 b0, b1;
-        ld4(qb + k, b0, b1);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[k * lda], b0.x, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 1) * lda], b0.y, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 2) * lda], b1.x, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[(k + 3) * lda], b1.y, acc, 0, 0, 0);
+        P2 b0, b1;
+        Rk<T>::ld4(qb + k, b0, b1);
+        acc = Rk<T>::mma(qa[k * lda], b0.x, acc);
+        acc = Rk<T>::mma(qa[(k + 1) * lda], b0.y, acc);
+        acc = Rk<T>::mma(qa[(k + 2) * lda], b1.x, acc);
+        acc = Rk<T>::mma(qa[(k + 3) * lda], b1.y, acc);
     }
 }
 
 // B not transposed, N a multiple of 32: one wave owns 16 rows x 32 columns.
-// Lane (il, kl) loads B rows n0 + 2il and n0 + 2il + 1 of K column kl (16
-// contiguous bytes) and feeds them to two MFMAs, one over the wave's even
-// output columns and one over its odd ones, so one A load serves two MFMAs
-// and a B load instruction covers 4 x 256 contiguous bytes.  One-wave
-// workgroups: the dispatcher hands out work in units of one wave, so a grid
-// a little over a multiple of the CU count does not leave a few CUs a whole
-// extra round.
-template <typename BT>
+// Lane (il, kl) loads B rows n0 + 2il and n0 + 2il + 1 of K column kl (one
+// load of two contiguous elements) and feeds them to two MFMAs, one over the
+// wave's even output columns and one over its odd ones, so one A load serves
+// two MFMAs.  One-wave workgroups: the dispatcher hands out work in units of
+// one wave, so a grid a little over a multiple of the CU count does not
+// leave a few CUs a whole extra round.
+template <typename T>
 __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
+    typedef typename Rk<T>::pair P2;
+    typedef typename Rk<T>::acc Acc;
     constexpr int U = 8;   // K steps per batch; two batches in flight
     const int lane = threadIdx.x;
     const int n0 = blockIdx.x * 32;
@@ -817,36 +856,36 @@ __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
     const int il = lane & 15, kl = lane >> 4;
     const int mi = min(m0 + il, a.M - 1);
     const long long astep = 4 * a.A.ld, bstep = 4 * a.B.ld;
-    d4 ev = {0.0, 0.0, 0.0, 0.0}, od = {0.0, 0.0, 0.0, 0.0};
+    Acc ev = {0, 0, 0, 0}, od = {0, 0, 0, 0};
     for (int kd = 0; kd < a.K;) {
         const int len = rhs_chunk(a.B, a.B.c0 + kd, a.K - kd);
-        const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + kl);
-        const BT *pb = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + n0 + 2 * il, a.B.c0 + kd + kl));
+        const T *pa = optr<T>(a.A, elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + kl));
+        const T *pb = optr<T>(a.B, elem_off(a.B, a.B.r0 + n0 + 2 * il, a.B.c0 + kd + kl));
         const int nk = len / 4;
         int t = 0;
         if (nk >= U) {
-            double av[U];
-            d2 bv[U];
+            T av[U];
+            P2 bv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 av[u] = pa[u * astep];
-                bv[u] = ld2(pb + u * bstep);
+                bv[u] = *reinterpret_cast<const P2 *>(pb + u * bstep);
             }
             for (; t + U <= nk; t += U) {
-                double an[U];
-                d2 bn[U];
+                T an[U];
+                P2 bn[U];
                 const bool more = t + 2 * U <= nk;
                 if (more) {
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         an[u] = pa[(t + U + u) * astep];
-                        bn[u] = ld2(pb + (t + U + u) * bstep);
+                        bn[u] = *reinterpret_cast<const P2 *>(pb + (t + U + u) * bstep);
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    ev = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u].x, ev, 0, 0, 0);
-                    od = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u].y, od, 0, 0, 0);
+                    ev = Rk<T>::mma(av[u], bv[u].x, ev);
+                    od = Rk<T>::mma(av[u], bv[u].y, od);
                 }
                 if (more) {
 #pragma unroll
@@ -858,35 +897,25 @@ __global__ __launch_bounds__(64) void k_rhs_w2(RArgs a) {
             }
         }
         for (; t < nk; ++t) {
-            const double x = pa[t * astep];
-            const d2 y = ld2(pb + t * bstep);
-            ev = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.x, ev, 0, 0, 0);
-            od = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y.y, od, 0, 0, 0);
+            const T x = pa[t * astep];
+            const P2 y = *reinterpret_cast<const P2 *>(pb + t * bstep);
+            ev = Rk<T>::mma(x, y.x, ev);
+            od = Rk<T>::mma(x, y.y, od);
         }
         kd += len;
     }
-    // element r of a lane: row m0 + kl + 4r, columns n0 + 2il (ev), + 1 (od)
-    double *ce = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 2 * il);
-    double *co = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 2 * il + 1);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        if (m0 + kl + 4 * r >= a.M) break;
-        double v = a.alpha * ev[r], w = a.alpha * od[r];
-        if (a.beta != 0.0) {
-            v += a.beta * ce[4 * r];
-            w += a.beta * co[4 * r];
-        }
-        ce[4 * r] = v;
-        co[4 * r] = w;
-    }
+    rhs_store<T>(a, m0, n0 + 2 * il, lane, ev);
+    rhs_store<T>(a, m0, n0 + 2 * il + 1, lane, od);
 }
 
 // B transposed (K down a stored column), N a multiple of 32, K chunks of
 // whole 16-wide groups: one wave owns 16 rows x 32 columns, the two 16-column
 // halves taking the same A registers (rhs_run_tb16's K order, 4 consecutive
 // K per lane), so A costs one load per two MFMAs.  One-wave workgroups.
-template <typename BT>
+template <typename T>
 __global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
+    typedef typename Rk<T>::pair P2;
+    typedef typename Rk<T>::acc Acc;
     constexpr int G = 2;   // 16-wide K groups per batch; two batches in flight
     const int lane = threadIdx.x;
     const int n0 = blockIdx.x * 32;
@@ -895,41 +924,41 @@ __global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
     const int il = lane & 15, kl = lane >> 4;
     const int mi = min(m0 + il, a.M - 1);
     const long long lda = a.A.ld;
-    d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+    Acc c0 = {0, 0, 0, 0}, c1 = {0, 0, 0, 0};
     for (int kd = 0; kd < a.K;) {
         const int len = rhs_chunk(a.B, a.B.r0 + kd, a.K - kd);
-        const double *qa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + 4 * kl);
-        const BT *qb0 = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + il));
-        const BT *qb1 = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + 16 + il));
+        const T *qa = optr<T>(a.A, elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + 4 * kl));
+        const T *qb0 = optr<T>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + il));
+        const T *qb1 = optr<T>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + n0 + 16 + il));
         const int ng = len / 16;
-        double av[G][4];
-        d2 bv[G][4];
-        auto load = [&](int g0, double (&x)[G][4], d2 (&y)[G][4]) {
+        T av[G][4];
+        P2 bv[G][4];
+        auto load = [&](int g0, T (&x)[G][4], P2 (&y)[G][4]) {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const long long k = 16LL * (g0 + g);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) x[g][j] = qa[(k + j) * lda];
-                ld4(qb0 + k, y[g][0], y[g][1]);
-                ld4(qb1 + k, y[g][2], y[g][3]);
+                Rk<T>::ld4(qb0 + k, y[g][0], y[g][1]);
+                Rk<T>::ld4(qb1 + k, y[g][2], y[g][3]);
             }
         };
-        auto mma = [&](const double (&x)[4], const d2 (&y)[4]) {
-            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[0], y[0].x, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[0], y[2].x, c1, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[1], y[0].y, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[1], y[2].y, c1, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[2], y[1].x, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[2], y[3].x, c1, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[3], y[1].y, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[3], y[3].y, c1, 0, 0, 0);
+        auto mma = [&](const T (&x)[4], const P2 (&y)[4]) {
+            c0 = Rk<T>::mma(x[0], y[0].x, c0);
+            c1 = Rk<T>::mma(x[0], y[2].x, c1);
+            c0 = Rk<T>::mma(x[1], y[0].y, c0);
+            c1 = Rk<T>::mma(x[1], y[2].y, c1);
+            c0 = Rk<T>::mma(x[2], y[1].x, c0);
+            c1 = Rk<T>::mma(x[2], y[3].x, c1);
+            c0 = Rk<T>::mma(x[3], y[1].y, c0);
+            c1 = Rk<T>::mma(x[3], y[3].y, c1);
         };
         int g0 = 0;
         if (ng >= G) {
             load(0, av, bv);
             for (; g0 + G <= ng; g0 += G) {
-                double an[G][4];
-                d2 bn[G][4];
+                T an[G][4];
+                P2 bn[G][4];
                 const bool more = g0 + 2 * G <= ng;
                 if (more) load(g0 + G, an, bn);
 #pragma unroll
@@ -947,41 +976,29 @@ __global__ __launch_bounds__(64) void k_rhs_tw2(RArgs a) {
             }
         }
         for (; g0 < ng; ++g0) {
-            double x[1][4];
-            d2 y[1][4];
+            T x[1][4];
+            P2 y[1][4];
             const long long k = 16LL * g0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[0][j] = qa[(k + j) * lda];
-            ld4(qb0 + k, y[0][0], y[0][1]);
-            ld4(qb1 + k, y[0][2], y[0][3]);
+            Rk<T>::ld4(qb0 + k, y[0][0], y[0][1]);
+            Rk<T>::ld4(qb1 + k, y[0][2], y[0][3]);
             mma(x[0], y[0]);
         }
         kd += len;
     }
-    // element r of a lane: row m0 + kl + 4r, columns n0 + il (c0), + 16 (c1)
-    double *ca = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + il);
-    double *cb = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + 16 + il);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        if (m0 + kl + 4 * r >= a.M) break;
-        double v = a.alpha * c0[r], w = a.alpha * c1[r];
-        if (a.beta != 0.0) {
-            v += a.beta * ca[4 * r];
-            w += a.beta * cb[4 * r];
-        }
-        ca[4 * r] = v;
-        cb[4 * r] = w;
-    }
+    rhs_store<T>(a, m0, n0 + il, lane, c0);
+    rhs_store<T>(a, m0, n0 + 16 + il, lane, c1);
 }
 
-//
-// KS = 4 (split K): the block's four waves share one 16 x 16 output and take
-// a quarter of K each, reduced through LDS in a fixed order.  Each wave's
-// serial MFMA and load chain is a quarter as long, which is what bounds the
-// small launches (a diagonal-tile product: 8 blocks) and a large launch's
-// tail.
-template <int TB, int KS, typename BT>
+// The general form: one wave per 16 x 16 output.  KS = 4 (split K): the
+// block's four waves share one 16 x 16 output and take a quarter of K each,
+// reduced through LDS in a fixed order.  Each wave's serial MFMA and load
+// chain is a quarter as long, which is what bounds the small launches (a
+// diagonal-tile product: 8 blocks).
+template <int TB, int KS, typename T>
 __global__ __launch_bounds__(256) void k_rhs(RArgs a) {
+    typedef typename Rk<T>::acc Acc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n0 = (KS == 1 ? blockIdx.x * 4 + wave : blockIdx.x) * 16;
     const int m0 = blockIdx.y * 16;
@@ -996,52 +1013,44 @@ __global__ __launch_bounds__(256) void k_rhs(RArgs a) {
     // lanes past M or N read a valid row/column instead: they only feed
     // output rows/columns that are never stored
     const int mi = min(m0 + il, a.M - 1), ni = min(n0 + il, a.N - 1);
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    Acc acc = {0, 0, 0, 0};
     for (int kd = 0; kd < a.K;) {
         const int len = rhs_chunk(a.B, (TB == 0 ? a.B.c0 : a.B.r0) + kd, a.K - kd);
         if (TB == 1 && (len & 15) == 0 && len >= 128) {
-            const double *qa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + 4 * kl);
-            const BT *qb = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + ni));
-            rhs_run_tb16(qa, a.A.ld, qb, len / 16, acc);
+            const T *qa = optr<T>(a.A, elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + 4 * kl));
+            const T *qb = optr<T>(a.B, elem_off(a.B, a.B.r0 + kd + 4 * kl, a.B.c0 + ni));
+            rhs_run_tb16<T>(qa, a.A.ld, qb, len / 16, acc);
         } else {
-            const double *pa = a.A.p + elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + kl);
-            const BT *pb;
+            const T *pa = optr<T>(a.A, elem_off(a.A, a.A.r0 + mi, a.A.c0 + kd + kl));
+            const T *pb;
             long long bstep;
             if (TB == 0) {
-                pb = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + ni, a.B.c0 + kd + kl));
+                pb = optr<T>(a.B, elem_off(a.B, a.B.r0 + ni, a.B.c0 + kd + kl));
                 bstep = 4 * a.B.ld;
             } else {
-                pb = bptr<BT>(a.B, elem_off(a.B, a.B.r0 + kd + kl, a.B.c0 + ni));
+                pb = optr<T>(a.B, elem_off(a.B, a.B.r0 + kd + kl, a.B.c0 + ni));
                 bstep = 4;
             }
-            rhs_run<16>(pa, 4 * a.A.ld, pb, bstep, len / 4, acc);
+            rhs_run<16, T>(pa, 4 * a.A.ld, pb, bstep, len / 4, acc);
         }
         kd += len;
     }
     if constexpr (KS > 1) {
-        __shared__ d4 part[KS - 1][64];
+        __shared__ Acc part[KS - 1][64];
         if (wave > 0) part[wave - 1][lane] = acc;
         __syncthreads();
         if (wave > 0) return;
 #pragma unroll
         for (int w = 0; w < KS - 1; ++w) {
-            const d4 q = part[w][lane];
+            const Acc q = part[w][lane];
             acc[0] += q[0];
             acc[1] += q[1];
             acc[2] += q[2];
             acc[3] += q[3];
         }
     }
-    // element r of a lane sits at row lane/16 + 4r, column lane%16
     if (n0 + il >= a.N) return;
-    double *c = a.C.p + elem_off(a.C, a.C.r0 + m0 + kl, a.C.c0 + n0 + il);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        if (m0 + kl + 4 * r >= a.M) break;
-        double v = a.alpha * acc[r];
-        if (a.beta != 0.0) v += a.beta * c[4 * r];
-        c[4 * r] = v;
-    }
+    rhs_store<T>(a, m0, n0 + il, lane, acc);
 }
 
 // The block's rows of o (starting at a multiple of X past o.r0) stay in one
@@ -1100,7 +1109,7 @@ hipError_t launch_one(const Gemm &g, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename BT>
+template <typename T>
 void launch_rhs_kernels(const Gemm &g, const RArgs &a, int tb, hipStream_t s) {
     const Opnd &B = g.B;
     // split K when the unsplit grid would not give every CU a block (a
@@ -1110,15 +1119,15 @@ void launch_rhs_kernels(const Gemm &g, const RArgs &a, int tb, hipStream_t s) {
     if (blocks1 < 256 && g.K % 64 == 0 && g.K >= 256) {
         dim3 grid((unsigned)((g.N + 15) / 16), (unsigned)((g.M + 15) / 16));
         if (tb)
-            hipLaunchKernelGGL((k_rhs<1, 4, BT>), grid, dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_rhs<1, 4, T>), grid, dim3(256), 0, s, a);
         else
-            hipLaunchKernelGGL((k_rhs<0, 4, BT>), grid, dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_rhs<0, 4, T>), grid, dim3(256), 0, s, a);
     } else if (!tb && g.N % 32 == 0 && (B.ts > 0 ? B.ts % 32 == 0 && B.r0 % 32 == 0
                                                   : B.ld % 2 == 0 && B.r0 % 2 == 0) &&
                ((uintptr_t)B.p & 15) == 0) {
         // 32-column groups inside one tile, aligned pairs of B rows
         dim3 grid((unsigned)(g.N / 32), (unsigned)((g.M + 15) / 16));
-        hipLaunchKernelGGL(k_rhs_w2<BT>, grid, dim3(64), 0, s, a);
+        hipLaunchKernelGGL(k_rhs_w2<T>, grid, dim3(64), 0, s, a);
     } else if (tb && g.N % 32 == 0 && g.K % 16 == 0 &&
                (B.ts > 0 ? B.ts % 32 == 0 && B.c0 % 32 == 0 && B.r0 % 16 == 0
                          : B.ld % 4 == 0 && B.r0 % 4 == 0) &&
@@ -1126,13 +1135,13 @@ void launch_rhs_kernels(const Gemm &g, const RArgs &a, int tb, hipStream_t s) {
         // 32-column groups inside one tile, K chunks of whole 16-wide groups
         // starting on 16-byte boundaries
         dim3 grid((unsigned)(g.N / 32), (unsigned)((g.M + 15) / 16));
-        hipLaunchKernelGGL(k_rhs_tw2<BT>, grid, dim3(64), 0, s, a);
+        hipLaunchKernelGGL(k_rhs_tw2<T>, grid, dim3(64), 0, s, a);
     } else {
         dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 15) / 16));
         if (tb)
-            hipLaunchKernelGGL((k_rhs<1, 1, BT>), grid, dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_rhs<1, 1, T>), grid, dim3(256), 0, s, a);
         else
-            hipLaunchKernelGGL((k_rhs<0, 1, BT>), grid, dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_rhs<0, 1, T>), grid, dim3(256), 0, s, a);
     }
 }
 
@@ -1146,9 +1155,10 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
         if (!tb && ((B.r0 % 16) || (B.c0 % 4))) return hipErrorInvalidValue;
         if (tb && ((B.c0 % 16) || (B.r0 % 4))) return hipErrorInvalidValue;
     }
-    // float B: 4 consecutive elements of a stored column on 16-byte
-    // boundaries (the transposed loads), pairs on 8-byte boundaries
-    if (g.b32 && ((B.ts <= 0 && (B.ld % 4 || B.r0 % 4)) || ((uintptr_t)B.p & 15)))
+    // the transposed loads take 4 consecutive elements of a stored column
+    // on 16-byte boundaries (8 bytes for the pairs of the untransposed
+    // form), which for float needs a plain B's ld and r0 multiples of 4
+    if (g.f32 && ((B.ts <= 0 && (B.ld % 4 || B.r0 % 4)) || (g.rowmap & ~1)))
         return hipErrorInvalidValue;
     RArgs a;
     a.A = g.A;
@@ -1159,7 +1169,8 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
     a.K = g.K;
     a.alpha = g.alpha;
     a.beta = g.beta;
-    if (g.b32)
+    a.rowmap = g.rowmap;
+    if (g.f32)
         launch_rhs_kernels<float>(g, a, tb, s);
     else
         launch_rhs_kernels<double>(g, a, tb, s);
@@ -1170,7 +1181,7 @@ hipError_t launch_rhs(const Gemm &g, int tb, hipStream_t s) {
 
 hipError_t launch_gemm(const Gemm &g, GemmKind kind, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
-    if (g.b32 && kind != GEMM_RHS && kind != GEMM_RHS_TB) return hipErrorInvalidValue;
+    if (g.f32 && kind != GEMM_RHS && kind != GEMM_RHS_TB) return hipErrorInvalidValue;
     // Operand constraints of the streaming loads (16-byte row-pair loads,
     // 16-wide K steps that never straddle a tile).
     const Opnd *ops[3] = {&g.A, &g.B, &g.C};

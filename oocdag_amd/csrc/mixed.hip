@@ -144,6 +144,33 @@ __global__ void k_axpy(double *y, const double *x, long long n, double a) {
         y[i] += a * x[i];
 }
 
+// Right-hand sides (nrhs x Np, ld ldt, element (j, i) at j + i*ldt) to and
+// from FP32 for the correction solves.  Column j is scaled by 2^-e, e the
+// binary exponent of its largest magnitude cmax[j] (exact scaling), so the
+// narrowed values neither overflow nor lose range to subnormals; rows
+// j >= nrhs (padding) stay unscaled.
+__device__ __forceinline__ int col_exp(const double *cmax, int j, int nrhs) {
+    const double m = j < nrhs ? cmax[j] : 0.0;
+    return (m > 0.0 && m < INFINITY) ? ilogb(m) : 0;
+}
+
+__global__ void k_to_f32_cols(const double *x, float *y, long long n, int ldt, int nrhs,
+                              const double *cmax) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = (float)ldexp(x[i], -col_exp(cmax, (int)(i % ldt), nrhs));
+}
+
+// y = beta*y + 2^e * x (beta 0 or 1)
+__global__ void k_from_f32_cols(double *y, const float *x, long long n, int ldt, int nrhs,
+                                const double *cmax, double beta) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double v = ldexp((double)x[i], col_exp(cmax, (int)(i % ldt), nrhs));
+        y[i] = beta != 0.0 ? y[i] + v : v;
+    }
+}
+
 double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
         .count();
@@ -356,10 +383,11 @@ hipError_t tile_inverses(const double *dY, int N, int NB, const double *dinv_all
 // tiles; after it, one update of everything beyond with K = W*NB (the
 // factor's W tile columns, or tile rows, at once).
 //
-// f32 != 0: dY and Z hold float elements (the FP32 copy of a low-precision
-// factor), widened to double on load.
-hipError_t potrs_inv_t(const void *dY, int N, int NB, const void *Z, int f32, double *Xt,
-                       int nrhs, long long ldx, double *Yt, hipStream_t s) {
+// f32 != 0: the whole solve in FP32 -- dY and Z the FP32 copy of a
+// low-precision factor, Xt and Yt float, FP32 MFMAs (result rows per
+// rowmap) -- for the correction solves of iterative refinement.
+hipError_t potrs_inv_t(const void *dY, int N, int NB, const void *Z, int f32, int rowmap,
+                       void *Xt, int nrhs, long long ldx, void *Yt, hipStream_t s) {
     const int bb = (N + NB - 1) / NB, W = solve_step_width(NB);
     const Opnd Lp = panel((double *)dY, NB, bb, 1);
     const size_t per = (size_t)NB * NB;
@@ -368,26 +396,28 @@ hipError_t potrs_inv_t(const void *dY, int N, int NB, const void *Z, int f32, do
     const GemmKind nt = nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL;
     const GemmKind tt = nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB;
     hipError_t e;
-    auto diag = [&](const double *src, double *dst, int k, GemmKind kind) -> hipError_t {
+    auto diag = [&](const void *src, void *dst, int k, GemmKind kind) -> hipError_t {
         Gemm g;
         g.A = plain((double *)src, ldx, 0, k * NB);
         g.B = plain(f32 ? (double *)((const float *)Z + k * per) : (double *)Z + k * per, NB);
-        g.C = plain(dst, ldx, 0, k * NB);
+        g.C = plain((double *)dst, ldx, 0, k * NB);
         g.M = nrhs; g.N = NB; g.K = NB;
         g.alpha = 1.0; g.beta = 0.0; g.lower = 0;
-        g.b32 = f32;
+        g.f32 = f32;
+        g.rowmap = rowmap;
         return launch_gemm(g, kind, s);
     };
     // C[:, c0 .. c0+n*NB) -= src[:, a0 .. a0+k*NB) * op(L block at (br, bc))
-    auto update = [&](const double *src, int a0, double *dst, int c0, int br, int bc, int n,
+    auto update = [&](const void *src, int a0, void *dst, int c0, int br, int bc, int n,
                       int k, GemmKind kind) -> hipError_t {
         Gemm g;
         g.A = plain((double *)src, ldx, 0, a0 * NB);
         g.B = Lp; g.B.r0 = br * NB; g.B.c0 = bc * NB;
-        g.C = plain(dst, ldx, 0, c0 * NB);
+        g.C = plain((double *)dst, ldx, 0, c0 * NB);
         g.M = nrhs; g.N = n * NB; g.K = k * NB;
         g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
-        g.b32 = f32;
+        g.f32 = f32;
+        g.rowmap = rowmap;
         return launch_gemm(g, kind, s);
     };
     for (int s0 = 0; s0 < bb; s0 += W) {   // Y^T_k = B^T_k L_kk^-T = B^T_k Z_k
@@ -673,10 +703,38 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
     int rc = OOCDAG_OK;
     int it = 0;
     double eta = INFINITY, prev = INFINITY;
-    // a low-precision factor's solves read its FP32 copy (few right-hand sides)
-    const int lp32 = f.precision > 0 && f.W32 && f.Z32 && nrhs <= 64;
-    const void *LW = lp32 ? (const void *)f.W32 : (const void *)f.W;
-    const void *LZ = lp32 ? (const void *)f.Z32 : (const void *)f.Z;
+    // a low-precision factor's solves run in FP32 on its FP32 copy (few
+    // right-hand sides): the factor's own error is far above FP32 rounding
+    int lp32 = f.precision > 0 && f.W32 && f.Z32 && nrhs <= 64;
+    const int rowmap = lp32 ? mfma_rowmap(0, s) : 0;
+    float *X32 = nullptr, *T32 = nullptr;
+    const size_t vb32 = (size_t)(ldt * Np) * 4;
+    if (rowmap < 0 || (lp32 && (!ok(salloc(&X32, vb32, s)) || !ok(salloc(&T32, vb32, s))))) {
+        for (void *p : {(void *)Bt, (void *)Xt, (void *)Rt, (void *)T, (void *)nrm, (void *)X32})
+            if (p) (void)hipFreeAsync(p, s);
+        return rowmap < 0 ? OOCDAG_EHIP : OOCDAG_ENOMEM;
+    }
+    const long long nel = ldt * Np;
+    // x (FP64) <- solve(r), or x += solve(r); cmax: r's column maxima
+    auto solve = [&](const double *r, double *x, const double *cmax, double beta) -> bool {
+        if (!lp32) {
+            if (beta == 0.0) {
+                if (x != r && !ok(hipMemcpyAsync(x, r, vb, hipMemcpyDeviceToDevice, s))) return false;
+                return ok(potrs_inv_t(f.W, N, NB, f.Z, 0, 0, x, nrhs, ldt, T, s));
+            }
+            // r is overwritten by its solution, then added to x
+            if (!ok(potrs_inv_t(f.W, N, NB, f.Z, 0, 0, (void *)r, nrhs, ldt, T, s))) return false;
+            hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(256), 0, s, x, r, nel, 1.0);
+            return ok(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_to_f32_cols, dim3(1024), dim3(256), 0, s, r, X32, nel, (int)ldt, nrhs, cmax);
+        if (!ok(hipGetLastError()) ||
+            !ok(potrs_inv_t(f.W32, N, NB, f.Z32, 1, rowmap, X32, nrhs, ldt, T32, s)))
+            return false;
+        hipLaunchKernelGGL(k_from_f32_cols, dim3(1024), dim3(256), 0, s, x, X32, nel, (int)ldt, nrhs,
+                           cmax, beta);
+        return ok(hipGetLastError());
+    };
     std::vector<double> bnorm(nrhs), hx(nrhs), hr(nrhs);
     do {
         if (!ok(hipMemsetAsync(Bt, 0, vb, s)) || !ok(launch_transpose(dB, ldb, Bt, ldt, N, nrhs, s))) {
@@ -685,8 +743,7 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
         }
         hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Bt, ldt, nrhs, N, nrm);
         if (!ok(hipMemcpyAsync(bnorm.data(), nrm, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
-            !ok(hipMemcpyAsync(Xt, Bt, vb, hipMemcpyDeviceToDevice, s)) ||
-            !ok(potrs_inv_t(LW, N, NB, LZ, lp32, Xt, nrhs, ldt, T, s))) {
+            !solve(Bt, Xt, nrm, 0.0)) {
             rc = OOCDAG_EHIP;
             break;
         }
@@ -713,11 +770,10 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
                 rc = 1;  // stalled
                 break;
             }
-            if (!ok(potrs_inv_t(LW, N, NB, LZ, lp32, Rt, nrhs, ldt, T, s))) {
+            if (!solve(Rt, Xt, nrm + ldt, 1.0)) {
                 rc = OOCDAG_EHIP;
                 break;
             }
-            hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(256), 0, s, Xt, Rt, ldt * Np, 1.0);
             ++it;
         }
         if (rc < 0) break;
@@ -725,6 +781,8 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
             rc = OOCDAG_EHIP;
     } while (0);
     for (double *p : {Bt, Xt, Rt, T, nrm})
+        if (p) (void)hipFreeAsync(p, s);
+    for (float *p : {X32, T32})
         if (p) (void)hipFreeAsync(p, s);
     if (iters) *iters = it;
     if (eta_out) *eta_out = eta;

@@ -53,9 +53,11 @@ struct Gemm {
     int M, N, K;
     double alpha, beta;
     int lower;
-    // B.p points to float elements (same element offsets), widened to
-    // double on load; GEMM_RHS / GEMM_RHS_TB only
-    int b32 = 0;
+    // GEMM_RHS / GEMM_RHS_TB only: A, B and C point to float elements (same
+    // element offsets) and the products run on FP32 MFMAs, whose result
+    // rows follow rowmap (mfma_rowmap(0, ...))
+    int f32 = 0;
+    int rowmap = 0;
 };
 
 // Kernel families (distinct kernel symbols, so profiles separate them).
