@@ -403,6 +403,9 @@ __global__ __launch_bounds__(NT, 2) void k_update_f32_dtv(LpArgs a) {
 constexpr int BK16 = 64;            // bf16 per K step
 constexpr int S16 = BK16 + 8;       // LDS row stride (bf16), rows are m
 
+// ROWMAP: the probed FP32 result layout of the bf16 MFMA (0: lane holds rows
+// 4*(lane/16) + r, four consecutive rows of one column; 1: rows lane/16 + 4r).
+template <int ROWMAP>
 __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
     __shared__ unsigned short As[2][BM * S16];
     __shared__ unsigned short Bs[2][BM * S16];
@@ -469,23 +472,54 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
         if (more) lstore(cur ^ 1);
         __syncthreads();
     }
+    // Epilogue, per 16-row block i: all 16 FP64 loads of the lane first (as
+    // 16-byte pairs for ROWMAP 0), then the updates and stores, so the
+    // loads' latencies overlap instead of alternating with stores the
+    // compiler cannot reorder them across.  Elements above the diagonal
+    // (diagonal blocks only) are written back unchanged.
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int mb16 = m0 + wm * 64 + i * 16;
         const float si = a.scale[mb16 / a.nb];
+        const int gr0 = a.r0 + mb16;
+        double *cp[4];
+        double old[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gc = a.r0 + n0 + wn * 64 + j * 16 + (lane & 15);
+            const int ti = gr0 / a.nb, tk = gc / a.nb;
+            cp[j] = a.C + tix(ti, tk, a.c_ym, 1) * per + (gr0 - ti * a.nb) +
+                    (long long)(gc - tk * a.nb) * a.nb;
+            if (ROWMAP == 0) {
+                const d2 *q = reinterpret_cast<const d2 *>(cp[j] + 4 * (lane >> 4));
+                const d2 lo = q[0], hi = q[1];
+                old[j][0] = lo.x;
+                old[j][1] = lo.y;
+                old[j][2] = hi.x;
+                old[j][3] = hi.y;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) old[j][r] = cp[j][(lane >> 4) + 4 * r];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int n = n0 + wn * 64 + j * 16 + (lane & 15);
             const double s = (double)si * (double)a.scale[n / a.nb];
-            const int gr0 = a.r0 + mb16, gc = a.r0 + n;
-            const int ti = gr0 / a.nb, tk = gc / a.nb;
-            double *c = a.C + tix(ti, tk, a.c_ym, 1) * per + (gr0 - ti * a.nb) +
-                        (long long)(gc - tk * a.nb) * a.nb;
+            const int gc = a.r0 + n;
+            double v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int rr = a.rowmap ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
-                if (gr0 + rr < gc) continue;
-                c[rr] -= s * (double)acc[i][j][r];
+                const int rr = ROWMAP == 0 ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r;
+                v[r] = gr0 + rr < gc ? old[j][r] : old[j][r] - s * (double)acc[i][j][r];
+            }
+            if (ROWMAP == 0) {
+                d2 *q = reinterpret_cast<d2 *>(cp[j] + 4 * (lane >> 4));
+                q[0] = d2{v[0], v[1]};
+                q[1] = d2{v[2], v[3]};
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cp[j][(lane >> 4) + 4 * r] = v[r];
             }
         }
     }
@@ -581,8 +615,10 @@ hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const
         hipLaunchKernelGGL((k_update_f32_dtv<8, 6, 1>), dim3((unsigned)n), dim3(NT), 0, s, a);
     else if (mode == 0)
         hipLaunchKernelGGL(k_update_f32, dim3((unsigned)n), dim3(NT), 0, s, a);
+    else if (rowmap == 0)
+        hipLaunchKernelGGL(k_update_bf16<0>, dim3((unsigned)n), dim3(NT), 0, s, a);
     else
-        hipLaunchKernelGGL(k_update_bf16, dim3((unsigned)n), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(k_update_bf16<1>, dim3((unsigned)n), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
