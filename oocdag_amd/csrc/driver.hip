@@ -634,9 +634,10 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
         // while the others are still being copied in (nothing to overlap the
         // copy of a whole first panel with otherwise)
         const int w0 = (p == 0 && Yn >= 4) ? std::min(8, Yn / 4) : 0;
-        // --- A2Y (oocquark.cpp:253-275): the panel's tile columns -> Y
+        // --- A2Y (oocquark.cpp:253-275): the panel's tile columns -> Y (a
+        // split first panel loads them one at a time below)
         if (p == 0) {
-            for (int jj = 0; jj < (w0 > 0 ? w0 : Yn); ++jj)
+            for (int jj = 0; jj < (w0 > 0 ? 0 : Yn); ++jj)
                 if ((rc = h2d_column(dY + (size_t)tile_index(jj, jj, Ym, 1) * nb2, (int)jb + jj,
                                      (int)jb + jj, nth, t_h2d, no_wait)))
                     return rc;
@@ -760,13 +761,37 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
                               on_column, &after_updates);
             if (rc) return rc;
         } else {
-            // columns [0, w0), then each later column's update from them
-            // (one K = w0*NB launch) as soon as that column is in HBM, then
-            // columns [w0, Yn): the trailing part of a packed panel is itself
-            // a packed panel of Ym - w0 tile rows at tile_index(w0, w0, Ym)
-            rc = factor_panel(dY, NB, Ym, w0, (int)jb * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
-                              on_column);
-            if (rc) return rc;
+            // columns [0, w0) left-looking, each as soon as it is in HBM:
+            // its update from the columns before it (one K = c*NB launch),
+            // then its blocked factorization; then each later column's
+            // update from columns [0, w0) (one K = w0*NB launch) on arrival;
+            // then columns [w0, Yn): the trailing part of a packed panel is
+            // itself a packed panel of Ym - w0 tile rows at
+            // tile_index(w0, w0, Ym)
+            const Opnd P0 = panel(dY, NB, Ym, 1);
+            for (int c = 0; c < w0; ++c) {
+                if ((rc = h2d_column(dY + (size_t)tile_index(c, c, Ym, 1) * nb2, (int)jb + c,
+                                     (int)jb + c, nth, t_h2d, no_wait)))
+                    return rc;
+                HIPTRY(hipEventRecord(ev_y, st.h2d));
+                HIPTRY(hipStreamWaitEvent(st.comp, ev_y, 0));
+                if (c > 0) {
+                    Gemm g;
+                    g.A = P0; g.A.r0 = c * NB; g.A.c0 = 0;
+                    g.B = g.A;
+                    g.C = P0; g.C.r0 = c * NB; g.C.c0 = c * NB;
+                    g.M = (Ym - c) * NB;
+                    g.N = NB;
+                    g.K = c * NB;
+                    g.alpha = -1.0;
+                    g.beta = 1.0;
+                    g.lower = 1;
+                    HIPTRY(launch_update(g, st.comp));
+                }
+                HIPTRY(factor_column(P0, c * NB, (Ym - c) * NB, NB, ((int)jb + c) * NB, sc.dinv,
+                                     sc.dinfo, st.comp));
+                if ((rc = on_column(c))) return rc;
+            }
             for (int c = w0; c < Yn; ++c) {
                 if ((rc = h2d_column(dY + (size_t)tile_index(c, c, Ym, 1) * nb2, (int)jb + c,
                                      (int)jb + c, nth, t_h2d, no_wait)))
