@@ -199,14 +199,22 @@ def bench_mpir(ooc, n: int, nb: int, nrhs: int, mus, precs) -> dict:
     return out
 
 
-def bench_batched(ooc, nsys: int, reps: int) -> dict:
-    """10000 independent SPD systems, n log-uniform in [32, 1024], A = G G^T/n
-    + 0.1 I, 1..8 N(0,1) right-hand sides; FP64 and FP32+IR; systems/s of the
-    graph-replayed persistent kernel and per-system backward error."""
+def batched_workload(nsys: int):
+    """Sizes n (log-uniform in [32, 1024]) and right-hand-side counts (1..8)
+    of the batched workload."""
     import numpy as np
     rng = np.random.default_rng(2024)
     n = np.exp(rng.uniform(np.log(32), np.log(1024), size=nsys)).astype(np.int32)
     r = rng.integers(1, 9, size=nsys).astype(np.int32)
+    return n, r
+
+
+def bench_batched(ooc, n, r, reps: int, cpu_rec=None) -> dict:
+    """10000 independent SPD systems, n log-uniform in [32, 1024], A = G G^T/n
+    + 0.1 I, 1..8 N(0,1) right-hand sides; FP64 and FP32+IR; systems/s of the
+    graph-replayed persistent kernel and per-system backward error."""
+    import numpy as np
+    nsys = len(n)
     bt = ooc.Batch(n, r)
     t0 = time.perf_counter()
     bt.generate(mu=0.1, seed=2024)
@@ -221,7 +229,63 @@ def bench_batched(ooc, nsys: int, reps: int) -> dict:
         log(f"[batched] {prec}: {st['systems_per_second']:.0f} systems/s ({st['seconds'] * 1e3:.1f} ms"
             f"/batch), variant T={st['variant_T']} RB={st['variant_RB']}, max berr "
             f"{st['max_backward_error']:.2e}, failed {st['failed']}, mean iters {st['mean_iterations']:.2f}")
+    if cpu_rec:
+        out["cpu_baseline"] = cpu_rec
+        out["speedup_vs_cpu"] = round(out["fp64"]["systems_per_second"] / cpu_rec["value"], 1)
     return out
+
+
+_CPU_SYSTEMS = []
+
+
+def _cpu_batched_share(args):
+    """One worker's share of the CPU baseline: systems k, k + w, k + 2w ...,
+    one BLAS thread; returns its own elapsed seconds."""
+    k, w = args
+    from scipy.linalg import cho_factor, cho_solve
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):
+        mine = _CPU_SYSTEMS[k::w]
+        t0 = time.perf_counter()
+        for A, B in mine:
+            cho_solve(cho_factor(A, lower=True, check_finite=False), B, check_finite=False)
+        return time.perf_counter() - t0
+
+
+def cpu_batched_baseline(n, r, sample: int = 2000, procs: int = 0, reps: int = 5) -> dict:
+    """CPU reference timing of the batched workload: LAPACK dpotrf + dpotrs
+    (scipy) over a sample of the batch's sizes and right-hand-side counts,
+    one BLAS thread per call, the systems spread over forked worker processes
+    (one per core; no exec, and run before the batched mode touches the
+    GPU).  Throughput = sample / the slowest worker's time, median of reps
+    passes.  The matrices are
+    drawn on the host from the same distribution (A = G G^T/n + 0.1 I), not
+    the device's values; factor + solve per system, generation outside the
+    clock."""
+    import multiprocessing as mp
+    import numpy as np
+    procs = procs or min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(7)
+    idx = rng.choice(len(n), size=min(sample, len(n)), replace=False)
+    _CPU_SYSTEMS.clear()
+    for i in idx:
+        k = int(n[i])
+        G = rng.standard_normal((k, k))
+        _CPU_SYSTEMS.append((np.asfortranarray(G @ G.T / k + 0.1 * np.eye(k)),
+                             np.asfortranarray(rng.standard_normal((k, int(r[i]))))))
+    with mp.get_context("fork").Pool(procs) as pool:
+        pool.map(_cpu_batched_share, [(k, 8 * procs) for k in range(procs)])   # warm
+        runs = sorted(max(pool.map(_cpu_batched_share, [(k, procs) for k in range(procs)]))
+                      for _ in range(reps))
+    dt = runs[len(runs) // 2]
+    _CPU_SYSTEMS.clear()
+    nsum = float(sum(int(n[i]) for i in idx))
+    return {"value": round(len(idx) / dt, 1), "unit": "systems/s", "cores": procs,
+            "kind": "lapack",
+            "sample": f"{len(idx)} of the {len(n)} systems' (n, nrhs), host-drawn matrices, "
+                      f"scipy cho_factor/cho_solve, 1 BLAS thread per process, "
+                      f"slowest of {procs} processes {dt:.3f} s, median of {reps} passes "
+                      f"(mean n {nsum / len(idx):.0f})"}
 
 
 def serve_workload(count: int, seed: int = 7):
@@ -347,6 +411,14 @@ def main() -> int:
               f"runs on one MI355X; multi-GPU sharding is a design note only).", file=sys.stderr)
         return 2
 
+    cpu_batched = None
+    if args.mode == "batched" and not args.no_cpu:
+        # the CPU reference's worker processes fork before this process
+        # touches the GPU
+        cpu_batched = cpu_batched_baseline(*batched_workload(10000))
+        log(f"[batched] CPU LAPACK: {cpu_batched['value']:.0f} systems/s "
+            f"({cpu_batched['sample']})")
+
     import oocdag_amd as ooc
     if ooc.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
@@ -373,7 +445,8 @@ def main() -> int:
         return 0
 
     if args.mode == "batched":
-        res = bench_batched(ooc, 10000, max(1, args.steps) * 3)
+        res = bench_batched(ooc, *batched_workload(10000), max(1, args.steps) * 3,
+                            cpu_rec=cpu_batched)
         print(json.dumps({"metric": "batched small SPD factor+solve throughput", "value":
                           round(res["fp64"]["systems_per_second"], 1), "unit": "systems/s",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
