@@ -131,6 +131,32 @@ def test_solve_large_grids(gpu, prec):
     assert eta <= 2 * 8 * EPS
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_mpir_many_rhs(gpu, prec):
+    """72 right-hand sides: the solves and the residual take the general GEMM
+    kernels (not the few-right-hand-side ones), whose K runs across the 32
+    tile columns (rows) of a super-step (n = 6144, NB = 128: two super-steps);
+    an FP32 factor's corrections then solve in FP64."""
+    ooc = gpu
+    n, nb, nrhs = 6144, 128, 72
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, 1e-2, seed=17)
+    A = gram_host(ooc, buf, n, nb)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=17, stream_id=2)
+    B = dB.to_array((n, nrhs))
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, prec, tol=8 * EPS, max_iter=30)
+    X = dX.to_array((n, nrhs))
+    assert st["fell_back"] == 0 and st["iterations"] <= 3, st
+    R = B - A @ X
+    eta = max(np.abs(R[:, k]).max() / (np.abs(A).sum(1).max() * np.abs(X[:, k]).max() +
+                                        np.abs(B[:, k]).max()) for k in range(nrhs))
+    assert eta <= 2 * 8 * EPS
+    Xref = np.linalg.solve(A, B)
+    assert np.abs(X - Xref).max() / np.abs(Xref).max() <= 1e2 * np.linalg.cond(A, 1) * EPS
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
 def test_mpir_update_variants_agree(gpu, prec):
     """Both FP32 update kernels accumulate every element over the same
