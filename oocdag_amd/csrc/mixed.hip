@@ -478,7 +478,8 @@ int mp_super_width(int NB, int bb) { return std::max(1, std::min(bb, 2048 / NB))
 // so the next super-column's FP64 work overlaps the bulk update.  lpbuf
 // holds two converted super-columns (by parity), scale 2*bb floats, amax bb.
 int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int *dinfo,
-                 void *lpbuf, float *scale, unsigned *amax, hipStream_t s, hipStream_t s1) {
+                 void *lpbuf, float *scale, unsigned *amax, float *colbuf, hipStream_t s,
+                 hipStream_t s1) {
     const int bb = (N + NB - 1) / NB;
     const size_t inv_per_tile = (size_t)(NB / 64) * 64 * 64;
     const Opnd P = panel(W, NB, bb, 1);
@@ -496,7 +497,8 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
     // factorization of super-column J+1 (else after it on s)
     const bool overlap = (mp_variant() & 2) == 0;
     const int rowmap = mfma_rowmap(mode, s);
-    if (rowmap < 0) return -1;
+    const int rowmap32 = mfma_rowmap(0, s);
+    if (rowmap < 0 || rowmap32 < 0) return -1;
     const int sw = mp_super_width(NB, bb);
     const int nsc = (bb + sw - 1) / sw;
     const size_t esz = mode == 0 ? 4 : 2;
@@ -527,13 +529,14 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
             MP_TRY(factor_column(P, t * NB, (bb - t) * NB, NB, t * NB,
                                  dinv_all + (size_t)t * inv_per_tile, dinfo, s));
             if (t + 1 < j1) {
-                Gemm g;
-                g.A = P; g.A.r0 = (t + 1) * NB; g.A.c0 = t * NB;
-                g.B = g.A;
-                g.C = P; g.C.r0 = (t + 1) * NB; g.C.c0 = (t + 1) * NB;
-                g.M = (bb - t - 1) * NB; g.N = (j1 - t - 1) * NB; g.K = NB;
-                g.alpha = -1.0; g.beta = 1.0; g.lower = 1;
-                MP_TRY(launch_gemm(g, GEMM_UPDATE, s));
+                // the rest of the super-column from tile column t, on the
+                // FP32 update kernel (FP32 copy of the column, FP32 sums
+                // subtracted from the FP64 tiles): its error is far below
+                // the low-precision bulk updates'
+                const int nrt = bb - t - 1;
+                MP_TRY(convert_super(W, NB, bb, t, 1, t + 1, nrt, 0, nullptr, nullptr, colbuf, s));
+                MP_TRY(launch_update_lp(0, W, bb, NB, (t + 1) * NB, colbuf, (long long)nrt * NB, NB,
+                                        nullptr, nrt * NB, (j1 - t - 1) * NB, rowmap32, s));
             }
         }
         if (j1 >= bb) break;
@@ -600,7 +603,7 @@ size_t mp_factor_bytes(int N, int NB) {
     const size_t tiles = bb * (bb + 1) / 2 * NB * NB;
     const size_t sw = (size_t)mp_super_width(NB, (int)bb);
     return (tiles + bb * NB * NB + bb * NB * 64) * 8 + (tiles + bb * NB * NB) * 4 +
-           2 * bb * NB * sw * NB * 4;
+           2 * bb * NB * sw * NB * 4 + bb * NB * NB * 4;
 }
 
 void mp_release(MpFactor &f, hipStream_t s) {
@@ -626,7 +629,7 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
     f.info = 0;
     f.bytes = mp_factor_bytes(N, NB);
     void *lp = nullptr;
-    float *scale = nullptr;
+    float *scale = nullptr, *colbuf = nullptr;
     unsigned *amax = nullptr;
     int *info = nullptr;
     double *anorm = nullptr;
@@ -637,6 +640,7 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
     const size_t sw = (size_t)mp_super_width(NB, bb);
     const size_t lp_bytes = precision > 0 ? 2 * (size_t)bb * NB * sw * NB * (precision == 1 ? 4 : 2) : 8;
     if (!ok(salloc(&lp, lp_bytes, s0)) || !ok(salloc(&scale, (size_t)2 * bb * 4, s0)) ||
+        (precision > 0 && !ok(salloc(&colbuf, (size_t)bb * NB * NB * 4, s0))) ||
         !ok(salloc(&amax, (size_t)bb * 4, s0)) || !ok(salloc(&info, sizeof(int), s0)) ||
         !ok(salloc(&anorm, (size_t)Np * 8, s0)))
         return OOCDAG_ENOMEM;
@@ -652,7 +656,7 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
             rc = OOCDAG_EHIP;
             break;
         }
-        int r = factor_mixed(f.W, N, NB, precision, f.dinv, info, lp, scale, amax, s0, s1);
+        int r = factor_mixed(f.W, N, NB, precision, f.dinv, info, lp, scale, amax, colbuf, s0, s1);
         if (r) { rc = OOCDAG_EHIP; break; }
         if (!ok(tile_inverses(f.W, N, NB, f.dinv, f.Z, s0))) {
             rc = OOCDAG_EHIP;
@@ -682,6 +686,7 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
     } while (0);
     (void)hipFreeAsync(lp, s0);
     (void)hipFreeAsync(scale, s0);
+    if (colbuf) (void)hipFreeAsync(colbuf, s0);
     (void)hipFreeAsync(amax, s0);
     (void)hipFreeAsync(info, s0);
     (void)hipFreeAsync(anorm, s0);
