@@ -173,6 +173,21 @@ __global__ void k_convert_super_f32(const double *W, int nb, int ym, int j0, int
     }
 }
 
+// Any R x K region of the packed lower panel (global rows gr0.., columns
+// gc0..; every tile it touches stored) -> FP32 column-major X (ld R).
+__global__ void k_convert_region_f32(const double *W, int nb, int ym, int gr0, int gc0, int R,
+                                     int K, float *X) {
+    const long long total = (long long)R * K;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long k = idx / R, r = idx - k * R;
+        const int gr = gr0 + (int)r, gc = gc0 + (int)k;
+        const int ti = gr / nb, tk = gc / nb;
+        X[idx] = (float)W[tix(ti, tk, ym, 1) * (long long)nb * nb + (gr - ti * nb) +
+                          (long long)(gc - tk * nb) * nb];
+    }
+}
+
 // BF16: one 64 x 64 patch per block, transposed through LDS so both the
 // column-major FP64 reads and the row-major BF16 writes are contiguous
 __global__ __launch_bounds__(256) void k_convert_super_bf16(const double *W, int nb, int ym, int j0,
@@ -580,6 +595,16 @@ hipError_t convert_super(const double *W, int nb, int ym, int j0, int sw, int i0
     hipLaunchKernelGGL(k_convert_super_bf16, dim3(nrt * nb / 64, sw * nb / 64), dim3(256), 0, s, W,
                        nb, ym, j0, i0, (const float *)scale, (unsigned short *)X,
                        (long long)sw * nb);
+    return hipGetLastError();
+}
+
+hipError_t convert_region_f32(const double *W, int nb, int ym, int gr0, int gc0, int R, int K,
+                              float *X, hipStream_t s) {
+    if (R <= 0 || K <= 0) return hipSuccess;
+    const long long total = (long long)R * K;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 32768);
+    hipLaunchKernelGGL(k_convert_region_f32, dim3(blocks), dim3(256), 0, s, W, nb, ym, gr0, gc0, R,
+                       K, X);
     return hipGetLastError();
 }
 

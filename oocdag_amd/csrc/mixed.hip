@@ -30,6 +30,8 @@ hipError_t convert_super(const double *W, int nb, int ym, int j0, int sw, int i0
 hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const void *X,
                             long long ldx, int K, const float *scale, int M, int N, int rowmap,
                             hipStream_t s);
+hipError_t convert_region_f32(const double *W, int nb, int ym, int gr0, int gc0, int R, int K,
+                              float *X, hipStream_t s);
 
 namespace {
 
@@ -521,13 +523,34 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
     do {                                    \
         if ((x) != hipSuccess) return -4;   \
     } while (0)
+    // Tile column t in FP64 by factor_column's 64-wide left-looking steps,
+    // split recursively into halves while a half is a multiple of 128 wide
+    // (NB = 512: 256 + 256, each 128 + 128): a right half's update from its
+    // left half (K = the half's width) runs on the FP32 update kernel, so the
+    // FP64 steps stream the tall panel far less often.
+    std::function<hipError_t(int, int, double *)> factor_cols = [&](int c0, int w,
+                                                                   double *dinv_c) -> hipError_t {
+        const int h = w / 2;
+        if (w < 256 || h % 128)
+            return factor_column(P, c0, bb * NB - c0, w, c0, dinv_c, dinfo, s);
+        hipError_t e = factor_cols(c0, h, dinv_c);
+        if (e != hipSuccess) return e;
+        const int r0 = c0 + h, M = bb * NB - r0;
+        if ((e = convert_region_f32(W, NB, bb, r0, c0, M, h, colbuf, s)) != hipSuccess ||
+            (e = launch_update_lp(0, W, bb, NB, r0, colbuf, M, h, nullptr, M, h, rowmap32, s)) !=
+                hipSuccess)
+            return e;
+        return factor_cols(r0, h, dinv_c + (size_t)(h / 64) * 64 * 64);
+    };
+    auto factor_column_halves = [&](int t) -> hipError_t {
+        return factor_cols(t * NB, NB, dinv_all + (size_t)t * inv_per_tile);
+    };
     for (int J = 0; J < nsc; ++J) {
         const int j0 = J * sw, j1 = std::min(j0 + sw, bb);
         // super-column J received bulk(J-2) on s1 (bulks run in order)
         if (J >= 2) MP_TRY(hipStreamWaitEvent(s, ev_bulk[J & 1], 0));
         for (int t = j0; t < j1; ++t) {
-            MP_TRY(factor_column(P, t * NB, (bb - t) * NB, NB, t * NB,
-                                 dinv_all + (size_t)t * inv_per_tile, dinfo, s));
+            MP_TRY(factor_column_halves(t));
             if (t + 1 < j1) {
                 // the rest of the super-column from tile column t, on the
                 // FP32 update kernel (FP32 copy of the column, FP32 sums
