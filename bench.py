@@ -8,7 +8,7 @@ between steps, outside the timed region (the reference also times only
 Cholesky(), oocdag.cpp:51,90).  Default workload: n = 196608, NB = 1024
 (155.4 GB of packed lower tiles).
 
-Prints one JSON line with the driver's contract fields plus
+Prints one JSON line (metric, value, unit, n_gpus, steps, timing, config ...) plus
   roofline     -- the panel-update GEMM (dominant kernel), timed by HIP events
                   on its own stream, algorithmic flops / kernel seconds vs the
                   78.6 TFLOP/s FP64 matrix peak;

@@ -1,6 +1,6 @@
 """GPU parity of the factorization paths, through the C ABI.
 
-Bar (north-star): scaled residual ||A - L L^T||_inf / (n eps ||A||_inf) <= 10
+Bar: scaled residual ||A - L L^T||_inf / (n eps ||A||_inf) <= 10
 (LIN/pdlafchk.f semantics, computed by the oracle) and L within 1e-10
 (normwise relative, max-abs) of the oracle's factor, which follows the
 reference's op order with host dpotrf tiles.  The strict upper triangle of A

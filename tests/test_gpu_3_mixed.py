@@ -1,5 +1,5 @@
-"""Mixed-precision factorization + FP64 iterative refinement (north-star
-addition; no reference code).  Checked against an FP64 oracle on the same
+"""Mixed-precision factorization + FP64 iterative refinement (an addition with
+no counterpart in the reference).  Checked against an FP64 oracle on the same
 inputs: numpy/LAPACK solve of the same matrix copied back from the device.
 Bars: normwise backward error <= tol (the FP64 level asked for), forward
 error within kappa * eps * 1e2 of the FP64 solution, and a correct answer

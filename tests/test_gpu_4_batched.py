@@ -1,4 +1,5 @@
-"""Batched small SPD factor + solve (north-star addition; no reference code).
+"""Batched small SPD factor + solve (an addition with no counterpart in the
+reference).
 Checked per system against an FP64 oracle (numpy/LAPACK solve of the same
 matrix copied back from the device); a system's result must be bit-identical
 alone and inside a batch."""

@@ -1,4 +1,4 @@
-"""Continuous-batching solve server (north-star addition; no reference code).
+"""Continuous-batching solve server (an addition with no counterpart in the reference).
 Every request must reach the requested FP64-level backward error, and each
 result must be the same alone and in the mix: |x_mix - x_alone| / |x_alone|
 <= 1e-12 (max norm)."""
