@@ -481,7 +481,7 @@ int mp_super_width(int NB, int bb) { return std::max(1, std::min(bb, 2048 / NB))
 // holds two converted super-columns (by parity), scale 2*bb floats, amax bb.
 int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int *dinfo,
                  void *lpbuf, float *scale, unsigned *amax, float *colbuf, hipStream_t s,
-                 hipStream_t s1) {
+                 hipStream_t s1, hipEvent_t w_ready) {
     const int bb = (N + NB - 1) / NB;
     const size_t inv_per_tile = (size_t)(NB / 64) * 64 * 64;
     const Opnd P = panel(W, NB, bb, 1);
@@ -568,8 +568,10 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
         const long long ldx = mode == 0 ? (long long)nrt * NB : (long long)K;
         // lp[J & 1] was last read by bulk(J-2), finished above
         MP_TRY(convert_super(W, NB, bb, j0, j1 - j0, j1, nrt, mode, amax, sc[J & 1], lp[J & 1], s));
-        // the next super-column also received bulk(J-1)
+        // the next super-column also received bulk(J-1); past the first
+        // super-column W is copied in on s1 (w_ready)
         if (J >= 1) MP_TRY(hipStreamWaitEvent(s, ev_bulk[(J - 1) & 1], 0));
+        if (J == 0 && w_ready) MP_TRY(hipStreamWaitEvent(s, w_ready, 0));
         const int k1 = std::min(j1 + sw, bb);
         MP_TRY(launch_update_lp(mode, W, bb, NB, j1 * NB, lp[J & 1], ldx, K, sc[J & 1],
                                 (bb - j1) * NB, (k1 - j1) * NB, rowmap, s));
@@ -667,19 +669,38 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
         !ok(salloc(&amax, (size_t)bb * 4, s0)) || !ok(salloc(&info, sizeof(int), s0)) ||
         !ok(salloc(&anorm, (size_t)Np * 8, s0)))
         return OOCDAG_ENOMEM;
+    hipEvent_t w_ready = nullptr;
+    if (!ok(hipEventCreateWithFlags(&w_ready, hipEventDisableTiming))) return OOCDAG_EHIP;
     int rc = OOCDAG_OK;
     do {
-        // ||A||_inf
-        hipLaunchKernelGGL(k_tile_rowabs, dim3(NB / 64, bb), dim3(256), 0, s0, dA, NB, bb, anorm);
         std::vector<double> hrow(Np);
         int big = INT_MAX, hinfo = INT_MAX;
-        if (!ok(hipMemcpyAsync(hrow.data(), anorm, Np * 8, hipMemcpyDeviceToHost, s0)) ||
-            !ok(hipMemcpyAsync(f.W, dA, tiles * 8, hipMemcpyDeviceToDevice, s0)) ||
+        // W <- A and ||A||_inf.  Low precision: the first super-column's tiles
+        // (contiguous, first in the packed order) on s0, the rest of the copy
+        // and then the row sums on s1, beside that super-column's FP64 work;
+        // the first update past it waits for the copy (w_ready), the row sums
+        // come back with the final synchronization.
+        const int sw0 = std::min(bb, mp_super_width(NB, bb));
+        const size_t head =
+            precision > 0 ? (size_t)tile_index(sw0, sw0, bb, 1) * NB * NB : tiles;   // tiles of columns < sw0
+        if (!ok(hipMemcpyAsync(f.W, dA, std::min(head, tiles) * 8, hipMemcpyDeviceToDevice, s0)) ||
             !ok(hipMemcpyAsync(info, &big, sizeof(int), hipMemcpyHostToDevice, s0))) {
             rc = OOCDAG_EHIP;
             break;
         }
-        int r = factor_mixed(f.W, N, NB, precision, f.dinv, info, lp, scale, amax, colbuf, s0, s1);
+        hipStream_t sn = s0;
+        if (head < tiles) {
+            if (!ok(hipMemcpyAsync(f.W + head, dA + head, (tiles - head) * 8,
+                                   hipMemcpyDeviceToDevice, s1)) ||
+                !ok(hipEventRecord(w_ready, s1))) {
+                rc = OOCDAG_EHIP;
+                break;
+            }
+            sn = s1;
+        }
+        hipLaunchKernelGGL(k_tile_rowabs, dim3(NB / 64, bb), dim3(256), 0, sn, dA, NB, bb, anorm);
+        int r = factor_mixed(f.W, N, NB, precision, f.dinv, info, lp, scale, amax, colbuf, s0, s1,
+                             head < tiles ? w_ready : nullptr);
         if (r) { rc = OOCDAG_EHIP; break; }
         if (!ok(tile_inverses(f.W, N, NB, f.dinv, f.Z, s0))) {
             rc = OOCDAG_EHIP;
@@ -699,6 +720,8 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
             hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s0, f.Z, f.Z32, (long long)zn);
         }
         if (!ok(hipGetLastError()) ||
+            !ok(hipMemcpyAsync(hrow.data(), anorm, Np * 8, hipMemcpyDeviceToHost, sn)) ||
+            !ok(hipStreamSynchronize(sn)) ||
             !ok(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s0)) ||
             !ok(hipStreamSynchronize(s0))) {
             rc = OOCDAG_EHIP;
@@ -707,6 +730,7 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
         f.Anorm = *std::max_element(hrow.begin(), hrow.end());
         f.info = hinfo == INT_MAX ? 0 : std::min(hinfo, N);
     } while (0);
+    (void)hipEventDestroy(w_ready);
     (void)hipFreeAsync(lp, s0);
     (void)hipFreeAsync(scale, s0);
     if (colbuf) (void)hipFreeAsync(colbuf, s0);
