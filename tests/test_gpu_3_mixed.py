@@ -131,6 +131,29 @@ def test_solve_large_grids(gpu, prec):
     assert eta <= 2 * 8 * EPS
 
 
+@pytest.mark.parametrize("n", [2048, 5000])
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_mpir_nb1024(gpu, prec, n):
+    """NB = 1024: super-columns of two tile columns, each tile column factored
+    as halves of 512, 256 and 128 columns; n = 2048 is a single super-column
+    (A copied whole before the factorization), n = 5000 has a ragged last
+    tile and three super-columns."""
+    ooc = gpu
+    nb, nrhs = 1024, 3
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, 1e-2, seed=19)
+    A = gram_host(ooc, buf, n, nb)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=19, stream_id=2)
+    B = dB.to_array((n, nrhs))
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, prec, tol=8 * EPS, max_iter=30)
+    X = dX.to_array((n, nrhs))
+    assert st["fell_back"] == 0 and st["iterations"] <= {"fp32": 3, "bf16": 16}[prec], st
+    Xref = np.linalg.solve(A, B)
+    assert np.abs(X - Xref).max() / np.abs(Xref).max() <= 1e2 * np.linalg.cond(A, 1) * EPS
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_mpir_many_rhs(gpu, prec):
     """72 right-hand sides: the solves and the residual take the general GEMM
