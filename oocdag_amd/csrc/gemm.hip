@@ -733,9 +733,9 @@ __device__ __forceinline__ void rhs_store(const RArgs &a, int m0, int n, int lan
     }
 }
 
-// K may span several tiles of a packed B (the solves' bulk updates take 8
-// tile columns or rows at once): the kernels walk K in chunks that end at
-// tile edges, with the operand pointers recomputed per chunk.
+// K may span several tiles of a packed B (the solves' bulk updates take
+// 4096/NB tile columns or rows at once): the kernels walk K in chunks that
+// end at tile edges, with the operand pointers recomputed per chunk.
 __device__ __forceinline__ int rhs_chunk(const Opnd &B, int gk, int left) {
     return B.ts > 0 ? min(left, B.ts - gk % B.ts) : left;
 }
