@@ -913,7 +913,7 @@ extern "C" int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const do
     if (!st) st = &dummy;
     *st = oocdag_mpir_stats{};
     if (!dA || !dB || !dX || N <= 0 || nrhs <= 0 || ldb < N || ldx < N || NB < 128 || NB % 128 ||
-        precision < 0 || precision > 2 || (ldb & 1) || (ldx & 1 && ldx != N))
+        precision < 0 || precision > 2)
         return OOCDAG_EARG;
     hipStream_t s = nullptr, s1 = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return OOCDAG_EHIP;

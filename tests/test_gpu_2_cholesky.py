@@ -126,6 +126,51 @@ def test_cholesky_reports_non_spd_column(gpu, ora, bad):
     assert info == linfo == bad + 1
 
 
+@pytest.mark.parametrize("n,nb,budget_tiles", [
+    (1, 64, 2),          # 1 x 1: L = sqrt(a)
+    (7, 64, 2),          # one tile, almost all of it padding
+    (64, 64, 2),         # exactly one full tile
+    (100, 256, 3),       # one ragged tile, NB above N
+    (257, 256, 4),       # two panels, the second a one-row tile column
+    (257, 256, 1 << 12),
+])
+def test_cholesky_small_sizes(gpu, ora, n, nb, budget_tiles):
+    """N <= NB (a single tile, padded with the identity) and a one-row edge
+    tile, through every entry of the path: the column-major out-of-core
+    entry, device tiles (potrf_tiles), and both solves."""
+    ooc = gpu
+    A0 = ora.pdmatgen(n, nb)
+    Lref = A0.copy(order="F")
+    _, info_ref = ora.cholesky(Lref, nb, 1 << 30)
+    assert info_ref == 0
+    A = A0.copy(order="F")
+    _, info = ooc.cholesky(A, nb, budget_tiles * nb * nb * 8)
+    assert info == 0
+    assert rel(A, Lref) < L_TOL
+    assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))
+    assert ora.llt_resid(A) <= 10.0
+    if n == 1:
+        assert A[0, 0] == pytest.approx(np.sqrt(A0[0, 0]), rel=1e-15)
+
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.pdmatgen_tiles(buf, n, nb)
+    assert ooc.potrf_tiles(buf, n, nb) == 0
+    T = np.zeros((n, n), order="F")
+    ooc.tiles_to_host(buf, T, nb)
+    assert rel(T, Lref) < L_TOL
+
+    rng = np.random.default_rng(n)
+    B = np.asfortranarray(rng.standard_normal((n, 3)))
+    Xref = np.linalg.solve(A0, B)
+    X = B.copy(order="F")
+    ooc.potrs(A, X)
+    assert np.max(np.abs(X - Xref)) / np.max(np.abs(Xref)) < 1e-12
+    dB = ooc.DeviceBuffer.from_array(B)
+    ooc.potrs_tiles(buf, n, nb, dB, 3, n)
+    X = dB.to_array((n, 3))
+    assert np.max(np.abs(X - Xref)) / np.max(np.abs(Xref)) < 1e-12
+
+
 def test_cholesky_budget_error(gpu):
     A = np.eye(1024, order="F")
     with pytest.raises(gpu.OocdagError):

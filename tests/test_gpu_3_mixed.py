@@ -42,8 +42,10 @@ def test_gram_generator_properties(gpu):
 
 # (2048, 256): one super-column (all updates FP64 inside it);
 # (6144, 128): three super-columns of 16 tile columns -- narrow and bulk
-# low-precision updates, both buffer parities.
-@pytest.mark.parametrize("n,nb", [(2048, 256), (6144, 128)])
+# low-precision updates, both buffer parities; (100, 128) and (129, 256):
+# a single padded tile (the second factored as two 128-column halves);
+# (300, 128): one super-column narrower than 2048 columns, ragged.
+@pytest.mark.parametrize("n,nb", [(2048, 256), (6144, 128), (100, 128), (129, 256), (300, 128)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
 @pytest.mark.parametrize("mu", [1e-2, 1e-6])
 def test_mpir_matches_fp64_oracle(gpu, prec, mu, n, nb):
