@@ -18,6 +18,24 @@ def rel(a, b):
     return np.max(np.abs(np.tril(a) - np.tril(b))) / np.max(np.abs(np.tril(b)))
 
 
+def expected_traffic(n, nb, widths):
+    """Bytes an out-of-core run must copy in and out for the panel widths:
+    per panel of Yn tile columns at tile column jb (Ym = bb - jb tile rows),
+    A2Y loads its packed tile columns, sum_jj (Ym - jj) tiles
+    (oocquark.cpp:253-275), and A2X loads rows jb.. of every earlier tile
+    column, jb * Ym tiles (oocquark.cpp:301-319); every tile of the packed
+    lower triangle is copied out once (oocquark.cpp:389-419)."""
+    bb = -(-n // nb)
+    h2d, jb = 0, 0
+    for yn in widths:
+        ym = bb - jb
+        h2d += sum(ym - jj for jj in range(yn)) + jb * ym
+        jb += yn
+    assert jb == bb
+    tile = nb * nb * 8
+    return h2d * tile, bb * (bb + 1) // 2 * tile
+
+
 @pytest.mark.parametrize("n,nb,budget_tiles", [
     (4096, 256, 56),     # 28 MiB: the 4-panel OOC configuration
     (1000, 64, 96),      # ragged edge, many panels
@@ -35,6 +53,9 @@ def test_cholesky_host_ooc_matches_oracle(gpu, ora, n, nb, budget_tiles):
     t, info = ooc.cholesky(A, nb, mem)
     print(f"n={n} nb={nb} panels={ooc.panel_plan(n, nb, mem)} t={t:.4f}s")
     assert info == info_ref == 0
+    st = ooc.last_stats()
+    h2d, d2h = expected_traffic(n, nb, ooc.panel_plan(n, nb, mem))
+    assert (st["h2d_bytes"], st["d2h_bytes"]) == (h2d, d2h)
     assert rel(A, Lref) < L_TOL
     assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))
     r = ora.llt_resid(A)
@@ -78,6 +99,9 @@ def test_cholesky_host_tiles_ooc(gpu, ora, n, nb, budget_tiles, pinned):
         t.generate()
         secs, info = ooc.cholesky_host_tiles(t, mem)
         assert info == 0 and secs > 0
+        st = ooc.last_stats()
+        h2d, d2h = expected_traffic(n, nb, ooc.panel_plan(n, nb, mem))
+        assert (st["h2d_bytes"], st["d2h_bytes"]) == (h2d, d2h)
         L = ooc.tiles_to_dense(t.array(), n, nb)
     finally:
         t.free()
@@ -228,6 +252,9 @@ def test_large_factor_out_of_core_matches_hbm(gpu):
         t.generate()
         secs, info = ooc.cholesky_host_tiles(t, mem)
         assert info == 0 and secs > 0
+        st = ooc.last_stats()
+        h2d, d2h = expected_traffic(n, nb, ooc.panel_plan(n, nb, mem))
+        assert (st["h2d_bytes"], st["d2h_bytes"]) == (h2d, d2h)
         L_ooc = t.array().copy()
     finally:
         t.free()
