@@ -175,6 +175,11 @@ def test_cholesky_small_sizes(gpu, ora, n, nb, budget_tiles):
     assert ora.llt_resid(A) <= 10.0
     if n == 1:
         assert A[0, 0] == pytest.approx(np.sqrt(A0[0, 0]), rel=1e-15)
+    # the GPU residual: rounding-level, and a planted error seen as the oracle sees it
+    assert ooc.llt_resid(A) <= 10.0
+    P = A.copy(order="F")
+    P[n - 1, 0] += 1e-9
+    assert ooc.llt_resid(P) == pytest.approx(ora.llt_resid(P), rel=1e-3)
 
     buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
     ooc.pdmatgen_tiles(buf, n, nb)
@@ -182,6 +187,7 @@ def test_cholesky_small_sizes(gpu, ora, n, nb, budget_tiles):
     T = np.zeros((n, n), order="F")
     ooc.tiles_to_host(buf, T, nb)
     assert rel(T, Lref) < L_TOL
+    assert ooc.llt_resid_tiles(buf, n, nb) <= 10.0
 
     rng = np.random.default_rng(n)
     B = np.asfortranarray(rng.standard_normal((n, 3)))
