@@ -221,3 +221,35 @@ def test_mpir_non_spd_reports_info(gpu):
     dX = ooc.DeviceBuffer(n * 8)
     st = ooc.mpir_solve_tiles(buf, n, nb, dB, 1, dX, "fp32")
     assert st["info"] > 0 and st["fell_back"] == 1
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16"])
+def test_mpir_padded_leading_dimensions(gpu, prec):
+    """B and X inside larger column-major arrays (odd ldb = N + 3, ldx = N + 1)
+    through the C ABI: the same solution, bit for bit, as with ldb = ldx = N,
+    and the padding rows of X left as they were."""
+    import ctypes as C
+    ooc = gpu
+    n, nb, nrhs = 300, 128, 3
+    ldb, ldx = n + 3, n + 1
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, 1e-2, seed=4)
+    rng = np.random.default_rng(4)
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    dB = ooc.DeviceBuffer.from_array(B)
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, prec)
+    X = dX.to_array((n, nrhs))
+    Bp = np.full((ldb, nrhs), 7.0, order="F")
+    Bp[:n] = B
+    Xp = np.full((ldx, nrhs), -3.0, order="F")
+    dBp, dXp = ooc.DeviceBuffer.from_array(Bp), ooc.DeviceBuffer.from_array(Xp)
+    stp = ooc.MpirStats()
+    tol = 8 * EPS
+    rc = ooc.lib().oocdag_mpir_solve_tiles(buf.ptr, n, nb, dBp.ptr, nrhs, ldb, dXp.ptr, ldx,
+                                           ooc.PRECISIONS[prec], tol, 30, C.byref(stp))
+    assert rc == 0
+    Xp = dXp.to_array((ldx, nrhs))
+    assert np.array_equal(Xp[:n], X)
+    assert (Xp[n:] == -3.0).all()
+    assert stp.as_dict()["iterations"] == st["iterations"]
