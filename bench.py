@@ -1,19 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark: FP64 Cholesky TFLOP/s (n^3/3 flops) on one MI355X.
+"""Benchmark: FP64 out-of-core Cholesky TFLOP/s (n^3/3 flops) on one MI355X.
 
-One step = one factorization of the n x n PDMATGEN('S','D', ISEED=100)
-matrix (the reference's input, oocdag.cpp:298-302) whose packed lower tiles
-are already resident in HBM when the timed region starts; the generator runs
-between steps, outside the timed region (the reference also times only
-Cholesky(), oocdag.cpp:51,90).  Default workload: n = 196608, NB = 1024
-(155.4 GB of packed lower tiles).
+Default workload (the reference's own path, Cholesky() oocdag.cpp:18-112, at
+n = 196608, NB = 1024): the PDMATGEN('S','D', ISEED=100) matrix
+(oocdag.cpp:298-302) held in host memory as packed lower tiles (155.4 GB),
+the device limited to a 64 GiB budget -- below the packed triangle, so the
+factorization streams column panels over PCIe with double-buffered copies.
+One step = one out-of-core factorization, timed PCIe-inclusive from the call
+to its return (the reference times only Cholesky(), oocdag.cpp:51,90); the
+matrix is regenerated into the host tiles between steps, outside the clock.
+
+--steps and --warmup are upper bounds under a wall-clock budget (--budget-s,
+from process start): a step starts only when the time it is expected to take
+still fits; the JSON line reports the steps and warmups actually run.
 
 Prints one JSON line (metric, value, unit, n_gpus, steps, timing, config ...) plus
   roofline     -- the panel-update GEMM (dominant kernel), timed by HIP events
-                  on its own stream, algorithmic flops / kernel seconds vs the
-                  78.6 TFLOP/s FP64 matrix peak;
-  cpu_baseline -- the CPU oracle (a port of the reference driver, run on the
-                  host's cores) timed on a bounded sample (n = --cpu-n).
+                  on the stream each launch runs on, algorithmic flops / kernel
+                  seconds vs the 78.6 TFLOP/s FP64 matrix peak; traffic from
+                  the committed rocprofv3 PMC summary of the same command;
+  cpu_baseline -- the CPU restatement of the reference driver (oracle/oracle.c)
+                  on the host's cores at n = 4096, NB = 256, a 28 MiB budget
+                  (the reference's 4-panel schedule [2, 3, 5, 6]), median of
+                  repeated runs.
 """
 from __future__ import annotations
 
@@ -22,6 +31,8 @@ import json
 import os
 import sys
 import time
+
+_T_START = time.monotonic()
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -33,53 +44,92 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def verify_small(ooc) -> float:
-    """Scaled residual of a small factorization through the same path (the
-    full-size L is checked by the GPU tests; here a cheap guard that the
-    timed code computes a correct factor)."""
+def elapsed() -> float:
+    return time.monotonic() - _T_START
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def verify_small_ooc(ooc) -> float:
+    """Scaled residual of a small factorization through the timed entry point
+    (host tiles, a budget of a few tile columns: several panels, streamed),
+    formed on the GPU against the regenerated matrix.  A cheap guard that the
+    timed code computes a correct factor; the full-size factor is checked by
+    --verify-full and by the GPU tests."""
+    n, nb = 12288 - 300, 1024
+    t = ooc.HostTiles(n, nb, pinned=False)
+    try:
+        t.generate()
+        _, info = ooc.cholesky_host_tiles(t, 40 * nb * nb * 8)
+        if info != 0:
+            raise RuntimeError(f"small out-of-core check: info={info}")
+        buf = ooc.DeviceBuffer(t.elems * 8)
+        ooc.lib().oocdag_memcpy_h2d(buf.ptr, t.ptr, t.elems * 8)
+        r = ooc.llt_resid_tiles(buf, n, nb)
+        buf.free()
+    finally:
+        t.free()
+    return float(r)
+
+
+def cpu_baseline(n: int = 4096, nb: int = 256, memsize: int = 28 << 20,
+                 min_runs: int = 5, target_s: float = 3.0) -> dict:
+    """The reference driver restated on the CPU (oracle/oracle.c: the
+    reference's panel planner find_Yn, A2Y/A2X staging into a host buffer of
+    memBlock tiles and the same tile-task order), one task per tile on all
+    host cores, each tile kernel single-threaded MKL (the reference linked
+    mkl_sequential for its host dpotrf, Makefile:17,45) when the MKL runtime
+    is present, else the oracle's own loops.  n = 4096, NB = 256 with a
+    28 MiB budget: memBlock = 56 tiles, panels [2, 3, 5, 6].  Median of at
+    least `min_runs` runs (until about `target_s` seconds), each timed from
+    after the buffer allocation to the final join (oocdag.cpp:51,90)."""
     import numpy as np
-    import oracle
-    n, nb = 4096, 1024
-    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
-    ooc.pdmatgen_tiles(buf, n, nb)
-    assert ooc.potrf_tiles(buf, n, nb) == 0
-    A = oracle.pdmatgen(n, nb)
-    ooc.tiles_to_host(buf, A, nb)
-    buf.free()
-    return float(oracle.llt_resid(A))
-
-
-def cpu_baseline(n: int, nb: int) -> dict:
-    """The reference driver restated on the CPU (oracle/oracle.c: same panel
-    loop, staging and tile-task order), one task per tile on all host cores,
-    each tile kernel single-threaded MKL when the MKL runtime is present (the
-    reference linked MKL for its host dpotrf, Makefile:17,45), else the
-    oracle's own loops.  The whole matrix is one panel (budget = lower
-    triangle + one X column)."""
     import oracle
     oracle.build()
     backend = oracle.set_backend(1)
     threads = oracle.lib().ora_get_threads()
-    A = oracle.pdmatgen(n, nb)
-    bb = -(-n // nb)
-    mem = (bb * (bb + 1) // 2 + bb) * nb * nb * 8
-    t, info = oracle.cholesky(A, nb, mem)
+    A0 = oracle.pdmatgen(n, nb)
+    plan = oracle.panel_plan(n, nb, memsize)
+    times = []
+    t_begin = time.perf_counter()
+    while len(times) < min_runs or (time.perf_counter() - t_begin < target_s and len(times) < 201):
+        A = A0.copy(order="F")
+        t, info = oracle.cholesky(A, nb, memsize)
+        if info != 0:
+            raise RuntimeError(f"CPU baseline factorization failed, info={info}")
+        times.append(t)
+    resid = float(oracle.llt_resid(A))
     oracle.set_backend(0)
-    if info != 0:
-        raise RuntimeError(f"CPU baseline factorization failed, info={info}")
-    kern = "MKL dgemm/dsyrk/dtrsm/dpotrf (sequential) per tile" if backend == 1 else \
-        "the oracle's own FP64 tile loops"
-    return {"value": round(n ** 3 / 3 / t / 1e12, 5), "unit": "TFLOP/s", "cores": threads,
+    med = float(np.median(times))
+    blas = "MKL (libmkl_rt, sequential per tile task)" if backend == 1 else \
+        "the oracle's own FP64 tile loops (no MKL runtime found)"
+    return {"value": round(n ** 3 / 3 / med / 1e12, 5), "unit": "TFLOP/s", "cores": threads,
             "kind": "port",
-            "sample": f"CPU restatement of the reference driver (oracle/oracle.c), n={n} nb={nb}, "
-                      f"{threads} threads, {kern}, {t:.2f} s"}
+            "sample": f"CPU restatement of the reference driver (oracle/oracle.c) at n={n} "
+                      f"nb={nb} memsize={memsize >> 20} MiB -> panels {plan}, median of "
+                      f"{len(times)} runs = {med * 1e3:.1f} ms (min {min(times) * 1e3:.1f}, "
+                      f"max {max(times) * 1e3:.1f}), scaled residual {resid:.2e}; "
+                      f"{threads} threads on {os.cpu_count()} logical CPUs ({cpu_model()}); "
+                      f"BLAS: {blas}",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "blas": blas,
+            "median_s": med, "runs": len(times), "panels": plan}
 
 
 def update_algorithmic_bytes(n: int, nb: int) -> int:
-    """Average algorithmic HBM bytes per update-GEMM launch of the look-ahead
-    factorization: each launch reads its lower trapezoid C once, writes it
-    once and reads the K = nb wide panel column once (B is rows of that same
-    column); 16 B per C element + 8 B per panel element."""
+    """Average algorithmic HBM bytes per update-GEMM launch of the
+    HBM-resident look-ahead factorization (--mode hbm): each launch reads its
+    lower trapezoid C once, writes it once and reads the K = nb wide panel
+    column once (B is rows of that same column); 16 B per C element + 8 B per
+    panel element."""
     bb = -(-n // nb)
     tot = cnt = 0
     for j in range(bb - 1):
@@ -93,71 +143,126 @@ def update_algorithmic_bytes(n: int, nb: int) -> int:
     return tot // max(cnt, 1)
 
 
-def measured_traffic(n: int, nb: int):
-    """HBM bytes per update-GEMM launch from the committed PMC summary of the
-    same workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
-    reading of a copy kernel of known size); None when no summary covers this
-    (n, nb)."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"update_gemm_n{n}_pmc_traffic.md")
+def measured_traffic(name: str, key: dict):
+    """HBM bytes per update-GEMM launch from a committed PMC summary
+    (profiles/<name>: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    the reading of a copy kernel of known size).  The summary's
+    `traffic_bytes_per_launch:` line must match every field of `key`; None
+    when no summary covers the workload."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     fields = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
-                    if int(fields["nb"]) == nb:
-                        return int(fields["bytes"]), os.path.relpath(path, os.path.dirname(path) + "/..")
+                    if all(fields.get(k) == str(v) for k, v in key.items()):
+                        return int(fields["bytes"]), "profiles/" + name
     except (OSError, KeyError, ValueError):
         pass
     return None, None
 
 
-def bench_ooc(ooc, n: int, nb: int, budget_gib: float, verify_full: bool = False) -> dict:
-    """Out-of-core FP64 factorization as the reference runs it: the matrix in
-    host memory (packed lower tiles, pinned), the device limited to a budget
-    below the packed triangle, panels streamed over PCIe.  PCIe-inclusive
-    wall time of the factorization; copy-stream busy time gives GB/s."""
+class Budget:
+    """Wall-clock budget of the whole run, counted from process start."""
+
+    def __init__(self, seconds: float, reserve: float):
+        self.deadline = seconds
+        self.reserve = reserve   # kept for the work after the timed steps
+
+    def fits(self, need: float) -> bool:
+        return elapsed() + need + self.reserve <= self.deadline
+
+
+def bench_ooc(ooc, n: int, nb: int, budget_gib: float, steps: int, warmup: int,
+              budget: Budget, verify_full: bool = False) -> dict:
+    """The out-of-core FP64 factorization as the reference runs it: the
+    matrix in host memory (packed lower tiles, pageable), the device limited
+    to a budget below the packed triangle, column panels streamed over PCIe.
+    Each step regenerates the matrix (outside the clock), then times one
+    oocdag_cholesky_host_tiles call PCIe-inclusive; the update GEMM's HIP
+    events give the roofline figures, the copy streams' busy time GB/s."""
     mem = int(budget_gib * (1 << 30))
     plan = ooc.panel_plan(n, nb, mem)
     t0 = time.perf_counter()
-    t = ooc.HostTiles(n, nb, pinned=None)
-    t1 = time.perf_counter()
-    t.generate()
-    t2 = time.perf_counter()
-    log(f"[ooc] n={n} nb={nb}: {t.elems * 8 / 1e9:.1f} GB {'pinned' if t.pinned else 'pageable'} "
-        f"host tiles in {t1 - t0:.1f} s, "
-        f"generated in {t2 - t1:.1f} s; budget {budget_gib} GiB, panels {plan}")
-    ooc.set_progress(16)
-    try:
+    t = ooc.HostTiles(n, nb, pinned=False)
+    log(f"[ooc] n={n} nb={nb}: {t.elems * 8 / 1e9:.1f} GB pageable host tiles; budget "
+        f"{budget_gib} GiB, panels {plan}; {elapsed():.1f} s since start")
+    gen_s, fac_s = [], []
+    fresh = False
+    ooc.set_progress(32)
+
+    def factor(timed: bool):
+        nonlocal fresh
+        if not fresh:
+            g0 = time.perf_counter()
+            t.generate()
+            gen_s.append(time.perf_counter() - g0)
+            log(f"[ooc] generated in {gen_s[-1]:.1f} s")
+        ooc.profile_enable(timed)
+        t1 = time.perf_counter()
         secs, info = ooc.cholesky_host_tiles(t, mem)
-        st = ooc.last_stats()
-        tiles = t.array()
-        last = float(tiles[-1])   # L(N-1, N-1) when N is a multiple of nb
-        resid = None
-        if verify_full:
-            # the factor that came back to the host, copied into HBM whole and
-            # checked there (LIN residual against the regenerated matrix)
-            t3 = time.perf_counter()
-            buf = ooc.DeviceBuffer(t.elems * 8)
-            ooc.lib().oocdag_memcpy_h2d(buf.ptr, t.ptr, t.elems * 8)
-            resid = ooc.llt_resid_tiles(buf, n, nb)
-            buf.free()
-            log(f"[ooc] scaled residual of the out-of-core factor {resid:.4e} "
-                f"({time.perf_counter() - t3:.1f} s)")
-    finally:
-        t.free()
-    flops = n ** 3 / 3.0
-    out = {"seconds": secs, "tflops": flops / secs / 1e12, "info": info, "panels": plan,
-           "host_tiles_pinned": t.pinned,
-           "budget_bytes": mem, "h2d_bytes": st["h2d_bytes"], "d2h_bytes": st["d2h_bytes"],
-           "h2d_seconds": st["h2d_seconds"], "d2h_seconds": st["d2h_seconds"],
-           "h2d_GBps": st["h2d_bytes"] / max(st["h2d_seconds"], 1e-12) / 1e9,
-           "d2h_GBps": st["d2h_bytes"] / max(st["d2h_seconds"], 1e-12) / 1e9,
-           "L_last_diag": last, "scaled_residual_full": resid}
-    log(f"[ooc] {secs:.3f} s, {out['tflops']:.2f} TFLOP/s PCIe-inclusive; H2D "
-        f"{out['h2d_bytes'] / 1e9:.1f} GB at {out['h2d_GBps']:.1f} GB/s, D2H "
-        f"{out['d2h_bytes'] / 1e9:.1f} GB at {out['d2h_GBps']:.1f} GB/s, info {info}")
-    return out
+        t2 = time.perf_counter()
+        fresh = False
+        prof = ooc.profile_read()
+        ooc.profile_enable(False)
+        if info != 0:
+            raise RuntimeError(f"out-of-core factorization info={info}")
+        fac_s.append(t2 - t1)
+        return secs, t2 - t1, prof, ooc.last_stats()
+
+    def step_cost():
+        return (max(gen_s) if gen_s else 30.0) + (max(fac_s) if fac_s else 60.0)
+
+    warm_done = 0
+    for w in range(warmup):
+        # the first warmup always runs; later ones only while every timed
+        # step still fits after them
+        if w >= 1 and not budget.fits((steps + 1) * step_cost()):
+            break
+        secs, wall, _, _ = factor(False)
+        warm_done += 1
+        log(f"[ooc] warmup {warm_done}: {secs:.3f} s ({n ** 3 / 3 / secs / 1e12:.2f} TFLOP/s); "
+            f"{elapsed():.1f} s since start")
+    rows = []
+    for k in range(steps):
+        if k >= 1 and not budget.fits(step_cost()):
+            log(f"[ooc] step {k + 1} would cross the {budget.deadline:.0f} s budget: stopping "
+                f"after {k} steps")
+            break
+        secs, wall, prof, st = factor(True)
+        rows.append((secs, wall, prof, st))
+        log(f"[ooc] step {k + 1}: {secs:.3f} s, {n ** 3 / 3 / secs / 1e12:.2f} TFLOP/s "
+            f"PCIe-inclusive; update GEMM {prof['launches']} launches {prof['seconds']:.3f} s "
+            f"{prof['flops'] / max(prof['seconds'], 1e-12) / 1e12:.2f} TFLOP/s; H2D "
+            f"{st['h2d_bytes'] / 1e9:.1f} GB D2H {st['d2h_bytes'] / 1e9:.1f} GB; "
+            f"{elapsed():.1f} s since start")
+    tiles = t.array()
+    last = float(tiles[-1])   # L(N-1, N-1) when N is a multiple of nb
+    resid = None
+    if verify_full:
+        t3 = time.perf_counter()
+        buf = ooc.DeviceBuffer(t.elems * 8)
+        ooc.lib().oocdag_memcpy_h2d(buf.ptr, t.ptr, t.elems * 8)
+        resid = ooc.llt_resid_tiles(buf, n, nb)
+        buf.free()
+        log(f"[ooc] scaled residual of the out-of-core factor {resid:.4e} "
+            f"({time.perf_counter() - t3:.1f} s)")
+    t.free()
+    secs = [r[0] for r in rows]
+    h2d_b = sum(r[3]["h2d_bytes"] for r in rows)
+    h2d_t = sum(r[3]["h2d_seconds"] for r in rows)
+    d2h_b = sum(r[3]["d2h_bytes"] for r in rows)
+    d2h_t = sum(r[3]["d2h_seconds"] for r in rows)
+    return {"step_seconds": secs, "warmup_done": warm_done, "steps_done": len(rows),
+            "generate_seconds": gen_s, "panels": plan, "budget_bytes": mem,
+            "h2d_bytes_per_step": h2d_b / max(len(rows), 1),
+            "d2h_bytes_per_step": d2h_b / max(len(rows), 1),
+            "h2d_GBps": h2d_b / max(h2d_t, 1e-12) / 1e9, "d2h_GBps": d2h_b / max(d2h_t, 1e-12) / 1e9,
+            "prof_launches": sum(r[2]["launches"] for r in rows),
+            "prof_seconds": sum(r[2]["seconds"] for r in rows),
+            "prof_flops": sum(r[2]["flops"] for r in rows),
+            "prof_bytes": sum(r[2]["bytes"] for r in rows),
+            "L_last_diag": last, "scaled_residual_full": resid}
 
 
 def bench_mpir(ooc, n: int, nb: int, nrhs: int, mus, precs) -> dict:
@@ -382,19 +487,25 @@ def bench_serve(ooc, count: int, rate: float, slots: int, budget_gb: float) -> d
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3,
+                    help="timed steps (an upper bound under --budget-s)")
+    ap.add_argument("--warmup", type=int, default=1,
+                    help="untimed warmup steps (an upper bound under --budget-s)")
+    ap.add_argument("--budget-s", type=float, default=420.0,
+                    help="wall-clock budget of the whole run from process start; no step "
+                         "starts that is expected to cross it")
     ap.add_argument("--n", type=int, default=196608)
     ap.add_argument("--nb", type=int, default=1024)
-    ap.add_argument("--cpu-n", type=int, default=28672)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--verify-full", action="store_true",
                     help="after the timed steps, form the LIN scaled residual of the full-size "
                          "factor on the GPU (another n^3/3 flops, outside the clock)")
-    ap.add_argument("--mode", choices=["fp64", "mpir", "batched", "serve", "ooc"], default="fp64",
-                    help="fp64: the headline HBM-resident factorization; mpir: the mixed-"
-                         "precision + refinement workload (n=65536, nb=512, 16 RHS)")
+    ap.add_argument("--mode", choices=["ooc", "hbm", "mpir", "batched", "serve"], default="ooc",
+                    help="ooc: the headline out-of-core factorization (host tiles, device "
+                         "budget, PCIe-inclusive); hbm: the same factorization with the packed "
+                         "tiles resident in HBM; mpir: mixed precision + refinement (n=65536, "
+                         "nb=512, 16 RHS); batched: 10000 small systems; serve: Poisson stream")
     ap.add_argument("--budget-gib", type=float, default=64.0,
                     help="ooc mode: device budget (GiB) for the out-of-core factorization")
     ap.add_argument("--mpir-precisions", default="fp64,fp32,bf16",
@@ -424,6 +535,17 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
 
+    if args.copy_calibration:
+        nbytes = args.copy_calibration
+        src, dst = ooc.DeviceBuffer(nbytes), ooc.DeviceBuffer(nbytes)
+        ooc.lib().oocdag_memset_d(src.ptr, 1, nbytes)
+        for _ in range(3):
+            ooc.lib().oocdag_memcpy_d2d(dst.ptr, src.ptr, nbytes)
+        ooc.lib().oocdag_device_sync()
+        log(f"[bench] copy calibration: 3 device copies of {nbytes} bytes")
+        src.free()
+        dst.free()
+
     if args.mode == "serve":
         res = bench_serve(ooc, 120, 4.0, 3, 200.0)
         print(json.dumps({"metric": "served SPD solves: p99 latency at offered load",
@@ -431,17 +553,6 @@ def main() -> int:
                           "n_gpus": 1, "higher_is_better": False, "dtype": "f64/f32",
                           "config": {"workload": "Poisson 4 req/s, n~logU[64,32768], 1-64 rhs, "
                                                  "30% matrix reuse", "results": res}}), flush=True)
-        return 0
-
-    if args.mode == "ooc":
-        res = bench_ooc(ooc, args.n, args.nb, args.budget_gib, args.verify_full)
-        print(json.dumps({"metric": "Cholesky TFLOP/s (n^3/3), out of core, PCIe-inclusive",
-                          "value": round(res["tflops"], 4), "unit": "TFLOP/s", "n_gpus": 1,
-                          "higher_is_better": True, "dtype": "f64",
-                          "data": "synthetic: PDMATGEN('S','D', ISEED=100) into pinned host tiles",
-                          "config": {"workload": f"FP64 out-of-core tiled Cholesky n={args.n} "
-                                                 f"nb={args.nb}, {args.budget_gib} GiB device budget",
-                                     "results": res}}), flush=True)
         return 0
 
     if args.mode == "batched":
@@ -468,96 +579,129 @@ def main() -> int:
               flush=True)
         return 0
 
-    if args.copy_calibration:
-        nbytes = args.copy_calibration
-        src, dst = ooc.DeviceBuffer(nbytes), ooc.DeviceBuffer(nbytes)
-        ooc.lib().oocdag_memset_d(src.ptr, 1, nbytes)
-        for _ in range(3):
-            ooc.lib().oocdag_memcpy_d2d(dst.ptr, src.ptr, nbytes)
-        ooc.lib().oocdag_device_sync()
-        log(f"[bench] copy calibration: 3 device copies of {nbytes} bytes")
-        src.free()
-        dst.free()
-
     n, nb = args.n, args.nb
     flops = n ** 3 / 3.0
-    log(f"[bench] n={n} nb={nb}: allocating {ooc.tiles_elems(n, nb) * 8 / 1e9:.1f} GB of tiles")
-    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
-    ooc.set_progress(16 if n >= 65536 else 0)
+    # kept back for the work after the timed steps: the small correctness
+    # check, the CPU baseline and the full residual when asked for
+    reserve = 15.0 + (0 if args.no_verify else 10.0) + (0 if args.no_cpu else 15.0) + \
+        (60.0 if args.verify_full else 0.0)
+    budget = Budget(args.budget_s, reserve)
 
-    for w in range(args.warmup):
-        ooc.pdmatgen_tiles(buf, n, nb)
-        info = ooc.potrf_tiles(buf, n, nb)
-        log(f"[bench] warmup {w + 1}/{args.warmup} done, info={info}")
-
-    times = []
-    prof_t = prof_f = 0.0
-    prof_n = 0
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        ooc.pdmatgen_tiles(buf, n, nb)   # synchronous; outside the timed region
-        log(f"[bench] step {k + 1}: generated in {time.perf_counter() - t0:.2f} s")
-        ooc.lib().oocdag_device_sync()
-        ooc.profile_enable(True)
-        t1 = time.perf_counter()
-        info = ooc.potrf_tiles(buf, n, nb)  # returns after the device finished
-        ooc.lib().oocdag_device_sync()
-        t2 = time.perf_counter()
-        p = ooc.profile_read()
-        ooc.profile_enable(False)
-        if info != 0:
-            log(f"[bench] factorization reported info={info}")
-            return 1
-        times.append(t2 - t1)
-        prof_t += p["seconds"]
-        prof_f += p["flops"]
-        prof_n += p["launches"]
-        log(f"[bench] step {k + 1}: {t2 - t1:.3f} s, {flops / (t2 - t1) / 1e12:.2f} TFLOP/s; "
-            f"update GEMM {p['launches']} launches {p['seconds']:.3f} s "
-            f"{p['flops'] / max(p['seconds'], 1e-12) / 1e12:.2f} TFLOP/s")
-    full_resid = None
-    if args.verify_full:
-        # LIN scaled residual of the timed factor itself (L L^T rebuilt on the
-        # device against the regenerated PDMATGEN matrix), outside the clock
-        t0 = time.perf_counter()
-        full_resid = ooc.llt_resid_tiles(buf, n, nb)
-        log(f"[bench] full-size scaled residual {full_resid:.4e} ({time.perf_counter() - t0:.1f} s)")
-    A_last = float("nan")
-    buf.free()
+    if args.mode == "ooc":
+        res = bench_ooc(ooc, n, nb, args.budget_gib, max(1, args.steps), max(0, args.warmup),
+                        budget, args.verify_full)
+        times = res["step_seconds"]
+        steps_done, warm_done = res["steps_done"], res["warmup_done"]
+        traffic, traffic_src = measured_traffic(
+            f"update_gemm_ooc_n{n}_budget{args.budget_gib:g}g_pmc_traffic.md",
+            {"n": n, "nb": nb, "budget_gib": f"{args.budget_gib:g}"})
+        workload = (f"FP64 out-of-core tiled Cholesky n={n} nb={nb}: PDMATGEN('S','D',100) in "
+                    f"pageable host memory as packed lower tiles "
+                    f"({ooc.tiles_elems(n, nb) * 8 / 1e9:.1f} GB), device budget "
+                    f"{args.budget_gib:g} GiB (panels {res['panels']}), column panels streamed "
+                    f"over PCIe with double-buffered copies; timed PCIe-inclusive")
+        extra = {"panels": res["panels"], "budget_bytes": res["budget_bytes"],
+                 "h2d_GB_per_step": round(res["h2d_bytes_per_step"] / 1e9, 2),
+                 "d2h_GB_per_step": round(res["d2h_bytes_per_step"] / 1e9, 2),
+                 "h2d_GBps": round(res["h2d_GBps"], 2), "d2h_GBps": round(res["d2h_GBps"], 2),
+                 "generate_seconds": [round(x, 2) for x in res["generate_seconds"]],
+                 "L_last_diag": res["L_last_diag"]}
+        full_resid = res["scaled_residual_full"]
+        data = "synthetic: PDMATGEN('S','D', ISEED=100) generated on the GPU into host tiles"
+    else:
+        log(f"[bench] n={n} nb={nb}: allocating {ooc.tiles_elems(n, nb) * 8 / 1e9:.1f} GB of "
+            f"tiles in HBM")
+        buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+        ooc.set_progress(32 if n >= 65536 else 0)
+        step_cost = [0.0]
+        warm_done = 0
+        for w in range(max(0, args.warmup)):
+            if w >= 1 and not budget.fits((args.steps + 1) * step_cost[0]):
+                break
+            t0 = time.perf_counter()
+            ooc.pdmatgen_tiles(buf, n, nb)
+            info = ooc.potrf_tiles(buf, n, nb)
+            step_cost[0] = max(step_cost[0], time.perf_counter() - t0)
+            warm_done += 1
+            log(f"[bench] warmup {warm_done} done, info={info}")
+        times = []
+        prof = {"launches": 0, "seconds": 0.0, "flops": 0.0, "bytes": 0.0}
+        for k in range(max(1, args.steps)):
+            if k >= 1 and not budget.fits(step_cost[0]):
+                break
+            t0 = time.perf_counter()
+            ooc.pdmatgen_tiles(buf, n, nb)   # synchronous; outside the timed region
+            ooc.lib().oocdag_device_sync()
+            ooc.profile_enable(True)
+            t1 = time.perf_counter()
+            info = ooc.potrf_tiles(buf, n, nb)  # returns after the device finished
+            ooc.lib().oocdag_device_sync()
+            t2 = time.perf_counter()
+            p = ooc.profile_read()
+            ooc.profile_enable(False)
+            if info != 0:
+                log(f"[bench] factorization reported info={info}")
+                return 1
+            times.append(t2 - t1)
+            step_cost[0] = max(step_cost[0], t2 - t0)
+            for key in prof:
+                prof[key] += p[key]
+            log(f"[bench] step {k + 1}: {t2 - t1:.3f} s, {flops / (t2 - t1) / 1e12:.2f} TFLOP/s; "
+                f"update GEMM {p['launches']} launches {p['seconds']:.3f} s")
+        full_resid = None
+        if args.verify_full:
+            full_resid = ooc.llt_resid_tiles(buf, n, nb)
+            log(f"[bench] full-size scaled residual {full_resid:.4e}")
+        buf.free()
+        res = {"prof_launches": prof["launches"], "prof_seconds": prof["seconds"],
+               "prof_flops": prof["flops"], "prof_bytes": prof["bytes"]}
+        steps_done = len(times)
+        traffic, traffic_src = measured_traffic(f"update_gemm_n{n}_pmc_traffic.md", {"nb": nb})
+        workload = f"FP64 HBM-resident tiled Cholesky n={n} nb={nb}"
+        extra = {}
+        data = "synthetic: PDMATGEN('S','D', ISEED=100) generated on the GPU into HBM"
 
     t = sum(times) / len(times)
     value = flops / t / 1e12
-    ach = prof_f / max(prof_t, 1e-12) / 1e12
-    traffic, traffic_note = measured_traffic(n, nb)
-    resid = None if args.no_verify else verify_small(ooc)
+    ach = res["prof_flops"] / max(res["prof_seconds"], 1e-12) / 1e12
+    alg_bytes = res["prof_bytes"] / max(res["prof_launches"], 1)
+    resid = None if args.no_verify else verify_small_ooc(ooc)
+    if resid is not None:
+        log(f"[bench] small out-of-core check: scaled residual {resid:.3e}")
     rec = {
         "metric": "Cholesky TFLOP/s (n^3/3) vs FP64 peak",
         "value": round(value, 4),
         "unit": "TFLOP/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps_done,
+        "warmup": warm_done,
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
         "ms_per_step": round(t * 1e3, 2),
+        "step_ms": [round(x * 1e3, 1) for x in times],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: PDMATGEN('S','D', ISEED=100) generated on the GPU into HBM",
-        "config": {"workload": f"FP64 HBM-resident tiled Cholesky n={n} nb={nb}",
-                   "n": n, "nb": nb, "fraction_of_fp64_peak": round(value / FP64_PEAK, 4),
-                   "small_check_scaled_residual_n4096": resid,
-                   "scaled_residual_full": full_resid},
+        "data": data,
+        "config": {"workload": workload, "n": n, "nb": nb,
+                   "fraction_of_fp64_peak": round(value / FP64_PEAK, 4),
+                   "small_check_scaled_residual": resid,
+                   "scaled_residual_full": full_resid, **extra},
         "roofline": {"bound": "mfma", "achieved": round(ach, 4), "peak": FP64_PEAK,
                      "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK, 4), "traffic": traffic,
-                     "traffic_source": traffic_note,
-                     "algorithmic_bytes_per_launch": update_algorithmic_bytes(n, nb),
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": round(alg_bytes),
                      "kernel": "k_dgemm_dtv<1,0,true> panel update",
-                     "launches": prof_n, "kernel_seconds": round(prof_t, 4)},
+                     "launches": res["prof_launches"],
+                     "kernel_seconds": round(res["prof_seconds"], 4),
+                     "avg_launch_ms": round(res["prof_seconds"] / max(res["prof_launches"], 1) * 1e3,
+                                            3)},
     }
-    del A_last
     if not args.no_cpu:
-        log(f"[bench] cpu baseline n={args.cpu_n} ...")
-        rec["cpu_baseline"] = cpu_baseline(args.cpu_n, 512)
+        log("[bench] cpu baseline (n=4096, nb=256, 28 MiB budget) ...")
+        rec["cpu_baseline"] = cpu_baseline()
+    rec["wall_s"] = round(elapsed(), 1)
     print(json.dumps(rec), flush=True)
     return 0
 

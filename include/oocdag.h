@@ -11,9 +11,12 @@
  *               triangle used (the reference's test_A, oocdag.cpp:20,251).
  *   tiles       "packed lower tiles": the reference's device panel Y
  *               (oocquark.cpp:256) for the whole matrix: tile (i,j), i >= j,
- *               of edge NB at element offset (i + j*bb - j*(j+1)/2) * NB*NB,
- *               each tile column-major with ld = NB, bb = ceil(N/NB).  Rows and
- *               columns past N are padding (0, with 1 on the diagonal).
+ *               of edge NB at element offset (i + j*bb - j*(j+1)/2) * TS*TS,
+ *               each tile column-major with ld = TS, bb = ceil(N/NB), where
+ *               TS = oocdag_tile_edge(NB) is NB rounded up to a multiple of
+ *               64 (TS = NB when NB % 64 == 0).  Rows and columns past N, and
+ *               past NB inside a TS x TS slot, are padding (0, with 1 on the
+ *               diagonal of a diagonal tile).
  */
 #ifndef OOCDAG_H
 #define OOCDAG_H
@@ -39,17 +42,30 @@ extern "C" {
  * own HIP streams).  The lower triangle of A is overwritten with L, the strict
  * upper triangle is untouched.  memsize is the device budget in bytes; the
  * panel widths follow the reference planner (find_Yn, oocdag.cpp:478-492)
- * with room for two staged X tile columns instead of one.  Returns the
- * wall-clock seconds of the factorization (oocdag.cpp:51,90,110), or a
- * negative value on error.  *info: 0 success; k > 0 the leading minor of
- * order k is not positive definite (LAPACK dpotrf convention; the reference
- * only printed it, oocquark.cpp:95); < 0 an OOCDAG_E* code.  NB must be a
- * multiple of 64.  The out-of-core path drives four HIP streams; run the
+ * with room for two staged X tile columns instead of one when the budget
+ * allows it (oocdag_cholesky_nx with nx = 1 runs the reference's schedule
+ * exactly).  Returns the wall-clock seconds of the factorization
+ * (oocdag.cpp:51,90,110), or a negative value on error.  *info: 0 success;
+ * k > 0 the leading minor of order k is not positive definite (LAPACK
+ * dpotrf convention; the reference only printed it, oocquark.cpp:95); < 0 an
+ * OOCDAG_E* code.  Any NB >= 1, as the reference takes: the tile grid,
+ * memBlock = memsize / (8 NB^2) tiles and the panel schedule are the
+ * reference's; a tile whose edge is not a multiple of 64 is held on the
+ * device in a slot of edge oocdag_tile_edge(NB) (NB rounded up to 64,
+ * padded with an identity block), so the device footprint is
+ * memBlock * 8 * oocdag_tile_edge(NB)^2 bytes.  The out-of-core path drives
+ * four HIP streams; run the
  * process with GPU_MAX_HW_QUEUES >= 5 (HIP's default is 4) so that none of
  * them shares a hardware queue -- shared, a finished column's copy to the
  * host holds back the next update (about 4 % at n = 196608).
  * ------------------------------------------------------------------------- */
 double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *info);
+/* The same with the number of staged X tile columns fixed: nx = 1 is the
+ * reference's schedule (find_Yn, oocdag.cpp:478-492: one X column, the same
+ * panels and A2Y/A2X copy volume, oocquark.cpp:253-319), nx = 2 double
+ * buffers X so each copy overlaps the previous update, nx = 0 is the
+ * default (2 when the budget leaves a panel column beside two X columns). */
+double oocdag_cholesky_nx(double *A, int N, int NB, int LDA, size_t memsize, int nx, int *info);
 
 /* ---------------------------------------------------------------------------
  * Factorization, host packed tiles in / out (out of core).
@@ -66,6 +82,8 @@ double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *i
  *   overwritten with L.
  * ------------------------------------------------------------------------- */
 double oocdag_cholesky_host_tiles(double *T, int N, int NB, size_t memsize, int *info);
+double oocdag_cholesky_host_tiles_nx(double *T, int N, int NB, size_t memsize, int nx,
+                                     int *info);
 
 /* PDMATGEN('S'/'H'/'F', DIAG, ISEED) straight into host packed lower tiles
  * (the layout above), generated on the GPU tile column group by group and
@@ -112,6 +130,9 @@ int oocdag_potrf_tiles(double *dY, int N, int NB, void *stream, int *info);
 
 /* Number of doubles of the packed lower tile storage for (N, NB). */
 size_t oocdag_tiles_elems(int N, int NB);
+/* Edge TS of the slot that holds one NB x NB tile (NB rounded up to a
+ * multiple of 64), or OOCDAG_EARG for NB < 1. */
+int oocdag_tile_edge(int NB);
 
 /* PDMATGEN('S'|'H'|'F', DIAG, ISEED) written straight into packed lower tiles
  * in HBM (values identical to oocdag_pdmatgen; padding as described above). */
@@ -291,9 +312,11 @@ int oocdag_dgemm_nt(int m, int n, int k, double alpha, const double *A, int lda,
                     const double *B, int ldb, double beta, double *C, int ldc, void *stream);
 int oocdag_dsyrk_ln(int n, int k, double alpha, const double *A, int lda, double beta,
                     double *C, int ldc, void *stream);
-/* B(m x n) <- B * L^-T, L n x n lower.  n % 64 == 0. */
+/* B(m x n) <- B * L^-T, L n x n lower.  Any n; n % 64 == 0 with even
+ * leading dimensions runs in place, other shapes through padded copies. */
 int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb, void *stream);
-/* In-place lower Cholesky of an n x n matrix, n % 64 == 0; strict upper
+/* In-place lower Cholesky of an n x n matrix, any n (n % 64 == 0 with an
+ * even lda in place, otherwise through a padded copy); strict upper
  * triangle untouched.  Synchronous; *info LAPACK-style. */
 int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream);
 
@@ -326,6 +349,10 @@ int oocdag_last_stats(double *seconds, double *h2d_bytes, double *d2h_bytes,
  * element) since enable.  Process-wide, not thread-safe. */
 int oocdag_profile_enable(int on);
 int oocdag_profile_read(long long *launches, double *seconds, double *flops);
+/* Algorithmic HBM bytes of the same launches since enable: every C element
+ * of the lower trapezoid read and written once (16 B) plus the K-deep
+ * operand strips read once (8 B per element); the roofline's byte count. */
+int oocdag_profile_bytes(double *bytes);
 
 /* Progress lines on stderr every this many factored tile columns (0 = off),
  * the successor of the reference's per-panel printf (oocdag.cpp:67). */

@@ -42,6 +42,10 @@ _lib = None
 SIGNATURES = {
     "oocdag_cholesky": (C.c_double, [_dp, C.c_int, C.c_int, C.c_int, C.c_size_t, _ip]),
     "oocdag_cholesky_host_tiles": (C.c_double, [C.c_void_p, C.c_int, C.c_int, C.c_size_t, _ip]),
+    "oocdag_cholesky_nx": (C.c_double, [_dp, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_int, _ip]),
+    "oocdag_cholesky_host_tiles_nx": (C.c_double, [C.c_void_p, C.c_int, C.c_int, C.c_size_t,
+                                                   C.c_int, _ip]),
+    "oocdag_tile_edge": (C.c_int, [C.c_int]),
     "oocdag_pdmatgen_host_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char, C.c_char,
                                              C.c_int]),
     "oocdag_host_alloc": (C.c_void_p, [C.c_size_t]),
@@ -82,6 +86,7 @@ SIGNATURES = {
     "oocdag_last_stats": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
     "oocdag_profile_enable": (C.c_int, [C.c_int]),
     "oocdag_profile_read": (C.c_int, [C.POINTER(C.c_longlong), _dp, _dp]),
+    "oocdag_profile_bytes": (C.c_int, [_dp]),
     "oocdag_set_progress": (C.c_int, [C.c_int]),
 }
 
@@ -196,14 +201,18 @@ def device_count() -> int:
 # reference entry points
 # ---------------------------------------------------------------------------
 
-def cholesky(A: np.ndarray, nb: int, memsize: int):
+def cholesky(A: np.ndarray, nb: int, memsize: int, nx: int | None = None):
     """Cholesky(A, N, NB, LDA, memsize) (oocdag.cpp:18): factors A in place.
 
     Returns (seconds, info) like the reference returns seconds; info follows
-    LAPACK dpotrf (k > 0: leading minor k not positive definite)."""
+    LAPACK dpotrf (k > 0: leading minor k not positive definite).  nx fixes
+    the staged X tile columns (1: the reference's find_Yn schedule)."""
     info = C.c_int(0)
     n = A.shape[1]
-    t = lib().oocdag_cholesky(_ptr(A), n, nb, A.shape[0], memsize, C.byref(info))
+    if nx is None:
+        t = lib().oocdag_cholesky(_ptr(A), n, nb, A.shape[0], memsize, C.byref(info))
+    else:
+        t = lib().oocdag_cholesky_nx(_ptr(A), n, nb, A.shape[0], memsize, nx, C.byref(info))
     if info.value < 0:
         raise OocdagError(f"oocdag_cholesky failed ({info.value}): {last_error()}")
     return t, info.value
@@ -252,25 +261,31 @@ class HostTiles:
             pass
 
 
-def cholesky_host_tiles(t: "HostTiles", memsize: int):
+def cholesky_host_tiles(t: "HostTiles", memsize: int, nx: int = 0):
     """Out-of-core factorization of host packed tiles in place
     (Cholesky(), oocdag.cpp:18, on the tile layout).  Returns (seconds, info)."""
     info = C.c_int(0)
-    secs = lib().oocdag_cholesky_host_tiles(t.ptr, t.n, t.nb, memsize, C.byref(info))
+    secs = lib().oocdag_cholesky_host_tiles_nx(t.ptr, t.n, t.nb, memsize, nx, C.byref(info))
     if info.value < 0:
         raise OocdagError(f"oocdag_cholesky_host_tiles failed ({info.value}): {last_error()}")
     return secs, info.value
 
 
+def tile_edge(nb: int) -> int:
+    """Edge of the slot that holds one nb x nb tile (nb rounded up to 64)."""
+    return int(lib().oocdag_tile_edge(nb))
+
+
 def tiles_to_dense(tiles: np.ndarray, n: int, nb: int) -> np.ndarray:
     """Lower triangle of the matrix held in packed tiles, as a dense array."""
     bb = -(-n // nb)
+    ts = -(-nb // 64) * 64
     out = np.zeros((bb * nb, bb * nb), order="F")
     k = 0
     for j in range(bb):
         for i in range(j, bb):
             out[i * nb:(i + 1) * nb, j * nb:(j + 1) * nb] = \
-                tiles[k * nb * nb:(k + 1) * nb * nb].reshape(nb, nb, order="F")
+                tiles[k * ts * ts:(k + 1) * ts * ts].reshape(ts, ts, order="F")[:nb, :nb]
             k += 1
     return np.tril(out[:n, :n])
 
@@ -326,8 +341,10 @@ def profile_enable(on: bool = True) -> None:
 
 def profile_read() -> dict:
     n, t, f = C.c_longlong(0), C.c_double(0), C.c_double(0)
+    b = C.c_double(0)
     lib().oocdag_profile_read(C.byref(n), C.byref(t), C.byref(f))
-    return {"launches": n.value, "seconds": t.value, "flops": f.value}
+    lib().oocdag_profile_bytes(C.byref(b))
+    return {"launches": n.value, "seconds": t.value, "flops": f.value, "bytes": b.value}
 
 
 def set_progress(every_columns: int) -> None:

@@ -51,8 +51,9 @@ struct UpdateProfile {
     bool on = false;
     std::vector<hipEvent_t> ev;   // start, end pairs
     std::vector<double> flops;    // algorithmic flops of each launch
+    std::vector<double> bytes;    // algorithmic HBM bytes of each launch
     long long launches = 0;
-    double seconds = 0, total_flops = 0;
+    double seconds = 0, total_flops = 0, total_bytes = 0;
 } g_prof;
 
 int g_progress_every = 0;  // print a line every this many factored tile columns
@@ -61,6 +62,18 @@ int g_progress_every = 0;  // print a line every this many factored tile columns
 // region on its top-left corner), lower part only, K-deep.
 double trap_update_flops(long long M, long long N, long long K) {
     return 2.0 * (double)K * ((double)M * N - 0.5 * (double)N * (N - 1));
+}
+
+// Algorithmic HBM bytes of a lower-trapezoid update: the C elements read
+// and written once (16 B each) and the K-deep operand strips read once
+// (8 B per element; B is the top rows of A when both describe the same
+// operand, as in every update of the factorization).
+double trap_update_bytes(const Gemm &g) {
+    const double M = g.M, N = g.N, K = g.K;
+    const double c = M * N - 0.5 * N * (N - 1);
+    const bool b_in_a = g.B.p == g.A.p && g.B.r0 == g.A.r0 && g.B.c0 == g.A.c0 &&
+                        g.B.ts == g.A.ts && g.B.ym == g.A.ym && g.B.tri == g.A.tri;
+    return 16.0 * c + 8.0 * M * K + (b_in_a ? 0.0 : 8.0 * N * K);
 }
 
 hipError_t launch_update(const Gemm &g, hipStream_t s) {
@@ -75,6 +88,7 @@ hipError_t launch_update(const Gemm &g, hipStream_t s) {
     g_prof.ev.push_back(a);
     g_prof.ev.push_back(b);
     g_prof.flops.push_back(trap_update_flops(g.M, g.N, g.K));
+    g_prof.bytes.push_back(trap_update_bytes(g));
     return hipSuccess;
 }
 
@@ -85,6 +99,7 @@ void profile_collect() {
         if (hipEventElapsedTime(&ms, g_prof.ev[i], g_prof.ev[i + 1]) == hipSuccess) {
             g_prof.seconds += ms * 1e-3;
             g_prof.total_flops += g_prof.flops[i / 2];
+            g_prof.total_bytes += g_prof.bytes[i / 2];
             g_prof.launches += 1;
         }
         (void)hipEventDestroy(g_prof.ev[i]);
@@ -92,6 +107,7 @@ void profile_collect() {
     }
     g_prof.ev.clear();
     g_prof.flops.clear();
+    g_prof.bytes.clear();
 }
 
 void set_err(const char *fmt, ...) {
@@ -188,38 +204,42 @@ bool make_plan(int N, int NB, size_t memsize, Plan &pl, int force_nx = 0) {
 // host packing between column-major A (LDA) and NB x NB tiles
 // ---------------------------------------------------------------------------
 
-// Packs tiles (ti0 .. bb-1, tj) of A into dst (consecutive NB^2 tiles),
-// zero-padding past N and putting 1 on the padded diagonal of tile (tj,tj).
+// Packs tiles (ti0 .. bb-1, tj) of A into dst: consecutive slots of edge
+// ts = tile_edge(NB) (ld = ts), each holding an NB x NB tile of A, padded
+// past N and past NB inside the slot with 0 and a 1 on the diagonal of tile
+// (tj, tj).
 void pack_column(const double *A, int N, int NB, int LDA, int tj, int ti0, double *dst,
                  int nth = 0) {
     const int bb = (N + NB - 1) / NB;
+    const int ts = tile_edge(NB);
     const int ntile = bb - ti0;
-    const size_t nb2 = (size_t)NB * NB;
+    const size_t ts2 = (size_t)ts * ts;
     if (nth <= 0) nth = omp_get_max_threads();
 #pragma omp parallel for schedule(static) collapse(2) num_threads(nth)
     for (int t = 0; t < ntile; ++t) {
-        for (int c = 0; c < NB; ++c) {
+        for (int c = 0; c < ts; ++c) {
             const int ti = ti0 + t;
             const int gr0 = ti * NB, gc = tj * NB + c;
-            double *d = dst + (size_t)t * nb2 + (size_t)c * NB;
+            double *d = dst + (size_t)t * ts2 + (size_t)c * ts;
             const int vr = std::max(0, std::min(NB, N - gr0));
-            if (gc < N) {
+            if (c < NB && gc < N) {
                 if (vr > 0) memcpy(d, A + (size_t)gc * LDA + gr0, (size_t)vr * sizeof(double));
-                for (int r = vr; r < NB; ++r) d[r] = 0.0;
+                for (int r = vr; r < ts; ++r) d[r] = 0.0;
             } else {
-                for (int r = 0; r < NB; ++r) d[r] = (gr0 + r == gc) ? 1.0 : 0.0;
+                for (int r = 0; r < ts; ++r) d[r] = (ti == tj && r == c) ? 1.0 : 0.0;
             }
         }
     }
 }
 
-// Writes tiles (ti0 .. bb-1, tj) from src back into A: valid rows/columns
-// only, and only the lower triangle of the diagonal tile.
+// Writes tiles (ti0 .. bb-1, tj) from slots src back into A: valid rows and
+// columns only, and only the lower triangle of the diagonal tile.
 void unpack_column(double *A, int N, int NB, int LDA, int tj, int ti0, const double *src,
                    int nth = 0) {
     const int bb = (N + NB - 1) / NB;
+    const int ts = tile_edge(NB);
     const int ntile = bb - ti0;
-    const size_t nb2 = (size_t)NB * NB;
+    const size_t ts2 = (size_t)ts * ts;
     if (nth <= 0) nth = omp_get_max_threads();
 #pragma omp parallel for schedule(static) collapse(2) num_threads(nth)
     for (int t = 0; t < ntile; ++t) {
@@ -227,7 +247,7 @@ void unpack_column(double *A, int N, int NB, int LDA, int tj, int ti0, const dou
             const int ti = ti0 + t;
             const int gr0 = ti * NB, gc = tj * NB + c;
             if (gc >= N) continue;
-            const double *s = src + (size_t)t * nb2 + (size_t)c * NB;
+            const double *s = src + (size_t)t * ts2 + (size_t)c * ts;
             int r0 = (ti == tj) ? c : 0;
             int vr = std::max(0, std::min(NB, N - gr0));
             if (vr > r0)
@@ -426,10 +446,18 @@ int factor_panel(double *dY, int NB, int Ym, int Yn, int gcol0, double *dinv, in
     return rc;
 }
 
-int read_info(int *dinfo, hipStream_t s, int N, int *info) {
+// The first failed pivot (1-based column of the padded device matrix, or
+// INT_MAX) as the LAPACK info of the N x N matrix: slot column h of tile
+// column t is matrix column t*NB + h (NB = 0: no slots, the index as is).
+int read_info(int *dinfo, hipStream_t s, int N, int *info, int NB = 0) {
     int h = INT_MAX;
     HIPTRY(hipMemcpyAsync(&h, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
+    if (h != INT_MAX && NB > 0) {
+        const int ts = tile_edge(NB);
+        const long long t = (h - 1) / ts, off = (h - 1) % ts;
+        h = (int)std::min<long long>(INT_MAX - 1, t * NB + std::min<long long>(off, NB - 1) + 1);
+    }
     *info = (h == INT_MAX) ? 0 : std::min(h, N);
     return OOCDAG_OK;
 }
@@ -513,14 +541,17 @@ int last_overlapping_column(int Ym, int Yn, int Ym2, int c) {
 //      the GPU) for the events that mark the overlapped columns of this panel
 //      final, read by every update and copied out.
 int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, double *secs,
-                 bool tiles_host = false) {
+                 bool tiles_host = false, int force_nx = 0) {
     Plan pl;
-    if (!make_plan(N, NB, memsize, pl)) {
+    if (!make_plan(N, NB, memsize, pl, force_nx)) {
         set_err("budget %zu bytes below two tile columns for N=%d NB=%d", memsize, N, NB);
         return OOCDAG_EBUDGET;
     }
+    // the reference's tile grid and memBlock (make_plan); on the device and
+    // in the staging slots each tile sits in an NBs x NBs slot
     const int bb = (N + NB - 1) / NB;
-    const size_t nb2 = (size_t)NB * NB;
+    const int NBs = tile_edge(NB);
+    const size_t nb2 = (size_t)NBs * NBs;
     const size_t col_bytes = (size_t)bb * nb2 * sizeof(double);
     constexpr int ND = 3;   // device->host slots
 
@@ -543,7 +574,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
         ~FreeDev() { (void)hipFree(p); }
     } fy{dY};
     Scratch sc;
-    int rc = sc.init(NB, st.comp);
+    int rc = sc.init(NBs, st.comp);
     if (rc) return rc;
 
     // pinned slots: 2 for host->device, ND for device->host
@@ -659,12 +690,12 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             HIPTRY(hipStreamWaitEvent(st.comp, ev_xready[b], 0));
             // ooc_syrk(X, Y, YM, YN, NB) (oocquark.cpp:321-356) as one launch
             Gemm g;
-            g.A = panel(X, NB, Ym, 0);
+            g.A = panel(X, NBs, Ym, 0);
             g.B = g.A;
-            g.C = panel(dY, NB, Ym, 1);
-            g.M = Ym * NB;
-            g.N = Yn * NB;
-            g.K = NB;
+            g.C = panel(dY, NBs, Ym, 1);
+            g.M = Ym * NBs;
+            g.N = Yn * NBs;
+            g.K = NBs;
             g.alpha = -1.0;
             g.beta = 1.0;
             g.lower = 1;
@@ -757,18 +788,18 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
         };
         if (w0 == 0) {
             post_prefetch();
-            rc = factor_panel(dY, NB, Ym, Yn, (int)jb * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
+            rc = factor_panel(dY, NBs, Ym, Yn, (int)jb * NBs, sc.dinv, sc.dinfo, st.comp, st.bulk,
                               on_column, &after_updates);
             if (rc) return rc;
         } else {
             // columns [0, w0) left-looking, each as soon as it is in HBM:
-            // its update from the columns before it (one K = c*NB launch),
+            // its update from the columns before it (one K = c*NBs launch),
             // then its blocked factorization; then each later column's
-            // update from columns [0, w0) (one K = w0*NB launch) on arrival;
+            // update from columns [0, w0) (one K = w0*NBs launch) on arrival;
             // then columns [w0, Yn): the trailing part of a packed panel is
             // itself a packed panel of Ym - w0 tile rows at
             // tile_index(w0, w0, Ym)
-            const Opnd P0 = panel(dY, NB, Ym, 1);
+            const Opnd P0 = panel(dY, NBs, Ym, 1);
             for (int c = 0; c < w0; ++c) {
                 if ((rc = h2d_column(dY + (size_t)tile_index(c, c, Ym, 1) * nb2, (int)jb + c,
                                      (int)jb + c, nth, t_h2d, no_wait)))
@@ -777,18 +808,18 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
                 HIPTRY(hipStreamWaitEvent(st.comp, ev_y, 0));
                 if (c > 0) {
                     Gemm g;
-                    g.A = P0; g.A.r0 = c * NB; g.A.c0 = 0;
+                    g.A = P0; g.A.r0 = c * NBs; g.A.c0 = 0;
                     g.B = g.A;
-                    g.C = P0; g.C.r0 = c * NB; g.C.c0 = c * NB;
-                    g.M = (Ym - c) * NB;
-                    g.N = NB;
-                    g.K = c * NB;
+                    g.C = P0; g.C.r0 = c * NBs; g.C.c0 = c * NBs;
+                    g.M = (Ym - c) * NBs;
+                    g.N = NBs;
+                    g.K = c * NBs;
                     g.alpha = -1.0;
                     g.beta = 1.0;
                     g.lower = 1;
                     HIPTRY(launch_update(g, st.comp));
                 }
-                HIPTRY(factor_column(P0, c * NB, (Ym - c) * NB, NB, ((int)jb + c) * NB, sc.dinv,
+                HIPTRY(factor_column(P0, c * NBs, (Ym - c) * NBs, NBs, ((int)jb + c) * NBs, sc.dinv,
                                      sc.dinfo, st.comp));
                 if ((rc = on_column(c))) return rc;
             }
@@ -798,14 +829,14 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
                     return rc;
                 HIPTRY(hipEventRecord(ev_y, st.h2d));
                 HIPTRY(hipStreamWaitEvent(st.comp, ev_y, 0));
-                const Opnd P = panel(dY, NB, Ym, 1);
+                const Opnd P = panel(dY, NBs, Ym, 1);
                 Gemm g;
-                g.A = P; g.A.r0 = c * NB; g.A.c0 = 0;
+                g.A = P; g.A.r0 = c * NBs; g.A.c0 = 0;
                 g.B = g.A;
-                g.C = P; g.C.r0 = c * NB; g.C.c0 = c * NB;
-                g.M = (Ym - c) * NB;
-                g.N = NB;
-                g.K = w0 * NB;
+                g.C = P; g.C.r0 = c * NBs; g.C.c0 = c * NBs;
+                g.M = (Ym - c) * NBs;
+                g.N = NBs;
+                g.K = w0 * NBs;
                 g.alpha = -1.0;
                 g.beta = 1.0;
                 g.lower = 1;
@@ -822,8 +853,8 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
             const std::function<int(int)> after_updates_b = [&](int jj) {
                 return after_updates(jj + w0);
             };
-            rc = factor_panel(dY + (size_t)tile_index(w0, w0, Ym, 1) * nb2, NB, Ym - w0, Yn - w0,
-                              ((int)jb + w0) * NB, sc.dinv, sc.dinfo, st.comp, st.bulk,
+            rc = factor_panel(dY + (size_t)tile_index(w0, w0, Ym, 1) * nb2, NBs, Ym - w0, Yn - w0,
+                              ((int)jb + w0) * NBs, sc.dinv, sc.dinfo, st.comp, st.bulk,
                               on_column_b, &after_updates_b);
             if (rc) return rc;
         }
@@ -835,7 +866,7 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     }
     if ((rc = loader.drain())) return rc;
     HIPTRY(hipStreamSynchronize(st.h2d));
-    rc = read_info(sc.dinfo, st.comp, N, info);
+    rc = read_info(sc.dinfo, st.comp, N, info, NB);
     *secs = now() - t0;
     profile_collect();
     g_stat[0] = *secs;
@@ -846,9 +877,11 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     return rc;
 }
 
+// Any NB >= 1, as Cholesky() takes (oocdag.cpp:241-247); a tile whose edge
+// is not a multiple of 64 is stored in a padded slot (tile_edge).
 int check_nb(int N, int NB) {
-    if (N < 0 || NB < 64 || NB % 64 != 0) {
-        set_err("N=%d NB=%d: NB must be a positive multiple of 64", N, NB);
+    if (N < 0 || NB < 1 || NB > (1 << 16)) {
+        set_err("N=%d NB=%d: NB must be in [1, 65536]", N, NB);
         return OOCDAG_EARG;
     }
     return OOCDAG_OK;
@@ -946,13 +979,13 @@ int oocdag_panel_plan_nx(int N, int NB, size_t memsize, int nx, int *widths, int
     return (int)pl.width.size();
 }
 
-double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *info) {
+double oocdag_cholesky_nx(double *A, int N, int NB, int LDA, size_t memsize, int nx, int *info) {
     g_err.clear();
     for (double &s : g_stat) s = 0;
     int dummy;
     if (!info) info = &dummy;
     *info = 0;
-    if (!A || N < 0 || LDA < std::max(1, N) || check_nb(N, NB)) {
+    if (!A || N < 0 || LDA < std::max(1, N) || nx < 0 || nx > 2 || check_nb(N, NB)) {
         if (g_err.empty()) set_err("invalid argument");
         *info = OOCDAG_EARG;
         return -1.0;
@@ -963,7 +996,37 @@ double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *i
         return -1.0;
     }
     double secs = 0;
-    int rc = cholesky_ooc(A, N, NB, LDA, memsize, info, &secs);
+    int rc = cholesky_ooc(A, N, NB, LDA, memsize, info, &secs, false, nx);
+    if (rc) {
+        *info = rc;
+        return -1.0;
+    }
+    return secs;
+}
+
+double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *info) {
+    return oocdag_cholesky_nx(A, N, NB, LDA, memsize, 0, info);
+}
+
+double oocdag_cholesky_host_tiles_nx(double *T, int N, int NB, size_t memsize, int nx,
+                                     int *info) {
+    g_err.clear();
+    for (double &s : g_stat) s = 0;
+    int dummy;
+    if (!info) info = &dummy;
+    *info = 0;
+    if (!T || N < 0 || nx < 0 || nx > 2 || check_nb(N, NB)) {
+        if (g_err.empty()) set_err("invalid argument");
+        *info = OOCDAG_EARG;
+        return -1.0;
+    }
+    if (N == 0) return 0.0;
+    if (!device_ok()) {
+        *info = OOCDAG_ENODEV;
+        return -1.0;
+    }
+    double secs = 0;
+    int rc = cholesky_ooc(T, N, NB, N, memsize, info, &secs, true, nx);
     if (rc) {
         *info = rc;
         return -1.0;
@@ -972,28 +1035,7 @@ double oocdag_cholesky(double *A, int N, int NB, int LDA, size_t memsize, int *i
 }
 
 double oocdag_cholesky_host_tiles(double *T, int N, int NB, size_t memsize, int *info) {
-    g_err.clear();
-    for (double &s : g_stat) s = 0;
-    int dummy;
-    if (!info) info = &dummy;
-    *info = 0;
-    if (!T || N < 0 || check_nb(N, NB)) {
-        if (g_err.empty()) set_err("invalid argument");
-        *info = OOCDAG_EARG;
-        return -1.0;
-    }
-    if (N == 0) return 0.0;
-    if (!device_ok()) {
-        *info = OOCDAG_ENODEV;
-        return -1.0;
-    }
-    double secs = 0;
-    int rc = cholesky_ooc(T, N, NB, N, memsize, info, &secs, true);
-    if (rc) {
-        *info = rc;
-        return -1.0;
-    }
-    return secs;
+    return oocdag_cholesky_host_tiles_nx(T, N, NB, memsize, 0, info);
 }
 
 int oocdag_pdmatgen_host_tiles(double *T, int N, int NB, char aform, char diag, int iseed) {
@@ -1006,32 +1048,44 @@ int oocdag_pdmatgen_host_tiles(double *T, int N, int NB, char aform, char diag, 
     }
     if (!device_ok()) return OOCDAG_ENODEV;
     const int bb = (N + NB - 1) / NB;
-    const size_t nb2 = (size_t)NB * NB;
-    // groups of tile columns up to ~4 GB, generated on the device into two
-    // alternating windows and copied down while the next group is generated
-    const size_t cap = std::max<size_t>((size_t)bb * nb2, ((size_t)4 << 30) / sizeof(double));
-    double *win[2] = {nullptr, nullptr};
+    const size_t nb2 = (size_t)tile_edge(NB) * tile_edge(NB);   // one slot
+    // groups of tile columns up to ~2 GB, generated on the device into two
+    // alternating windows and copied down while the next group is generated.
+    // Pinned T takes the copies directly; pageable T through two pinned
+    // staging buffers that the host empties with a parallel memcpy (the
+    // first touch of a large pageable array is then spread over all cores
+    // instead of the runtime's own staging thread).
+    const bool direct = host_pinned(T);
+    const size_t cap = std::max<size_t>((size_t)bb * nb2, ((size_t)2 << 30) / sizeof(double));
+    double *win[2] = {nullptr, nullptr}, *pin[2] = {nullptr, nullptr};
     hipStream_t s[2] = {nullptr, nullptr};
     struct Free {
-        double **w;
+        double **w, **p;
         hipStream_t *s;
         ~Free() {
             for (int i = 0; i < 2; ++i) {
                 if (s[i]) (void)hipStreamSynchronize(s[i]);
                 if (w[i]) (void)hipFree(w[i]);
+                if (p[i]) (void)hipHostFree(p[i]);
                 if (s[i]) (void)hipStreamDestroy(s[i]);
             }
         }
-    } fr{win, s};
+    } fr{win, pin, s};
     for (int i = 0; i < 2; ++i) {
         HIPTRY(hipStreamCreateWithFlags(&s[i], hipStreamNonBlocking));
         if (hipMalloc(&win[i], cap * sizeof(double)) != hipSuccess) {
             set_err("device allocation of %zu bytes failed", cap * sizeof(double));
             return OOCDAG_ENOMEM;
         }
+        if (!direct && hipHostMalloc(&pin[i], cap * sizeof(double), hipHostMallocDefault) !=
+                           hipSuccess) {
+            set_err("pinned host allocation of %zu bytes failed", cap * sizeof(double));
+            return OOCDAG_ENOMEM;
+        }
     }
-    int k = 0;
-    for (int j0 = 0; j0 < bb; ++k) {
+    // group g: tiles [first, first + count) of the packed layout
+    std::vector<long long> first, count;
+    for (int j0 = 0; j0 < bb;) {
         int j1 = j0;
         size_t tiles = 0;
         while (j1 < bb && (tiles + (size_t)(bb - j1)) * nb2 <= cap && j1 - j0 < 16) {
@@ -1039,11 +1093,28 @@ int oocdag_pdmatgen_host_tiles(double *T, int N, int NB, char aform, char diag, 
             ++j1;
         }
         const long long t0 = tile_index(j0, j0, bb, 1);
+        const int k = (int)first.size();
+        first.push_back(t0);
+        count.push_back((long long)tiles);
         hipStream_t sk = s[k & 1];
         HIPTRY(launch_gen_tiles(win[k & 1], N, NB, bb, j0, j1, aform, diag, iseed, sk, t0));
-        HIPTRY(hipMemcpyAsync(T + (size_t)t0 * nb2, win[k & 1], tiles * nb2 * sizeof(double),
+        double *dst = direct ? T + (size_t)t0 * nb2 : pin[k & 1];
+        HIPTRY(hipMemcpyAsync(dst, win[k & 1], tiles * nb2 * sizeof(double),
                               hipMemcpyDeviceToHost, sk));
+        if (!direct && k >= 1) {
+            // group k-1 has landed in its pinned buffer: move it into T while
+            // group k is generated; group k+1 reuses that buffer only after
+            // this copy (it is queued by this thread afterwards)
+            HIPTRY(hipStreamSynchronize(s[(k - 1) & 1]));
+            par_memcpy(T + (size_t)first[k - 1] * nb2, pin[(k - 1) & 1],
+                       (size_t)count[k - 1] * nb2);
+        }
         j0 = j1;
+    }
+    const int G = (int)first.size();
+    if (!direct && G >= 1) {
+        HIPTRY(hipStreamSynchronize(s[(G - 1) & 1]));
+        par_memcpy(T + (size_t)first[G - 1] * nb2, pin[(G - 1) & 1], (size_t)count[G - 1] * nb2);
     }
     HIPTRY(hipStreamSynchronize(s[0]));
     HIPTRY(hipStreamSynchronize(s[1]));
@@ -1070,8 +1141,10 @@ int oocdag_host_free(void *p) {
 size_t oocdag_tiles_elems(int N, int NB) {
     if (N <= 0 || NB <= 0) return 0;
     size_t bb = (size_t)((N + NB - 1) / NB);
-    return bb * (bb + 1) / 2 * (size_t)NB * NB;
+    return bb * (bb + 1) / 2 * (size_t)tile_edge(NB) * tile_edge(NB);
 }
+
+int oocdag_tile_edge(int NB) { return NB > 0 ? tile_edge(NB) : OOCDAG_EARG; }
 
 int oocdag_potrf_tiles(double *dY, int N, int NB, void *stream, int *info) {
     g_err.clear();
@@ -1082,6 +1155,7 @@ int oocdag_potrf_tiles(double *dY, int N, int NB, void *stream, int *info) {
     if (N == 0) return OOCDAG_OK;
     if (!device_ok()) return OOCDAG_ENODEV;
     const int bb = (N + NB - 1) / NB;
+    const int NBs = tile_edge(NB);
     Streams st;
     if (stream) {
         st.comp = (hipStream_t)stream;
@@ -1090,14 +1164,14 @@ int oocdag_potrf_tiles(double *dY, int N, int NB, void *stream, int *info) {
         HIPTRY(hipStreamCreateWithFlags(&st.comp, hipStreamNonBlocking));
     }
     Scratch sc;
-    int rc = sc.init(NB, st.comp);
+    int rc = sc.init(NBs, st.comp);
     if (rc) return rc;
     const double t0 = now();
     HIPTRY(hipStreamCreateWithFlags(&st.bulk, hipStreamNonBlocking));
-    rc = factor_panel(dY, NB, bb, bb, 0, sc.dinv, sc.dinfo, st.comp, st.bulk,
+    rc = factor_panel(dY, NBs, bb, bb, 0, sc.dinv, sc.dinfo, st.comp, st.bulk,
                       [](int) { return 0; });
     if (rc) return rc;
-    rc = read_info(sc.dinfo, st.comp, N, info);
+    rc = read_info(sc.dinfo, st.comp, N, info, NB);
     g_stat[0] = now() - t0;
     profile_collect();
     g_stat[1] = g_stat[2] = g_stat[3] = g_stat[4] = 0;
@@ -1129,7 +1203,7 @@ int oocdag_tiles_from_host(double *dY, const double *A, int N, int NB, int LDA) 
     g_err.clear();
     if (!dY || !A || N <= 0 || NB <= 0 || LDA < N) return OOCDAG_EARG;
     const int bb = (N + NB - 1) / NB;
-    const size_t nb2 = (size_t)NB * NB;
+    const size_t nb2 = (size_t)tile_edge(NB) * tile_edge(NB);
     std::vector<double> buf((size_t)bb * nb2);
     for (int tj = 0; tj < bb; ++tj) {
         pack_column(A, N, NB, LDA, tj, tj, buf.data());
@@ -1143,7 +1217,7 @@ int oocdag_tiles_to_host(const double *dY, double *A, int N, int NB, int LDA) {
     g_err.clear();
     if (!dY || !A || N <= 0 || NB <= 0 || LDA < N) return OOCDAG_EARG;
     const int bb = (N + NB - 1) / NB;
-    const size_t nb2 = (size_t)NB * NB;
+    const size_t nb2 = (size_t)tile_edge(NB) * tile_edge(NB);
     std::vector<double> buf((size_t)bb * nb2);
     for (int tj = 0; tj < bb; ++tj) {
         HIPTRY(hipMemcpy(buf.data(), dY + (size_t)tile_index(tj, tj, bb, 1) * nb2,
@@ -1269,17 +1343,15 @@ int oocdag_dsyrk_ln(int n, int k, double alpha, const double *A, int lda, double
     return OOCDAG_OK;
 }
 
-int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream) {
-    g_err.clear();
-    int dummy;
-    if (!info) info = &dummy;
-    *info = 0;
-    if (n < 0 || n % 64 || lda < std::max(1, n) || (lda & 1)) {
-        set_err("dpotrf_l: n must be a multiple of 64 and lda even");
-        return OOCDAG_EARG;
+// Device scratch freed on scope exit.
+struct DevTmp {
+    double *p = nullptr;
+    ~DevTmp() {
+        if (p) (void)hipFree(p);
     }
-    if (n == 0) return OOCDAG_OK;
-    hipStream_t s = (hipStream_t)stream;
+};
+
+int dpotrf_aligned(int n, double *A, int lda, int *info, hipStream_t s) {
     Scratch sc;
     // the left-looking sweep keeps one 64x64 inverse per block column
     HIPTRY(hipMalloc(&sc.dinv, (size_t)(n / 64) * 64 * 64 * sizeof(double)));
@@ -1290,28 +1362,45 @@ int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream) {
     return read_info(sc.dinfo, s, n, info);
 }
 
-int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb, void *stream) {
+int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream) {
     g_err.clear();
-    if (m < 0 || n < 0 || n % 64 || ldl < std::max(1, n) || ldb < std::max(1, m) || (ldl & 1) ||
-        (ldb & 1)) {
-        set_err("dtrsm_rltn: n must be a multiple of 64, leading dimensions even");
+    int dummy;
+    if (!info) info = &dummy;
+    *info = 0;
+    if (!A || n < 0 || lda < std::max(1, n)) {
+        set_err("dpotrf_l: invalid n or lda");
         return OOCDAG_EARG;
     }
-    if (m == 0 || n == 0) return OOCDAG_OK;
+    if (n == 0) return OOCDAG_OK;
     hipStream_t s = (hipStream_t)stream;
+    if (n % 64 == 0 && (lda & 1) == 0) return dpotrf_aligned(n, A, lda, info, s);
+    // any other shape: factored in a padded copy (n rounded up to 64, the
+    // padding an identity block, which leaves the n x n factor unchanged)
+    const int np = tile_edge(n);
+    DevTmp w;
+    HIPTRY(hipMalloc(&w.p, (size_t)np * np * sizeof(double)));
+    HIPTRY(hipMemcpy2DAsync(w.p, (size_t)np * 8, A, (size_t)lda * 8, (size_t)n * 8, n,
+                            hipMemcpyDeviceToDevice, s));
+    HIPTRY(launch_pad_tile(w.p, np, n, n, 1, s));
+    int rc = dpotrf_aligned(np, w.p, np, info, s);
+    if (rc) return rc;
+    if (*info > n) *info = 0;   // the padding's pivots are exactly 1
+    HIPTRY(hipMemcpy2DAsync(A, (size_t)lda * 8, w.p, (size_t)np * 8, (size_t)n * 8, n,
+                            hipMemcpyDeviceToDevice, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return OOCDAG_OK;
+}
+
+int dtrsm_aligned(int m, int n, const double *L, int ldl, double *B, int ldb, hipStream_t s) {
     // Invert L's 64x64 diagonal blocks in LDS (L itself is not modified),
     // then sweep block columns left to right:
     //   B[:, c] -= B[:, :c] L[c, :c]^T ;  B[:, c] <- B[:, c] D_c^T
     const int nblk = n / 64;
-    double *dinv = nullptr;
-    HIPTRY(hipMalloc(&dinv, (size_t)nblk * 64 * 64 * sizeof(double)));
-    struct F {
-        double *p;
-        ~F() { (void)hipFree(p); }
-    } fw{dinv};
+    DevTmp dinv;
+    HIPTRY(hipMalloc(&dinv.p, (size_t)nblk * 64 * 64 * sizeof(double)));
     for (int c = 0; c < nblk; ++c)
-        HIPTRY(launch_potrf64(plain((double *)L, ldl, c * 64, c * 64), 0, dinv + (size_t)c * 4096,
-                              nullptr, s));
+        HIPTRY(launch_potrf64(plain((double *)L, ldl, c * 64, c * 64), 0,
+                              dinv.p + (size_t)c * 4096, nullptr, s));
     for (int c = 0; c < nblk; ++c) {
         if (c > 0) {
             Gemm g;
@@ -1329,7 +1418,7 @@ int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb
         Gemm g;
         g.A = plain(B, ldb, 0, c * 64);
         g.C = g.A;
-        g.B = plain(dinv + (size_t)c * 4096, 64);
+        g.B = plain(dinv.p + (size_t)c * 4096, 64);
         g.M = m;
         g.N = 64;
         g.K = 64;
@@ -1342,14 +1431,48 @@ int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb
     return OOCDAG_OK;
 }
 
+int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb, void *stream) {
+    g_err.clear();
+    if (!L || !B || m < 0 || n < 0 || ldl < std::max(1, n) || ldb < std::max(1, m)) {
+        set_err("dtrsm_rltn: invalid sizes or leading dimensions");
+        return OOCDAG_EARG;
+    }
+    if (m == 0 || n == 0) return OOCDAG_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (n % 64 == 0 && (ldl & 1) == 0 && (ldb & 1) == 0)
+        return dtrsm_aligned(m, n, L, ldl, B, ldb, s);
+    // any other shape: L padded to np x np with an identity block and B to
+    // m x np with zero columns ([B 0] [L 0; 0 I]^-T = [B L^-T 0])
+    const int np = tile_edge(n);
+    const int ldm = m + (m & 1);
+    DevTmp lp, bp;
+    HIPTRY(hipMalloc(&lp.p, (size_t)np * np * sizeof(double)));
+    HIPTRY(hipMalloc(&bp.p, (size_t)ldm * np * sizeof(double)));
+    HIPTRY(hipMemcpy2DAsync(lp.p, (size_t)np * 8, L, (size_t)ldl * 8, (size_t)n * 8, n,
+                            hipMemcpyDeviceToDevice, s));
+    HIPTRY(launch_pad_tile(lp.p, np, n, n, 1, s));
+    HIPTRY(hipMemsetAsync(bp.p, 0, (size_t)ldm * np * sizeof(double), s));
+    HIPTRY(hipMemcpy2DAsync(bp.p, (size_t)ldm * 8, B, (size_t)ldb * 8, (size_t)m * 8, n,
+                            hipMemcpyDeviceToDevice, s));
+    int rc = dtrsm_aligned(m, np, lp.p, np, bp.p, ldm, s);
+    if (rc) return rc;
+    HIPTRY(hipMemcpy2DAsync(B, (size_t)ldb * 8, bp.p, (size_t)ldm * 8, (size_t)m * 8, n,
+                            hipMemcpyDeviceToDevice, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return OOCDAG_OK;
+}
+
 int oocdag_potrs_tiles(const double *dY, int N, int NB, double *dB, int nrhs, int ldb,
                        void *stream) {
     g_err.clear();
     if (!dY || !dB || N <= 0 || nrhs < 0 || ldb < N || check_nb(N, NB)) return OOCDAG_EARG;
     if (nrhs == 0) return OOCDAG_OK;
     hipStream_t s = (hipStream_t)stream;
+    // the solve runs on the padded matrix of the slots (padding rows of the
+    // right-hand sides zero, so their solution entries stay zero)
     const int bb = (N + NB - 1) / NB;
-    const long long Np = (long long)bb * NB;
+    const int NBs = tile_edge(NB);
+    const long long Np = (long long)bb * NBs;
     const long long ldx = nrhs + (nrhs & 1);
     double *dinv = nullptr, *Xt = nullptr;
     struct F {
@@ -1362,16 +1485,16 @@ int oocdag_potrs_tiles(const double *dY, int N, int NB, double *dB, int nrhs, in
     HIPTRY(hipMalloc(&dinv, (size_t)Np * 64 * sizeof(double)));
     HIPTRY(hipMalloc(&Xt, (size_t)(ldx * Np) * sizeof(double)));
     HIPTRY(hipMemsetAsync(Xt, 0, (size_t)(ldx * Np) * sizeof(double), s));
-    const Opnd Lp = panel((double *)dY, NB, bb, 1);
+    const Opnd Lp = panel((double *)dY, NBs, bb, 1);
     for (int k = 0; k < bb; ++k) {
         Opnd Lk = Lp;
-        Lk.r0 = k * NB;
-        Lk.c0 = k * NB;
-        HIPTRY(diag_inverses(Lk, NB, dinv + (size_t)k * NB * 64, s));
+        Lk.r0 = k * NBs;
+        Lk.c0 = k * NBs;
+        HIPTRY(diag_inverses(Lk, NBs, dinv + (size_t)k * NBs * 64, s));
     }
-    HIPTRY(launch_transpose(dB, ldb, Xt, ldx, N, nrhs, s));
-    HIPTRY(potrs_tiles_t(dY, N, NB, dinv, Xt, nrhs, ldx, s));
-    HIPTRY(launch_transpose(Xt, ldx, dB, ldb, nrhs, N, s));
+    HIPTRY(launch_transpose(dB, ldb, Xt, ldx, N, nrhs, s, NB, NBs, 1));
+    HIPTRY(potrs_tiles_t(dY, (int)Np, NBs, dinv, Xt, nrhs, ldx, s));
+    HIPTRY(launch_transpose(Xt, ldx, dB, ldb, nrhs, N, s, NB, NBs, 2));
     HIPTRY(hipStreamSynchronize(s));
     return OOCDAG_OK;
 }
@@ -1427,7 +1550,7 @@ struct ResidScratch {
     }
     int init(int N, int NB) {
         const int bb = (N + NB - 1) / NB;
-        const size_t col = (size_t)bb * NB * NB * sizeof(double);
+        const size_t col = (size_t)bb * tile_edge(NB) * tile_edge(NB) * sizeof(double);
         if (hipMalloc(&R, col) != hipSuccess || hipMalloc(&S, col) != hipSuccess) {
             set_err("residual scratch allocation failed");
             return OOCDAG_ENOMEM;
@@ -1531,6 +1654,7 @@ int oocdag_profile_enable(int on) {
     g_prof.launches = 0;
     g_prof.seconds = 0;
     g_prof.total_flops = 0;
+    g_prof.total_bytes = 0;
     return OOCDAG_OK;
 }
 
@@ -1539,6 +1663,12 @@ int oocdag_profile_read(long long *launches, double *seconds, double *flops) {
     if (launches) *launches = g_prof.launches;
     if (seconds) *seconds = g_prof.seconds;
     if (flops) *flops = g_prof.total_flops;
+    return OOCDAG_OK;
+}
+
+int oocdag_profile_bytes(double *bytes) {
+    profile_collect();
+    if (bytes) *bytes = g_prof.total_bytes;
     return OOCDAG_OK;
 }
 

@@ -184,7 +184,7 @@ struct GenArgs {
     LcgMap stepN;      // N-step map
     unsigned x1;
     double *Y;
-    int N, nb, bb, jt0, jt1;
+    int N, nb, ts, bb, jt0, jt1;   // logical tile edge nb, slot edge ts >= nb
     int symm_F;        // 1: AFORM 'F'
     int diagD;
     long long ntiles;  // tiles in the column range
@@ -198,16 +198,19 @@ __device__ __forceinline__ unsigned lcg_apply(LcgMap m, unsigned x) {
     return (m.a * x + m.c) & 0x7fffffffu;
 }
 
+// One thread per slot row r of a run of GEN_RUN slot columns of one tile.
+// Element (r, c) of tile (ti, tj) is matrix element (ti*nb + r, tj*nb + c)
+// when r, c < nb and both indices are below N; every other slot entry is
+// padding.  Consecutive valid columns of a row advance the LCG by one
+// N-step (lower triangle, stream position j*N + i) or one step (upper).
 __global__ __launch_bounds__(256) void k_gen_tiles(GenArgs g) {
-    // block -> (tile, 256-row chunk, run of GEN_RUN columns)
-    const int chunks = (g.nb + 255) / 256;
+    const int chunks = (g.ts + 255) / 256;
     long long b = blockIdx.x;
     int run = (int)(b % g.runs);
     b /= g.runs;
     int chunk = (int)(b % chunks);
     long long t = b / chunks;
     if (t >= g.ntiles) return;
-    // tile t (column-major over the lower tiles of columns jt0..) -> (ti, tj)
     int tj = g.jt0;
     long long rem = t;
     while (rem >= (long long)(g.bb - tj)) {
@@ -216,22 +219,22 @@ __global__ __launch_bounds__(256) void k_gen_tiles(GenArgs g) {
     }
     int ti = tj + (int)rem;
     int r = chunk * 256 + threadIdx.x;
-    if (r >= g.nb) return;
+    if (r >= g.ts) return;
     long long tidx = ti + (long long)tj * g.bb - ((long long)tj * (tj + 1)) / 2;
-    double *tile = g.Y + (tidx - g.tbase) * (long long)g.nb * g.nb;
+    double *tile = g.Y + (tidx - g.tbase) * (long long)g.ts * g.ts;
     const int gi = ti * g.nb + r;
+    const bool row_ok = r < g.nb && gi < g.N;
     const int cbeg = run * GEN_RUN;
-    const int cend = min(cbeg + GEN_RUN, g.nb);
+    const int cend = min(cbeg + GEN_RUN, g.ts);
     const double c01 = (double)0.1f;
     const double fN = (double)g.N;
-    // starting stream state for (gi, gj0)
     int gj = tj * g.nb + cbeg;
     unsigned x = 0;
     bool have = false;
     for (int c = cbeg; c < cend; ++c, ++gj) {
         double v;
-        if (gi >= g.N || gj >= g.N) {
-            v = (gi == gj) ? 1.0 : 0.0;
+        if (!row_ok || c >= g.nb || gj >= g.N) {
+            v = (ti == tj && r == c) ? 1.0 : 0.0;
         } else if (g.symm_F) {
             v = (gi == gj) ? 10.0 : c01;
         } else {
@@ -243,14 +246,12 @@ __global__ __launch_bounds__(256) void k_gen_tiles(GenArgs g) {
                     if (p & 1ull) x = lcg_apply(g.pow2[k], x);
                 have = true;
             } else {
-                // moving one column right: position +N below the diagonal,
-                // +1 above it
                 x = (gi >= gj) ? lcg_apply(g.stepN, x) : lcg_apply(g.pow2[0], x);
             }
             v = 1.0 - 2.0 * ((double)x / 2147483648.0);
             if (gi == gj && g.diagD) v = fabs(v) + fN;
         }
-        tile[r + (long long)c * g.nb] = v;
+        tile[r + (long long)c * g.ts] = v;
     }
 }
 
@@ -396,6 +397,7 @@ hipError_t launch_gen_tiles(double *Y, int N, int nb, int bb, int jt0, int jt1, 
     g.Y = Y;
     g.N = N;
     g.nb = nb;
+    g.ts = tile_edge(nb);
     g.bb = bb;
     g.jt0 = jt0;
     g.jt1 = jt1;
@@ -405,8 +407,8 @@ hipError_t launch_gen_tiles(double *Y, int N, int nb, int bb, int jt0, int jt1, 
     long long nt = 0;
     for (int j = jt0; j < jt1; ++j) nt += bb - j;
     g.ntiles = nt;
-    g.runs = (nb + GEN_RUN - 1) / GEN_RUN;
-    const int chunks = (nb + 255) / 256;
+    g.runs = (g.ts + GEN_RUN - 1) / GEN_RUN;
+    const int chunks = (g.ts + 255) / 256;
     long long blocks = nt * chunks * g.runs;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gen_tiles, dim3((unsigned)blocks), dim3(256), 0, s, g);

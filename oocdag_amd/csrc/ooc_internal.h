@@ -41,6 +41,14 @@ inline Opnd panel(double *p, int nb, int ym, int tri, int r0 = 0, int c0 = 0) {
     return o;
 }
 
+// Storage edge of a tile of logical edge nb: tile (i, j) of the matrix
+// (rows i*nb .. i*nb+nb-1) is kept in a ts x ts slot, ts = nb rounded up to a
+// multiple of 64 (the diagonal-block edge of the factorization kernels); rows
+// and columns nb .. ts-1 of a slot are padding (0, with 1 on the diagonal of
+// a diagonal tile), so the device factors a padded SPD matrix whose factor
+// holds L at the valid positions.  ts == nb when nb % 64 == 0.
+inline int tile_edge(int nb) { return (nb + 63) / 64 * 64; }
+
 inline long long tile_index(int ti, int tk, int ym, int tri) {
     return ti + (long long)tk * ym - (tri ? ((long long)tk * (tk + 1)) / 2 : 0);
 }
@@ -144,9 +152,11 @@ int factor_panel_la(const Opnd &P, int NB, int Ym, int Yn, int gcol0, double *di
 hipError_t launch_pad_tile(double *tile, int nb, int vr, int vc, int diag, hipStream_t s);
 
 // PDMATGEN('S'/'H' or 'F', DIAG, ISEED) values written straight into the
-// packed lower tiles (tri = 1, ym = bb) of tile columns [jt0, jt1), padded
-// as launch_pad_tile does beyond N.  Y points at the tile with packed index
-// tbase (0: the whole layout; else a window holding those columns only).
+// packed lower tiles (tri = 1, ym = bb) of tile columns [jt0, jt1): tiles of
+// logical edge nb in slots of edge ts = tile_edge(nb), padded beyond N and
+// beyond nb inside each slot (0, 1 on a diagonal tile's diagonal).  Y points
+// at the tile with packed index tbase (0: the whole layout; else a window
+// holding those columns only).
 hipError_t launch_gen_tiles(double *Y, int N, int nb, int bb, int jt0, int jt1, char aform,
                             char diag, int iseed, hipStream_t s,
                             long long tbase = 0);
@@ -164,8 +174,12 @@ hipError_t trsm_right_lower_n(const Opnd &B, int m, int n, const Opnd &L, const 
 // dinv_all: the 64-block inverses of every diagonal tile, tile after tile.
 hipError_t potrs_tiles_t(const double *dY, int N, int NB, const double *dinv_all, double *Xt,
                          int nrhs, long long ldx, hipStream_t s);
+// out(c, r) = in(r, c).  ts > 0: the tiled dimension holds tiles of nbl
+// rows in slots of ts; mode 1 maps in's row index, mode 2 in's column index
+// from logical to slot positions (the other side stays logical).
 hipError_t launch_transpose(const double *in, long long ld_in, double *out, long long ld_out,
-                            int rows, int cols, hipStream_t s);
+                            int rows, int cols, hipStream_t s, int nbl = 0, int ts = 0,
+                            int mode = 0);
 // Row sums of |A - L L^T| (rowsum) and of |A| (anorm) for the packed factor
 // dY of the PDMATGEN(aform, diag, iseed) matrix.  R and S: scratch of bb
 // tiles each.

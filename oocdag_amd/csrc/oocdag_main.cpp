@@ -1,4 +1,4 @@
-// oocdag N NB memsize_MB [--verify] [--resid]
+// oocdag N NB memsize_MB [--verify] [--resid] [--nx K]
 //
 // Command-line driver with the reference's argument order and output lines
 // (main, oocdag.cpp:223-475): generates the PDMATGEN('S','D', ISEED=100)
@@ -9,7 +9,10 @@
 // QUARK pool size there).  N <= 32 prints A and L like do_io
 // (oocdag.cpp:253,307-310,375-378).  --verify runs the VERIFY block's
 // identity check dpotrs(L, A) = I on the GPU path's factor (oocdag.cpp:379-464)
-// through oocdag_potrs; --resid prints the LIN scaled residual.
+// through oocdag_potrs; --resid prints the LIN scaled residual.  The panel
+// schedule is the reference's (find_Yn with one staged X tile column,
+// oocdag.cpp:478-492) unless --nx 2 (two X columns, copies overlapped with
+// the updates) or --nx 0 (the library's choice) is given.
 #include "oocdag.h"
 
 #include <chrono>
@@ -48,9 +51,11 @@ int main(int argc, char **argv) {
     const int NB = atoi(argv[2]);
     const size_t memsize = (size_t)atoi(argv[3]) * 1024 * 1024;
     bool verify = false, resid = false;
+    int nx = 1;
     for (int i = 4; i < argc; ++i) {
         if (!strcmp(argv[i], "--verify")) verify = true;
         if (!strcmp(argv[i], "--resid")) resid = true;
+        if (!strcmp(argv[i], "--nx") && i + 1 < argc) nx = atoi(argv[++i]);
     }
     if (N <= 0 || NB <= 0) {
         printf("invalid arguments N NB memsize(M)\n");
@@ -83,7 +88,7 @@ int main(int argc, char **argv) {
     }
 
     int widths[4096];
-    int np = oocdag_panel_plan(N, NB, memsize, widths, 4096);
+    int np = oocdag_panel_plan_nx(N, NB, memsize, nx, widths, 4096);
     int jb = 0;
     for (int p = 0; p < np && p < 4096; ++p) {
         int Ym = BB - jb, Yn = widths[p];
@@ -95,10 +100,10 @@ int main(int argc, char **argv) {
 
     printf("LLT start %lf\n", wall());
     int info = 0;
-    double llttime = oocdag_cholesky(Ap, N, NB, LDA, memsize, &info);
+    double llttime = oocdag_cholesky_nx(Ap, N, NB, LDA, memsize, nx, &info);
     printf("LLT end %lf\n", wall());
     if (llttime < 0 || info < 0) {
-        printf("oocdag_cholesky failed: info %d %s\n", info, oocdag_last_error());
+        printf("oocdag_cholesky_nx failed: info %d %s\n", info, oocdag_last_error());
         return 1;
     }
     if (info > 0) printf("dpotrf info %d: leading minor not positive definite\n", info);

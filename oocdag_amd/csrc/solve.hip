@@ -98,59 +98,54 @@ RowGen make_rowgen(int N, char aform, char diag, int iseed) {
     return g;
 }
 
-// Row sums of |A - R| and of |A| for tile column jt.
-//   R      : L L^T for tile rows jt..bb-1 of this column (rect tiles)
+// Row sums of |A - R| for tile column jt; one thread per slot row, so
+// each row's sum is added by one thread in a fixed column order and the
+// launches (one per tile column, ordered on one stream) accumulate in a
+// fixed order too: the residual is reproducible bit for bit.
+//   R      : L L^T for tile rows jt..bb-1 of this column (rect slots)
 //   Y      : packed factor (its diagonal tile's strict upper part holds A)
-// rowsum[gr] += sum |A - R| over the row's entries in this column,
-// anorm[gr] and anorm[gc] collect |A| of the lower entries (symmetry).
+// Tiles of logical edge nb sit in slots of edge ts; rows and columns past
+// nb in a slot, or past N, are padding and skipped.
 __global__ __launch_bounds__(256) void k_resid_column(RowGen g, const double *R, const double *Y,
-                                                      int nb, int bb, int jt, double *rowsum,
-                                                      double *anorm) {
-    const int r = blockIdx.x * 256 + threadIdx.x;  // row offset below the column top
-    const int rows = (bb - jt) * nb;
-    const int gr = jt * nb + r;
-    const int c0 = jt * nb;
-    __shared__ double colacc[256];
-    double rs = 0.0, as = 0.0;
+                                                      int nb, int ts, int bb, int jt,
+                                                      double *rowsum) {
+    const int r = blockIdx.x * 256 + threadIdx.x;  // slot row offset below the column top
+    const int rows = (bb - jt) * ts;
+    if (r >= rows) return;
+    const int ti = r / ts, rt = r - ti * ts;
+    const int gr = (jt + ti) * nb + rt;
+    if (rt >= nb || gr >= g.N) return;
+    const long long ts2 = (long long)ts * ts;
+    const long long ydiag = ((long long)jt * bb - ((long long)jt * (jt - 1)) / 2) * ts2;
+    double rs = 0.0;
     RowGenState st{0u, false};
-    const bool live = r < rows && gr < g.N;
-    const long long ydiag = ((long long)jt * bb - ((long long)jt * (jt - 1)) / 2) * nb * nb;
-    for (int cb = 0; cb < nb; cb += 256) {
-        colacc[threadIdx.x] = 0.0;
-        __syncthreads();
-        for (int c = cb; c < min(cb + 256, nb); ++c) {
-            const int gc = c0 + c;
-            double colv = 0.0;
-            if (live && gc < g.N) {
-                double a = gen_next(g, st, gr, gc);
-                if (gr >= gc) {
-                    int ti = r / nb;
-                    double l = R[(long long)ti * nb * nb + (r - ti * nb) + (long long)c * nb];
-                    rs += fabs(a - l);
-                    as += fabs(a);
-                    if (gr > gc) colv = fabs(a);
-                } else {
-                    double y = Y[ydiag + r + (long long)c * nb];  // r < nb here
-                    rs += fabs(a - y);
-                }
-            }
-            // column sums of |A| across the block's rows -> anorm[gc]
-            for (int o = 32; o > 0; o >>= 1) colv += __shfl_down(colv, o);
-            if ((threadIdx.x & 63) == 0 && colv != 0.0) atomicAdd(&colacc[c - cb], colv);
-        }
-        __syncthreads();
-        for (int c = threadIdx.x; c < min(256, nb - cb); c += 256)
-            if (colacc[c] != 0.0) atomicAdd(&anorm[c0 + cb + c], colacc[c]);
-        __syncthreads();
+    for (int c = 0; c < nb; ++c) {
+        const int gc = jt * nb + c;
+        if (gc >= g.N) break;
+        const double a = gen_next(g, st, gr, gc);
+        if (gr >= gc)
+            rs += fabs(a - R[ti * ts2 + rt + (long long)c * ts]);
+        else
+            rs += fabs(a - Y[ydiag + rt + (long long)c * ts]);   // ti == 0 here
     }
-    if (live) {
-        atomicAdd(&rowsum[gr], rs);
-        atomicAdd(&anorm[gr], as);
-    }
+    rowsum[gr] += rs;
+}
+
+// ||A||_inf row sums of the regenerated symmetric matrix: one thread per
+// row walks the whole row left to right (stream positions j*N + i below
+// the diagonal, i*N + j above it), a fixed order, no atomics.
+__global__ __launch_bounds__(256) void k_row_norms(RowGen g, double *anorm) {
+    const int gr = blockIdx.x * 256 + threadIdx.x;
+    if (gr >= g.N) return;
+    RowGenState st{0u, false};
+    double s = 0.0;
+    for (int gc = 0; gc < g.N; ++gc) s += fabs(gen_next(g, st, gr, gc));
+    anorm[gr] = s;
 }
 
 // Row sums of |A_host - A| over the strict upper tiles above tile (jt, jt):
-// U holds rows [0, jt*nb) of columns [jt*nb, jt*nb + nb) (ld = ldu).
+// U holds rows [0, jt*nb) of columns [jt*nb, jt*nb + nb) (ld = ldu).  One
+// thread per row; launches ordered on one stream.
 __global__ __launch_bounds__(256) void k_resid_upper(RowGen g, const double *U, long long ldu,
                                                      int nb, int jt, double *rowsum) {
     const int gr = blockIdx.x * 256 + threadIdx.x;
@@ -163,7 +158,7 @@ __global__ __launch_bounds__(256) void k_resid_upper(RowGen g, const double *U, 
         double a = gen_next(g, st, gr, gc);
         rs += fabs(U[gr + (long long)c * ldu] - a);
     }
-    atomicAdd(&rowsum[gr], rs);
+    rowsum[gr] += rs;
 }
 
 // Copy tile column jt (tiles jt..bb-1, packed) to a rect scratch column with
@@ -181,19 +176,25 @@ __global__ void k_masked_column(const double *Y, double *S, int nb, int bb, int 
 }
 
 // out(c, r) = in(r, c): in is rows x cols (ld_in), out is cols x rows (ld_out).
+// With slots (ts > 0) a logical index g of the tiled dimension becomes the
+// slot index (g / nbl) * ts + g % nbl: mode 1 maps in's row index (= out's
+// column index), mode 2 in's column index.
 __global__ void k_transpose(const double *in, long long ld_in, double *out, long long ld_out,
-                            int rows, int cols) {
+                            int rows, int cols, int nbl, int ts, int mode) {
     __shared__ double t[32][33];
     int bx = blockIdx.x * 32, by = blockIdx.y * 32;
     int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    auto map = [&](long long g) { return ts > 0 ? (g / nbl) * ts + g % nbl : g; };
     for (int k = 0; k < 32; k += 8) {
         int r = bx + tx, c = by + ty + k;
-        t[ty + k][tx] = (r < rows && c < cols) ? in[r + (long long)c * ld_in] : 0.0;
+        const long long ci = mode == 2 ? map(c) : c;
+        t[ty + k][tx] = (r < rows && c < cols) ? in[r + ci * ld_in] : 0.0;
     }
     __syncthreads();
     for (int k = 0; k < 32; k += 8) {
         int c = by + tx, r = bx + ty + k;
-        if (c < cols && r < rows) out[c + (long long)r * ld_out] = t[tx][ty + k];
+        const long long ro = mode == 1 ? map(r) : r;
+        if (c < cols && r < rows) out[c + ro * ld_out] = t[tx][ty + k];
     }
 }
 
@@ -304,42 +305,47 @@ hipError_t potrs_tiles_t(const double *dY, int N, int NB, const double *dinv_all
 }
 
 hipError_t launch_transpose(const double *in, long long ld_in, double *out, long long ld_out,
-                            int rows, int cols, hipStream_t s) {
+                            int rows, int cols, hipStream_t s, int nbl, int ts, int mode) {
+    if (ts == nbl) ts = 0;   // slots without padding: indices map to themselves
     dim3 grid((rows + 31) / 32, (cols + 31) / 32), block(32, 8);
-    hipLaunchKernelGGL(k_transpose, grid, block, 0, s, in, ld_in, out, ld_out, rows, cols);
+    hipLaunchKernelGGL(k_transpose, grid, block, 0, s, in, ld_in, out, ld_out, rows, cols, nbl,
+                       ts, mode);
     return hipGetLastError();
 }
 
 hipError_t llt_resid_tiles(const double *dY, int N, int NB, char aform, char diag, int iseed,
                            double *R, double *S, double *rowsum, double *anorm, hipStream_t s) {
     const int bb = (N + NB - 1) / NB;
+    const int ts = tile_edge(NB);   // slot edge: L L^T is rebuilt on the padded slots
     RowGen gen = make_rowgen(N, aform, diag, iseed);
-    const Opnd Lp = panel((double *)dY, NB, bb, 1);
+    const Opnd Lp = panel((double *)dY, ts, bb, 1);
     hipError_t e;
+    hipLaunchKernelGGL(k_row_norms, dim3((N + 255) / 256), dim3(256), 0, s, gen, anorm);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     for (int j = 0; j < bb; ++j) {
-        const int rows = (bb - j) * NB;
-        long long n = (long long)rows * NB;
+        const int rows = (bb - j) * ts;
+        long long n = (long long)rows * ts;
         int blocks = (int)std::min<long long>((n + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_masked_column, dim3(blocks), dim3(256), 0, s, dY, S, NB, bb, j);
+        hipLaunchKernelGGL(k_masked_column, dim3(blocks), dim3(256), 0, s, dY, S, ts, bb, j);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (j > 0) {
             Gemm g;
-            g.A = Lp; g.A.r0 = j * NB; g.A.c0 = 0;
+            g.A = Lp; g.A.r0 = j * ts; g.A.c0 = 0;
             g.B = g.A;
-            g.C = panel(R, NB, bb - j, 0);
-            g.M = rows; g.N = NB; g.K = j * NB;
+            g.C = panel(R, ts, bb - j, 0);
+            g.M = rows; g.N = ts; g.K = j * ts;
             g.alpha = 1.0; g.beta = 0.0; g.lower = 0;
             if ((e = launch_gemm(g, GEMM_GENERAL, s)) != hipSuccess) return e;
         }
         Gemm g;
-        g.A = panel(S, NB, bb - j, 0);
+        g.A = panel(S, ts, bb - j, 0);
         g.B = g.A;
-        g.C = panel(R, NB, bb - j, 0);
-        g.M = rows; g.N = NB; g.K = NB;
+        g.C = panel(R, ts, bb - j, 0);
+        g.M = rows; g.N = ts; g.K = ts;
         g.alpha = 1.0; g.beta = j > 0 ? 1.0 : 0.0; g.lower = 0;
         if ((e = launch_gemm(g, GEMM_GENERAL, s)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_resid_column, dim3((rows + 255) / 256), dim3(256), 0, s, gen, R, dY,
-                           NB, bb, j, rowsum, anorm);
+                           NB, ts, bb, j, rowsum);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

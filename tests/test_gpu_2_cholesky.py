@@ -18,13 +18,19 @@ def rel(a, b):
     return np.max(np.abs(np.tril(a) - np.tril(b))) / np.max(np.abs(np.tril(b)))
 
 
+def slot(nb):
+    """Edge of the slot holding one nb x nb tile: nb rounded up to 64."""
+    return -(-nb // 64) * 64
+
+
 def expected_traffic(n, nb, widths):
     """Bytes an out-of-core run must copy in and out for the panel widths:
     per panel of Yn tile columns at tile column jb (Ym = bb - jb tile rows),
     A2Y loads its packed tile columns, sum_jj (Ym - jj) tiles
     (oocquark.cpp:253-275), and A2X loads rows jb.. of every earlier tile
     column, jb * Ym tiles (oocquark.cpp:301-319); every tile of the packed
-    lower triangle is copied out once (oocquark.cpp:389-419)."""
+    lower triangle is copied out once (oocquark.cpp:389-419).  A tile moves
+    as its slot (nb rounded up to 64)."""
     bb = -(-n // nb)
     h2d, jb = 0, 0
     for yn in widths:
@@ -32,12 +38,12 @@ def expected_traffic(n, nb, widths):
         h2d += sum(ym - jj for jj in range(yn)) + jb * ym
         jb += yn
     assert jb == bb
-    tile = nb * nb * 8
+    tile = slot(nb) ** 2 * 8
     return h2d * tile, bb * (bb + 1) // 2 * tile
 
 
 @pytest.mark.parametrize("n,nb,budget_tiles", [
-    (4096, 256, 56),     # 28 MiB: the 4-panel OOC configuration
+    (4096, 256, 56),     # 28 MiB, default schedule (two X columns): 6 panels
     (1000, 64, 96),      # ragged edge, many panels
     (2048, 256, 60),     # whole lower triangle fits next to the X columns: one panel
     (1536, 128, 40),
