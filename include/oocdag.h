@@ -172,6 +172,23 @@ int oocdag_llt_resid_tiles(const double *dY, int N, int NB, char aform, char dia
                            double *resid);
 
 /* ---------------------------------------------------------------------------
+ * Solve residual max_k ||A x_k - b_k||_inf / (||A||_inf ||x_k||_inf eps N),
+ * eps = 2^-53, the maximum over the right-hand sides.
+ * Replaces: PDLASCHK(SYMM, DIAG, N, NRHS, X, ..., ANORM, RESID)
+ *           LIN/pdlaschk.f:22-26,186,295 (ANORM = ||A||_inf of the symmetric
+ *           matrix, formed here).
+ * oocdag_solve_resid: host arrays; the lower triangle of A (column-major,
+ * LDA) defines the symmetric matrix.  oocdag_solve_resid_tiles: device A in
+ * packed lower tiles with full (symmetric) diagonal tiles, as the tile
+ * generators write them; device X and B (ldx, ldb >= N).  Every reduction
+ * runs in a fixed order: the same inputs give the same bits.
+ * ------------------------------------------------------------------------- */
+int oocdag_solve_resid(const double *A, int N, int LDA, const double *X, int nrhs, int LDX,
+                       const double *B, int LDB, double *resid);
+int oocdag_solve_resid_tiles(const double *dA, int N, int NB, const double *dX, int nrhs, int ldx,
+                             const double *dB, int ldb, double *resid);
+
+/* ---------------------------------------------------------------------------
  * Mixed precision + iterative refinement (no reference counterpart: the
  * reference factors in FP64 only).  Factor a copy of the packed FP64 tiles
  * dA with the trailing updates in FP64 (precision 0), FP32 (1) or BF16 with
@@ -191,6 +208,8 @@ typedef struct {
     int info;              /* > 0: A itself is not positive definite */
     double backward_error; /* of the returned solution */
     double factor_seconds, solve_seconds, total_seconds;
+    double solve_resid;    /* PDLASCHK residual of the returned solution
+                              (LIN/pdlaschk.f:186,295; oocdag_solve_resid) */
 } oocdag_mpir_stats;
 
 int oocdag_mpir_solve_tiles(const double *dA, int N, int NB, const double *dB, int nrhs, int ldb,
@@ -217,8 +236,11 @@ int oocdag_normal_fill(double *dB, int rows, int cols, int ld, unsigned long lon
  * through a captured hipGraph `reps` times; its block size and register
  * tiling are picked by timing the compiled variants on a sample of the
  * batch.  precision 0: FP64 factor; 1: FP32 factor + FP64 refinement to the
- * backward error tol.  status_s: 0 ok, k > 0 not positive definite at
- * column k, -2 refinement did not reach tol (rerun it with precision 0).
+ * backward error tol.  Per system (optional arrays): berr, the normwise
+ * backward error max_k ||r_k|| / (||A|| ||x_k|| + ||b_k||); solve_resid,
+ * PDLASCHK's max_k ||r_k|| / (||A|| ||x_k|| eps n) (LIN/pdlaschk.f:186,295);
+ * iters; status: 0 ok, k > 0 not positive definite at column k, -2
+ * refinement did not reach tol (rerun it with precision 0).
  * ------------------------------------------------------------------------- */
 typedef struct {
     double seconds;             /* one batch, averaged over reps */
@@ -227,6 +249,7 @@ typedef struct {
     int failed;                 /* systems with status != 0 */
     int variant_T, variant_RB;  /* chosen block size and register tile */
     double tune_seconds;
+    double max_solve_resid;     /* PDLASCHK residual, max over systems with status 0 */
 } oocdag_batch_stats;
 
 long long oocdag_batch_layout(int nsys, const int *n, const int *nrhs, long long *a_off,
@@ -237,7 +260,7 @@ int oocdag_batch_generate(int nsys, const int *n, const int *nrhs, const long lo
 int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long long *a_off,
                          const long long *b_off, const double *dA, const double *dB, double *dX,
                          int precision, double tol, int max_iter, int reps, double *berr,
-                         int *iters, int *status, oocdag_batch_stats *stats);
+                         double *solve_resid, int *iters, int *status, oocdag_batch_stats *stats);
 /* Fixes the kernel variant oocdag_batched_solve uses for a precision
  * (0 FP64, 1 FP32+IR) instead of autotuning; variant -1 restores the
  * autotuner.  Returns the number of variants, or OOCDAG_EARG.  Variants
@@ -276,7 +299,8 @@ typedef struct {
 } oocdag_request;
 
 typedef struct {
-    int status;                  /* 0 ok, > 0 not positive definite, < 0 OOCDAG_E* */
+    int status;                  /* 0 ok, > 0 not positive definite, < 0 OOCDAG_E*
+                                    (OOCDAG_EBUDGET: needs more than the whole budget) */
     int route;                   /* 0 batched, 1 HBM mixed precision, 2 out-of-core */
     int cache_hit;
     int batch_size;              /* requests in the same launch (route 0) */
@@ -287,6 +311,9 @@ typedef struct {
     double input_seconds;        /* part of service_seconds spent materializing the
                                     request's matrix (A = G G^T / n + mu I, n^3 flops)
                                     and right-hand sides; 0 on a factor-cache hit */
+    double solve_resid;          /* PDLASCHK residual of the solution,
+                                    max_k ||A x_k - b_k|| / (||A|| ||x_k|| eps n)
+                                    (LIN/pdlaschk.f:186,295) */
 } oocdag_result;
 
 typedef struct oocdag_server oocdag_server;
@@ -298,6 +325,13 @@ int oocdag_server_wait(oocdag_server *srv, long long ticket, oocdag_result *res,
                        size_t x_len);
 int oocdag_server_stats(oocdag_server *srv, size_t *used_bytes, int *cached, int *pending,
                         int *running);
+/* The HBM bytes admission charges for a request: on a factor-cache hit, on a
+ * miss, the resident factor it leaves in the cache, and the least the
+ * out-of-core route needs (a miss above the budget runs out of core inside
+ * max(that, budget / 2), capped at the budget; a request that cannot fit
+ * even then completes with OOCDAG_EBUDGET). */
+int oocdag_server_estimate(const oocdag_request *rq, size_t *hit_bytes, size_t *miss_bytes,
+                           size_t *cached_bytes, size_t *ooc_min_bytes);
 int oocdag_server_destroy(oocdag_server *srv);
 
 /* ---------------------------------------------------------------------------
@@ -353,6 +387,11 @@ int oocdag_profile_read(long long *launches, double *seconds, double *flops);
  * of the lower trapezoid read and written once (16 B) plus the K-deep
  * operand strips read once (8 B per element); the roofline's byte count. */
 int oocdag_profile_bytes(double *bytes);
+
+/* The out-of-core entry points stage host data through page-locked buffers
+ * that they keep in a process-wide pool for the next call (at most 10 idle
+ * buffers); this frees the idle ones and returns how many it freed. */
+int oocdag_release_staging(void);
 
 /* Progress lines on stderr every this many factored tile columns (0 = off),
  * the successor of the reference's per-panel printf (oocdag.cpp:67). */

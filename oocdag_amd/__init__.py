@@ -74,6 +74,10 @@ SIGNATURES = {
     "oocdag_llt_resid": (C.c_int, [_dp, C.c_int, C.c_int, C.c_char, C.c_char, C.c_int, _dp]),
     "oocdag_llt_resid_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char, C.c_char,
                                          C.c_int, _dp]),
+    "oocdag_solve_resid": (C.c_int, [_dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp, C.c_int,
+                                     _dp]),
+    "oocdag_solve_resid_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                           C.c_int, C.c_void_p, C.c_int, _dp]),
     "oocdag_dev_alloc": (C.c_void_p, [C.c_size_t]),
     "oocdag_dev_free": (C.c_int, [C.c_void_p]),
     "oocdag_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -88,13 +92,15 @@ SIGNATURES = {
     "oocdag_profile_read": (C.c_int, [C.POINTER(C.c_longlong), _dp, _dp]),
     "oocdag_profile_bytes": (C.c_int, [_dp]),
     "oocdag_set_progress": (C.c_int, [C.c_int]),
+    "oocdag_release_staging": (C.c_int, []),
 }
 
 
 class MpirStats(C.Structure):
     _fields_ = [("precision_used", C.c_int), ("fell_back", C.c_int), ("iterations", C.c_int),
                 ("info", C.c_int), ("backward_error", C.c_double), ("factor_seconds", C.c_double),
-                ("solve_seconds", C.c_double), ("total_seconds", C.c_double)]
+                ("solve_seconds", C.c_double), ("total_seconds", C.c_double),
+                ("solve_resid", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -113,7 +119,8 @@ SIGNATURES.update({
 class BatchStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("systems_per_second", C.c_double),
                 ("max_backward_error", C.c_double), ("failed", C.c_int), ("variant_T", C.c_int),
-                ("variant_RB", C.c_int), ("tune_seconds", C.c_double)]
+                ("variant_RB", C.c_int), ("tune_seconds", C.c_double),
+                ("max_solve_resid", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -127,8 +134,8 @@ SIGNATURES.update({
     "oocdag_batch_set_variant": (C.c_int, [C.c_int, C.c_int]),
     "oocdag_mp_set_update_variant": (C.c_int, [C.c_int]),
     "oocdag_batched_solve": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
-                                       C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _ip,
-                                       _ip, C.POINTER(BatchStats)]),
+                                       C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp,
+                                       _ip, _ip, C.POINTER(BatchStats)]),
 })
 
 class Request(C.Structure):
@@ -143,7 +150,7 @@ class Result(C.Structure):
                 ("iterations", C.c_int), ("backward_error", C.c_double),
                 ("queue_seconds", C.c_double), ("service_seconds", C.c_double),
                 ("latency_seconds", C.c_double), ("x_checksum", C.c_double),
-                ("input_seconds", C.c_double)]
+                ("input_seconds", C.c_double), ("solve_resid", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -155,6 +162,9 @@ SIGNATURES.update({
     "oocdag_server_wait": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(Result), _dp, C.c_size_t]),
     "oocdag_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), _ip, _ip, _ip]),
     "oocdag_server_destroy": (C.c_int, [C.c_void_p]),
+    "oocdag_server_estimate": (C.c_int, [C.POINTER(Request), C.POINTER(C.c_size_t),
+                                         C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                         C.POINTER(C.c_size_t)]),
 })
 
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
@@ -328,6 +338,25 @@ def llt_resid(A: np.ndarray, aform: str = "S", diag: str = "D", iseed: int = 100
     return r.value
 
 
+def solve_resid(A: np.ndarray, X: np.ndarray, B: np.ndarray) -> float:
+    """PDLASCHK residual max_k ||A x_k - b_k|| / (||A|| ||x_k|| eps N) on the
+    GPU; the lower triangle of A defines the symmetric matrix."""
+    r = C.c_double(0)
+    _check(lib().oocdag_solve_resid(_ptr(A), A.shape[1], A.shape[0], _ptr(X), X.shape[1],
+                                    X.shape[0], _ptr(B), B.shape[0], C.byref(r)),
+           "solve_resid")
+    return r.value
+
+
+def solve_resid_tiles(buf: "DeviceBuffer", n: int, nb: int, X: "DeviceBuffer", B: "DeviceBuffer",
+                      nrhs: int, ldx: int, ldb: int) -> float:
+    """PDLASCHK residual with A in device packed tiles (full diagonal tiles)."""
+    r = C.c_double(0)
+    _check(lib().oocdag_solve_resid_tiles(buf.ptr, n, nb, X.ptr, nrhs, ldx, B.ptr, ldb,
+                                          C.byref(r)), "solve_resid_tiles")
+    return r.value
+
+
 def last_stats() -> dict:
     v = [C.c_double(0) for _ in range(5)]
     lib().oocdag_last_stats(*[C.byref(x) for x in v])
@@ -486,9 +515,12 @@ class Batch:
             raise OocdagError(f"batch_set_variant({precision}, {variant}) failed")
         return rc
 
-    def solve(self, precision="fp64", tol=None, max_iter=10, reps=1):
+    def solve(self, precision="fp64", tol=None, max_iter=10, reps=1, with_resid=False):
+        """Returns (stats, berr, iters, status), plus the per-system PDLASCHK
+        residual as a fifth item when with_resid."""
         k = len(self.n)
         berr = np.zeros(k)
+        lres = np.zeros(k)
         iters = np.zeros(k, dtype=np.int32)
         status = np.zeros(k, dtype=np.int32)
         st = BatchStats()
@@ -498,8 +530,11 @@ class Batch:
                                           self._l(self.a_off), self._l(self.b_off), self.A.ptr,
                                           self.B.ptr, self.X.ptr, PRECISIONS[precision], tol,
                                           max_iter, reps, berr.ctypes.data_as(_dp),
-                                          self._i(iters), self._i(status), C.byref(st)),
+                                          lres.ctypes.data_as(_dp), self._i(iters),
+                                          self._i(status), C.byref(st)),
                "batched_solve")
+        if with_resid:
+            return st.as_dict(), berr, iters, status, lres
         return st.as_dict(), berr, iters, status
 
     def host_system(self, s):
@@ -538,6 +573,16 @@ class Server:
                                         x.ctypes.data_as(_dp) if want_x else None,
                                         n * r if want_x else 0), "server_wait")
         return res.as_dict(), x
+
+    @staticmethod
+    def estimate(matrix_id, n, nrhs, mu=0.1, seed=1, rhs_seed=2, precision="fp32",
+                 tol=8 * 2.0 ** -53) -> dict:
+        """Admission's byte estimates for a request (hit, miss, cached, ooc_min)."""
+        rq = Request(matrix_id, n, nrhs, mu, seed, rhs_seed, PRECISIONS[precision], tol)
+        v = [C.c_size_t(0) for _ in range(4)]
+        _check(lib().oocdag_server_estimate(C.byref(rq), *[C.byref(x) for x in v]),
+               "server_estimate")
+        return dict(zip(("hit", "miss", "cached", "ooc_min"), (x.value for x in v)))
 
     def stats(self):
         u, c, p, r = C.c_size_t(0), C.c_int(0), C.c_int(0), C.c_int(0)

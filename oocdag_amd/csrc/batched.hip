@@ -45,6 +45,7 @@ constexpr int PMAX = 1024;        // max staged panel rows
 constexpr int NMAX = PMAX + 64;   // max system order
 constexpr int RMAX = 8;           // right-hand sides resident at a time
 constexpr int RTOTAL = 64;        // max right-hand sides per system
+constexpr double kEps = 1.1102230246251565e-16;   // 2^-53, PDLAMCH('eps')
 
 struct SysDesc {
     int n, nrhs, ld;       // ld: even leading dimension >= n of A, B, X and W
@@ -59,6 +60,7 @@ struct BatchArgs {
     double *X;
     void *W, *Dinv;
     double *berr;
+    double *lres;   // PDLASCHK residual per system (LIN/pdlaschk.f:186,295)
     int *iters, *status;
     int *counter;
     double tol;
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
         }
         // 3./4. solve and refine, RMAX right-hand sides at a time
         // (x in S.u.v.x, work vector in S.u.v.y)
-        double eta_all = 0.0;
+        double eta_all = 0.0, lres_all = 0.0;
         int it_all = 0;
         for (int r0 = 0; r0 < nrhs_all; r0 += RMAX) {
         const int nrhs = min(RMAX, nrhs_all - r0);
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
         }
         __syncthreads();
         int it = 0;
-        double eta = INFINITY, prev = INFINITY;
+        double eta = INFINITY, prev = INFINITY, lres = INFINITY;
         for (;;) {
             tri_solve<FT, T>(S, W, Dinv, n, ld, nrhs);
             for (int idx = tid; idx < n * nrhs; idx += NTB) {
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
                 }
             }
             // block max-reductions: amax, then per rhs the three norms
-            double e = 0.0;
+            double e = 0.0, lr = 0.0;
             S.red[tid] = amax;
             __syncthreads();
             for (int s2 = NTB / 2; s2 > 0; s2 >>= 1) {
@@ -372,9 +374,11 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
                     __syncthreads();
                 }
                 e = fmax(e, out[0] / (An * out[1] + out[2]));
+                    lr = fmax(lr, out[0] == 0.0 ? 0.0 : out[0] / (An * out[1] * kEps * n));
             }
             prev = eta;
             eta = e;
+            lres = lr;
             if (eta <= a.tol) break;
             if (it >= a.max_iter || !(eta < 0.5 * prev)) break;
             ++it;  // y holds r: next pass solves W d = r and adds d to x
@@ -384,11 +388,13 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
             X[i + (long long)k * ld] = x[k * NMAX + i];
         }
         eta_all = fmax(eta_all, eta);
+        lres_all = fmax(lres_all, lres);
         it_all = max(it_all, it);
         __syncthreads();
         }
         if (tid == 0) {
             a.berr[sidx] = eta_all;
+            a.lres[sidx] = lres_all;
             a.iters[sidx] = it_all;
             a.status[sidx] = (eta_all <= a.tol) ? 0 : -2;
         }
@@ -774,7 +780,7 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
         }
         // ---- solve and refine, LRMAX right-hand sides at a time; the
         // correction d in LDS, x accumulated in the caller's X
-        double eta_all = 0.0;
+        double eta_all = 0.0, lres_all = 0.0;
         int it_all = 0;
         for (int c0 = 0; c0 < nrhs_all; c0 += LRMAX) {
             const int nrhs = min(LRMAX, nrhs_all - c0);
@@ -788,7 +794,7 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
             }
             __syncthreads();
             int it = 0;
-            double eta = INFINITY, prev = INFINITY;
+            double eta = INFINITY, prev = INFINITY, lres = INFINITY;
             for (;;) {
                 tri_solve_ll<FT>(S, W, Dinv, n, ld, nrhs);
                 for (int idx = tid; idx < n * nrhs; idx += NTB) {
@@ -835,7 +841,7 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                         }
                     }
                 }
-                double e = 0.0;
+                double e = 0.0, lr = 0.0;
                 S.red[tid] = amax;
                 __syncthreads();
                 for (int s2 = NTB / 2; s2 > 0; s2 >>= 1) {
@@ -861,19 +867,23 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                         __syncthreads();
                     }
                     e = fmax(e, out[0] / (An * out[1] + out[2]));
+                    lr = fmax(lr, out[0] == 0.0 ? 0.0 : out[0] / (An * out[1] * kEps * n));
                 }
                 prev = eta;
                 eta = e;
+                lres = lr;
                 if (eta <= a.tol) break;
                 if (it >= a.max_iter || !(eta < 0.5 * prev)) break;
                 ++it;
             }
             eta_all = fmax(eta_all, eta);
+            lres_all = fmax(lres_all, lres);
             it_all = max(it_all, it);
             __syncthreads();
         }
         if (tid == 0) {
             a.berr[sidx] = eta_all;
+            a.lres[sidx] = lres_all;
             a.iters[sidx] = it_all;
             a.status[sidx] = (eta_all <= a.tol) ? 0 : -2;
         }
@@ -1041,7 +1051,8 @@ hipError_t gen_small_gram(double *A, int n, int ld, double mu, unsigned long lon
 extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long long *a_off,
                                     const long long *b_off, const double *dA, const double *dB,
                                     double *dX, int precision, double tol, int max_iter, int reps,
-                                    double *berr, int *iters, int *status, oocdag_batch_stats *st) {
+                                    double *berr, double *lres, int *iters, int *status,
+                                    oocdag_batch_stats *st) {
     oocdag_batch_stats dummy;
     if (!st) st = &dummy;
     *st = oocdag_batch_stats{};
@@ -1095,7 +1106,7 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         hipMallocAsync((void **)&dorder, nsys * sizeof(int), s) != hipSuccess ||
         hipMallocAsync((void **)&dint, 2 * (size_t)nsys * sizeof(int), s) != hipSuccess ||
         hipMallocAsync((void **)&dcounter, sizeof(int), s) != hipSuccess ||
-        hipMallocAsync((void **)&dberr, nsys * sizeof(double), s) != hipSuccess ||
+        hipMallocAsync((void **)&dberr, 2 * (size_t)nsys * sizeof(double), s) != hipSuccess ||
         hipMallocAsync(&dW, w * fts, s) != hipSuccess || hipMallocAsync(&dD, d * fts, s) != hipSuccess)
         return OOCDAG_ENOMEM;
     if (hipMemcpyAsync(dsd, sd.data(), nsys * sizeof(SysDesc), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1123,6 +1134,7 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         a.W = dW;
         a.Dinv = dD;
         a.berr = dberr;
+        a.lres = dberr + nsys;
         a.iters = dint;
         a.status = dint + nsys;
         a.counter = dcounter;
@@ -1226,18 +1238,24 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     st->seconds = tsum / std::max(1, reps);
     }
     st->systems_per_second = nsys / st->seconds;
-    std::vector<double> hb(nsys);
+    std::vector<double> hb(2 * (size_t)nsys);
     std::vector<int> hi(2 * (size_t)nsys);
-    if (hipMemcpyAsync(hb.data(), dberr, nsys * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (hipMemcpyAsync(hb.data(), dberr, 2 * (size_t)nsys * sizeof(double), hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
         hipMemcpyAsync(hi.data(), dint, 2 * (size_t)nsys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return OOCDAG_EHIP;
     for (int i = 0; i < nsys; ++i) {
         if (berr) berr[i] = hb[i];
+        if (lres) lres[i] = hb[nsys + i];
         if (iters) iters[i] = hi[i];
         if (status) status[i] = hi[nsys + i];
-        if (hi[nsys + i] != 0) st->failed += 1;
-        else st->max_backward_error = std::max(st->max_backward_error, hb[i]);
+        if (hi[nsys + i] != 0) {
+            st->failed += 1;
+        } else {
+            st->max_backward_error = std::max(st->max_backward_error, hb[i]);
+            st->max_solve_resid = std::max(st->max_solve_resid, hb[nsys + i]);
+        }
     }
     return OOCDAG_OK;
 }

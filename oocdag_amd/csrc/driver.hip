@@ -29,6 +29,7 @@
 #include <functional>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -491,22 +492,104 @@ struct CopyTimer {
     }
 };
 
-// Device scratch for the diagonal-block inverses and the info flag.
+// Device scratch for the diagonal-block inverses and the info flag;
+// stream-ordered (init(..., true)) when the owner releases it on its stream.
 struct Scratch {
     double *dinv = nullptr;
     int *dinfo = nullptr;
+    bool async = false;
     ~Scratch() {
         if (dinv) (void)hipFree(dinv);
         if (dinfo) (void)hipFree(dinfo);
     }
-    int init(int NB, hipStream_t s) {
-        HIPTRY(hipMalloc(&dinv, (size_t)(NB / 64) * 64 * 64 * sizeof(double)));
-        HIPTRY(hipMalloc(&dinfo, sizeof(int)));
+    void release(hipStream_t s) {
+        if (dinv) (void)hipFreeAsync(dinv, s);
+        if (dinfo) (void)hipFreeAsync(dinfo, s);
+        dinv = nullptr;
+        dinfo = nullptr;
+    }
+    int init(int NB, hipStream_t s, bool stream_ordered = false) {
+        async = stream_ordered;
+        const size_t b = (size_t)(NB / 64) * 64 * 64 * sizeof(double);
+        if (async) {
+            HIPTRY(hipMallocAsync((void **)&dinv, b, s));
+            HIPTRY(hipMallocAsync((void **)&dinfo, sizeof(int), s));
+        } else {
+            HIPTRY(hipMalloc(&dinv, b));
+            HIPTRY(hipMalloc(&dinfo, sizeof(int)));
+        }
         int big = INT_MAX;
         HIPTRY(hipMemcpyAsync(dinfo, &big, sizeof(int), hipMemcpyHostToDevice, s));
         return OOCDAG_OK;
     }
 };
+
+// Process-wide pool of pinned host staging buffers (the out-of-core slots,
+// the host-tiles generator): handed out best-fit, returned after use, so
+// repeated calls -- benchmark steps, the server's out-of-core route --
+// allocate page-locked memory once instead of per call.  At most
+// kPinnedKeep idle buffers are kept; oocdag_release_staging frees them.
+constexpr int kPinnedKeep = 10;
+std::mutex g_pin_mu;
+std::vector<std::pair<size_t, double *>> g_pin_free;   // idle buffers
+std::unordered_map<double *, size_t> g_pin_size;        // size of every pool buffer
+
+double *pinned_get(size_t bytes) {
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        int best = -1;
+        for (int i = 0; i < (int)g_pin_free.size(); ++i)
+            if (g_pin_free[i].first >= bytes &&
+                (best < 0 || g_pin_free[i].first < g_pin_free[best].first))
+                best = i;
+        if (best >= 0) {
+            double *p = g_pin_free[best].second;
+            g_pin_free.erase(g_pin_free.begin() + best);
+            return p;
+        }
+    }
+    double *p = nullptr;
+    if (hipHostMalloc((void **)&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pin_size[p] = bytes;
+    return p;
+}
+
+void pinned_put(double *p) {
+    if (!p) return;
+    double *drop = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        g_pin_free.emplace_back(g_pin_size[p], p);
+        if ((int)g_pin_free.size() > kPinnedKeep) {
+            int k = 0;   // drop the smallest idle buffer
+            for (int i = 1; i < (int)g_pin_free.size(); ++i)
+                if (g_pin_free[i].first < g_pin_free[k].first) k = i;
+            drop = g_pin_free[k].second;
+            g_pin_free.erase(g_pin_free.begin() + k);
+            g_pin_size.erase(drop);
+        }
+    }
+    if (drop) (void)hipHostFree(drop);
+}
+
+// Frees every idle pooled buffer; returns how many.
+int pinned_trim() {
+    std::vector<double *> v;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        for (auto &e : g_pin_free) {
+            v.push_back(e.second);
+            g_pin_size.erase(e.second);
+        }
+        g_pin_free.clear();
+    }
+    for (double *p : v) (void)hipHostFree(p);
+    return (int)v.size();
+}
 
 // Device tiles that tile column c of the next panel (Ym2 tile rows, packed
 // from tile 0) occupies overlap tile columns 0..k of the current panel (Ym
@@ -564,36 +647,47 @@ int cholesky_ooc(double *A, int N, int NB, int LDA, size_t memsize, int *info, d
     // for the column-major array) through pinned slots
     const bool direct = tiles_host && host_pinned(A);
 
+    // Everything this call allocates is released by one guard after its own
+    // four streams have drained (on an early return too): device memory is
+    // stream-ordered (hipMallocAsync / hipFreeAsync, no device-wide
+    // synchronization, so concurrent callers on other streams -- the
+    // server's slots -- are not stalled) and the pinned staging slots go
+    // back to a process-wide pool instead of hipHostFree.
     double *dY = nullptr;
-    if (hipMalloc(&dY, (size_t)pl.tiles_needed * nb2 * sizeof(double)) != hipSuccess) {
+    double *hslot_buf[2] = {nullptr, nullptr}, *dslot_buf[ND] = {};
+    std::vector<hipEvent_t> events;   // every event made below, destroyed on return
+    Scratch sc;
+    struct Release {
+        Streams *st;
+        double **dY;
+        Scratch *sc;
+        double **h, **d;
+        std::vector<hipEvent_t> *ev;
+        ~Release() {
+            for (hipStream_t q : {st->h2d, st->bulk, st->d2h, st->comp})
+                if (q) (void)hipStreamSynchronize(q);
+            if (*dY) (void)hipFreeAsync(*dY, st->comp);
+            sc->release(st->comp);
+            if (st->comp) (void)hipStreamSynchronize(st->comp);
+            for (int i = 0; i < 2; ++i) pinned_put(h[i]);
+            for (int i = 0; i < ND; ++i) pinned_put(d[i]);
+            for (hipEvent_t e : *ev) (void)hipEventDestroy(e);
+        }
+    } release{&st, &dY, &sc, hslot_buf, dslot_buf, &events};
+    if (hipMallocAsync((void **)&dY, (size_t)pl.tiles_needed * nb2 * sizeof(double), st.comp) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        dY = nullptr;
         set_err("device allocation of %lld tiles failed", pl.tiles_needed);
         return OOCDAG_ENOMEM;
     }
-    struct FreeDev {
-        double *p;
-        ~FreeDev() { (void)hipFree(p); }
-    } fy{dY};
-    Scratch sc;
-    int rc = sc.init(NBs, st.comp);
+    int rc = sc.init(NBs, st.comp, true);
     if (rc) return rc;
 
     // pinned slots: 2 for host->device, ND for device->host
-    double *hslot_buf[2] = {nullptr, nullptr}, *dslot_buf[ND] = {};
-    std::vector<hipEvent_t> events;   // every event made below, destroyed on return
-    struct Cleanup {
-        double **h, **d;
-        std::vector<hipEvent_t> *ev;
-        ~Cleanup() {
-            for (int i = 0; i < 2; ++i)
-                if (h[i]) (void)hipHostFree(h[i]);
-            for (int i = 0; i < ND; ++i)
-                if (d[i]) (void)hipHostFree(d[i]);
-            for (hipEvent_t e : *ev) (void)hipEventDestroy(e);
-        }
-    } cleanup{hslot_buf, dslot_buf, &events};
     for (int i = 0; i < 2 + ND && !direct; ++i) {
         double **p = i < 2 ? &hslot_buf[i] : &dslot_buf[i - 2];
-        if (hipHostMalloc(p, col_bytes, hipHostMallocDefault) != hipSuccess) {
+        if (!(*p = pinned_get(col_bytes))) {
             set_err("pinned host allocation of %zu bytes failed", col_bytes);
             return OOCDAG_ENOMEM;
         }
@@ -1066,7 +1160,7 @@ int oocdag_pdmatgen_host_tiles(double *T, int N, int NB, char aform, char diag, 
             for (int i = 0; i < 2; ++i) {
                 if (s[i]) (void)hipStreamSynchronize(s[i]);
                 if (w[i]) (void)hipFree(w[i]);
-                if (p[i]) (void)hipHostFree(p[i]);
+                pinned_put(p[i]);
                 if (s[i]) (void)hipStreamDestroy(s[i]);
             }
         }
@@ -1077,8 +1171,7 @@ int oocdag_pdmatgen_host_tiles(double *T, int N, int NB, char aform, char diag, 
             set_err("device allocation of %zu bytes failed", cap * sizeof(double));
             return OOCDAG_ENOMEM;
         }
-        if (!direct && hipHostMalloc(&pin[i], cap * sizeof(double), hipHostMallocDefault) !=
-                           hipSuccess) {
+        if (!direct && !(pin[i] = pinned_get(cap * sizeof(double)))) {
             set_err("pinned host allocation of %zu bytes failed", cap * sizeof(double));
             return OOCDAG_ENOMEM;
         }
@@ -1671,6 +1764,8 @@ int oocdag_profile_bytes(double *bytes) {
     if (bytes) *bytes = g_prof.total_bytes;
     return OOCDAG_OK;
 }
+
+int oocdag_release_staging(void) { return pinned_trim(); }
 
 int oocdag_set_progress(int every_columns) {
     g_progress_every = every_columns > 0 ? every_columns : 0;

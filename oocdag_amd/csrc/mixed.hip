@@ -82,6 +82,19 @@ __global__ void k_finish_diag(double *Y, int N, int nb, int bb, double mu) {
     }
 }
 
+// The same for one diagonal tile T of tile row/column t.
+__global__ void k_finish_diag_at(double *T, int N, int nb, int t, double mu) {
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nb * nb;
+         idx += gridDim.x * blockDim.x) {
+        int r = idx % nb, c = idx / nb;
+        int g = t * nb + r;
+        if (r == c)
+            T[idx] = (g < N) ? T[idx] + mu : 1.0;
+        else if (r < c)
+            T[idx] = T[c + (long long)r * nb];
+    }
+}
+
 // Row sums of |A| over the full symmetric matrix from its packed lower tiles
 // (diagonal tiles stored full).  One block per 64 rows r0..r0+63 of tile row
 // ti reads exactly those rows of A: the row part from tiles (ti, 0..ti), a
@@ -210,6 +223,54 @@ hipError_t gram_tiles(double *dY, int N, int NB, double mu, unsigned long long s
     (void)hipFreeAsync(G, s);
     hipError_t e2 = hipStreamSynchronize(s);
     return e != hipSuccess ? e : e2;
+}
+
+// G[r0 + r, c0 + c] of the Gram generator's N x N normal matrix (column
+// index c*N + r of its stream; 0 past N) into a rows x cols block (ld).
+__global__ void k_fill_normal_at(double *D, long long rows, long long cols, long long ld,
+                                 long long r0, long long c0, unsigned long long seed, long long N) {
+    const long long total = rows * cols;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long r = idx % rows, c = idx / rows, gr = r0 + r, gc = c0 + c;
+        D[r + c * ld] = (gr < N && gc < N)
+                            ? normal_at(seed, 1ull, (unsigned long long)(gc * N + gr))
+                            : 0.0;
+    }
+}
+
+// Tile column j of the Gram matrix A = G G^T / N + mu I (the matrix
+// gram_tiles writes) into `col`: tiles j..bb-1 of edge NB, consecutive,
+// ld = NB, the diagonal tile stored full.  G is streamed in blocks of kw
+// columns through Gp ((bb - j) * NB x kw), so neither G nor A has to fit in
+// device memory; the K sum runs block by block (values equal gram_tiles'
+// up to the rounding of that blocking).
+hipError_t gram_tile_column(double *col, int N, int NB, int j, double mu, unsigned long long seed,
+                            double *Gp, int kw, hipStream_t s) {
+    const int bb = (N + NB - 1) / NB;
+    const long long rows = (long long)(bb - j) * NB, Np = (long long)bb * NB;
+    hipError_t e;
+    for (long long k0 = 0; k0 < Np; k0 += kw) {
+        const int w = (int)std::min<long long>(kw, Np - k0);
+        hipLaunchKernelGGL(k_fill_normal_at, dim3(4096), dim3(256), 0, s, Gp, rows, (long long)w,
+                           rows, (long long)j * NB, k0, seed, (long long)N);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        Gemm g;
+        g.A = plain(Gp, rows);
+        g.B = g.A;   // the top NB rows: G rows of tile j
+        g.C = panel(col, NB, bb - j, 0);
+        g.M = (int)rows;
+        g.N = NB;
+        g.K = w;
+        g.alpha = 1.0 / (double)N;
+        g.beta = k0 == 0 ? 0.0 : 1.0;
+        g.lower = 0;
+        if ((e = launch_gemm(g, GEMM_GENERAL, s)) != hipSuccess) return e;
+    }
+    // the diagonal tile: lower triangle mirrored, mu on the diagonal
+    // (1 on the padded diagonal past N); k_finish_diag on a one-tile layout
+    hipLaunchKernelGGL(k_finish_diag_at, dim3(64), dim3(256), 0, s, col, N, NB, j, mu);
+    return hipGetLastError();
 }
 
 hipError_t normal_block(double *dB, int rows, int cols, long long ld, unsigned long long seed,
@@ -741,8 +802,22 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
     return rc;
 }
 
+// PDLASCHK's residual (LIN/pdlaschk.f:22-26,186,295) from per-right-hand-side
+// norms: max_k ||b_k - A x_k||_inf / (||x_k||_inf * ||A||_inf * eps * N),
+// eps = 2^-53 (PDLAMCH('eps')).
+double pdlaschk_resid(const double *rnorm, const double *xnorm, int nrhs, double anorm, int N) {
+    const double div = anorm * std::ldexp(1.0, -53) * (double)N;
+    double r = 0.0;
+    for (int k = 0; k < nrhs; ++k) {
+        const double q = rnorm[k] == 0.0 ? 0.0 : rnorm[k] / (xnorm[k] * div);
+        r = (q > r || !(q == q)) ? q : r;
+    }
+    return r;
+}
+
 int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX, int ldx,
-             double tol, int max_iter, hipStream_t s, int *iters, double *eta_out) {
+             double tol, int max_iter, hipStream_t s, int *iters, double *eta_out,
+             double *lres_out) {
     const int N = f.N, NB = f.NB, bb = (N + NB - 1) / NB;
     const long long Np = (long long)bb * NB;
     const long long ldt = nrhs + (nrhs & 1);
@@ -817,6 +892,7 @@ int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX,
             eta = 0;
             for (int k = 0; k < nrhs; ++k) eta = std::max(eta, hr[k] / (f.Anorm * hx[k] + bnorm[k]));
             if (!(eta == eta)) eta = INFINITY;
+            if (lres_out) *lres_out = pdlaschk_resid(hr.data(), hx.data(), nrhs, f.Anorm, N);
             if (eta <= tol) break;
             if (it >= max_iter || !(eta < 0.5 * prev)) {
                 rc = 1;  // stalled
@@ -857,15 +933,16 @@ int mp_factor_solve(const double *dA, int N, int NB, const double *dB, int nrhs,
             break;
         }
         int it = 0;
-        double eta = INFINITY;
+        double eta = INFINITY, lres = INFINITY;
         if (!f.info) {
             const double ts = now();
-            rc = mp_solve(f, dB, nrhs, ldb, dX, ldx, tol, max_iter, s, &it, &eta);
+            rc = mp_solve(f, dB, nrhs, ldb, dX, ldx, tol, max_iter, s, &it, &eta, &lres);
             st->solve_seconds += now() - ts;
             if (rc < 0) break;
         }
         st->iterations += it;
         st->backward_error = eta;
+        st->solve_resid = lres;
         if ((!f.info && rc == 0) || prec == 0) {
             st->precision_used = prec;
             rc = OOCDAG_OK;
@@ -883,6 +960,71 @@ int mp_factor_solve(const double *dA, int N, int NB, const double *dB, int nrhs,
     return rc;
 }
 
+namespace {
+
+// Copies the lower triangle of every diagonal slot (edge ts) of packed
+// lower tiles to its upper triangle: the matrix is then defined by its lower
+// triangle alone.  Writes only strict-upper entries, reads only lower ones.
+__global__ void k_symm_diag(double *Y, int ts, int bb) {
+    const int t = blockIdx.y;
+    double *T = Y + (t + (long long)t * bb - ((long long)t * (t + 1)) / 2) * (long long)ts * ts;
+    const long long n = (long long)ts * ts;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(idx % ts), c = (int)(idx / ts);
+        if (r < c) T[idx] = T[c + (long long)r * ts];
+    }
+}
+
+}  // namespace
+
+// PDLASCHK (LIN/pdlaschk.f:22-26,186,295) on the device: R = B - A X with A
+// symmetric in packed lower tiles (logical edge NB in slots of tile_edge(NB),
+// diagonal tiles full), then the per-right-hand-side infinity norms of R
+// and X and ||A||_inf, all reductions in a fixed order.
+int solve_resid_dev(const double *dA, int N, int NB, const double *dX, int nrhs, long long ldx,
+                    const double *dB, long long ldb, double *resid, hipStream_t s) {
+    const int ts = tile_edge(NB), bb = (N + NB - 1) / NB;
+    const long long Np = (long long)bb * ts, ldt = nrhs + (nrhs & 1);
+    const size_t vb = (size_t)(ldt * Np) * 8;
+    double *Xt = nullptr, *Rt = nullptr, *nrm = nullptr, *an = nullptr;
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    int rc = OOCDAG_OK;
+    do {
+        if (!ok(hipMalloc(&Xt, vb)) || !ok(hipMalloc(&Rt, vb)) ||
+            !ok(hipMalloc(&nrm, (size_t)2 * ldt * 8)) || !ok(hipMalloc(&an, (size_t)Np * 8))) {
+            rc = OOCDAG_ENOMEM;
+            break;
+        }
+        std::vector<double> hx(nrhs), hr(nrhs), ha(Np);
+        if (!ok(hipMemsetAsync(Xt, 0, vb, s)) || !ok(hipMemsetAsync(Rt, 0, vb, s)) ||
+            !ok(launch_transpose(dX, ldx, Xt, ldt, N, nrhs, s, NB, ts, 1)) ||
+            !ok(launch_transpose(dB, ldb, Rt, ldt, N, nrhs, s, NB, ts, 1)) ||
+            !ok(symm_t(dA, (int)Np, ts, Xt, Rt, ldt, nrhs, -1.0, 1.0, s))) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Xt, ldt, nrhs, (int)Np, nrm);
+        hipLaunchKernelGGL(k_rows_absmax, dim3(nrhs), dim3(256), 0, s, Rt, ldt, nrhs, (int)Np,
+                           nrm + ldt);
+        hipLaunchKernelGGL(k_tile_rowabs, dim3(ts / 64, bb), dim3(256), 0, s, dA, ts, bb, an);
+        if (!ok(hipGetLastError()) ||
+            !ok(hipMemcpyAsync(hx.data(), nrm, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipMemcpyAsync(hr.data(), nrm + ldt, nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipMemcpyAsync(ha.data(), an, (size_t)Np * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipStreamSynchronize(s))) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        double anorm = 0.0;   // over the matrix's rows only, not the slot padding
+        for (int g = 0; g < N; ++g) anorm = std::max(anorm, ha[(size_t)(g / NB) * ts + g % NB]);
+        *resid = pdlaschk_resid(hr.data(), hx.data(), nrhs, anorm, N);
+    } while (0);
+    for (double *p : {Xt, Rt, nrm, an})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
 }  // namespace ooc
 
 // ===========================================================================
@@ -891,6 +1033,46 @@ int mp_factor_solve(const double *dA, int N, int NB, const double *dB, int nrhs,
 #include "../../include/oocdag.h"
 
 using namespace ooc;
+
+extern "C" int oocdag_solve_resid_tiles(const double *dA, int N, int NB, const double *dX,
+                                        int nrhs, int ldx, const double *dB, int ldb,
+                                        double *resid) {
+    if (!dA || !dX || !dB || !resid || N <= 0 || NB <= 0 || nrhs <= 0 || ldx < N || ldb < N)
+        return OOCDAG_EARG;
+    return solve_resid_dev(dA, N, NB, dX, nrhs, ldx, dB, ldb, resid, nullptr);
+}
+
+extern "C" int oocdag_solve_resid(const double *A, int N, int LDA, const double *X, int nrhs,
+                                  int LDX, const double *B, int LDB, double *resid) {
+    if (!A || !X || !B || !resid || N <= 0 || nrhs <= 0 || LDA < N || LDX < N || LDB < N)
+        return OOCDAG_EARG;
+    const int NB = N >= 4096 ? 512 : 128;
+    const int bb = (N + NB - 1) / NB;
+    double *dY = nullptr, *dX = nullptr, *dB = nullptr;
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    int rc = OOCDAG_OK;
+    do {
+        if (!ok(hipMalloc(&dY, oocdag_tiles_elems(N, NB) * 8)) ||
+            !ok(hipMalloc(&dX, (size_t)N * nrhs * 8)) || !ok(hipMalloc(&dB, (size_t)N * nrhs * 8))) {
+            rc = OOCDAG_ENOMEM;
+            break;
+        }
+        if ((rc = oocdag_tiles_from_host(dY, A, N, NB, LDA))) break;
+        hipLaunchKernelGGL(k_symm_diag, dim3(64, bb), dim3(256), 0, 0, dY, NB, bb);
+        if (!ok(hipGetLastError()) ||
+            !ok(hipMemcpy2D(dX, (size_t)N * 8, X, (size_t)LDX * 8, (size_t)N * 8, nrhs,
+                            hipMemcpyHostToDevice)) ||
+            !ok(hipMemcpy2D(dB, (size_t)N * 8, B, (size_t)LDB * 8, (size_t)N * 8, nrhs,
+                            hipMemcpyHostToDevice))) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        rc = solve_resid_dev(dY, N, NB, dX, nrhs, N, dB, N, resid, nullptr);
+    } while (0);
+    for (double *p : {dY, dX, dB})
+        if (p) (void)hipFree(p);
+    return rc;
+}
 
 extern "C" int oocdag_gram_tiles(double *dY, int N, int NB, double mu, unsigned long long seed,
                                  void *stream) {

@@ -210,9 +210,16 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
 void mp_release(MpFactor &f, hipStream_t s);
 // refinement solve: 0 converged, 1 stalled, < 0 error
 int mp_solve(const MpFactor &f, const double *dB, int nrhs, int ldb, double *dX, int ldx,
-             double tol, int max_iter, hipStream_t s, int *iters, double *eta_out);
+             double tol, int max_iter, hipStream_t s, int *iters, double *eta_out,
+             double *lres_out = nullptr);
+// PDLASCHK solve residual max_k ||r_k|| / (||x_k|| ||A|| eps N) from norms.
+double pdlaschk_resid(const double *rnorm, const double *xnorm, int nrhs, double anorm, int N);
 hipError_t gram_tiles(double *dY, int N, int NB, double mu, unsigned long long seed,
                       hipStream_t s);
+// Tile column j (tiles j..bb-1, ld NB, diagonal tile full) of the Gram
+// matrix gram_tiles writes, with G streamed through Gp ((bb-j)*NB x kw).
+hipError_t gram_tile_column(double *col, int N, int NB, int j, double mu, unsigned long long seed,
+                            double *Gp, int kw, hipStream_t s);
 hipError_t normal_block(double *dB, int rows, int cols, long long ld, unsigned long long seed,
                         unsigned long long stream, hipStream_t s);
 hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt, long long ld,

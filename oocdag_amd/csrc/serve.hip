@@ -15,10 +15,13 @@
 //          the factor stays resident in the cache keyed by matrix id, so a
 //          later request with the same matrix only runs the refinement solve;
 //   out-of-core: a request whose working set exceeds the whole budget is
-//          factored in FP64 from pinned host memory under that budget
-//          (oocdag_cholesky) and solved by streaming the factor's tile
-//          columns through the device.
-// Every result carries its normwise backward error in FP64.
+//          generated tile column by tile column into host memory, factored
+//          in FP64 out of core (oocdag_cholesky_host_tiles) under the part
+//          of the budget it reserved, solved and checked by streaming the
+//          factor's and the matrix's tile columns through the device; at no
+//          point does it hold more HBM than its reservation.
+// Every result carries its normwise backward error and the PDLASCHK solve
+// residual (LIN/pdlaschk.f:186,295) in FP64.
 #include "ooc_internal.h"
 #include "../../include/oocdag.h"
 
@@ -64,15 +67,6 @@ bool is_small(const oocdag_request &r) { return r.n <= SMALL_N && r.nrhs <= 64; 
 // steps on this GPU (measured): such requests are factored in FP64.
 constexpr int MP_MIN_N = 8192;
 
-// row sums of |A| for a column block (rows < n, w columns, ld)
-__global__ void k_rowabs_block(const double *A, long long ld, int n, int w, double *rowsum) {
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < w; ++c) s += fabs(A[r + (long long)c * ld]);
-        rowsum[r] += s;
-    }
-}
-
 size_t tiles_bytes(int n, int nb) {
     size_t bb = (size_t)((n + nb - 1) / nb);
     return bb * (bb + 1) / 2 * nb * nb * 8;
@@ -85,7 +79,7 @@ struct Job {
 };
 
 struct Done {
-    oocdag_result r;
+    oocdag_result r{};
     std::vector<double> x;
 };
 
@@ -97,19 +91,6 @@ struct CacheEntry {
     int users = 0;
     bool ready = false;
 };
-
-__global__ void k_sum(const double *x, long long n, double *out) {
-    __shared__ double red[256];
-    double s = 0.0;
-    for (long long i = blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) s += x[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(out, red[0]);
-}
 
 }  // namespace
 
@@ -141,6 +122,27 @@ struct oocdag_server {
         // A tiles + factor + low-precision columns + Gram temporary (n^2)
         return tiles_bytes(r.n, nb) + mp_factor_bytes(r.n, nb) + np * np * 8 + solve;
     }
+    // The out-of-core route's reservation: at least what it cannot run
+    // without (two tile columns for the factorization, as the reference's
+    // budget check asks, oocdag.cpp:248-250; one tile column, a G panel and
+    // the right-hand sides for generation, solve and residual), otherwise
+    // half the budget.
+    static size_t ooc_min_bytes(const oocdag_request &r) {
+        const int nb = tile_nb(r.n);
+        const size_t bb = (size_t)((r.n + nb - 1) / nb), np = bb * nb;
+        const size_t col = bb * nb * nb * 8;
+        const size_t factor = 2 * col + (size_t)nb * 64 * 8 + 4096;
+        const size_t route = 2 * col + 4 * (size_t)(r.nrhs + 2) * np * 8 + np * 64 * 8 + 65536;
+        return std::max(factor, route);
+    }
+    size_t ooc_bytes(const oocdag_request &r) const {
+        return std::min(budget, std::max(ooc_min_bytes(r), budget / 2));
+    }
+    // a request no admission can ever satisfy
+    bool never_fits(const oocdag_request &r) const {
+        if (is_small(r)) return small_bytes(r) > budget;
+        return large_bytes(r, false) > budget && ooc_min_bytes(r) > budget;
+    }
     CacheEntry *find(long long id) {
         for (auto it = cache.begin(); it != cache.end(); ++it)
             if (it->id == id) {
@@ -169,16 +171,20 @@ struct oocdag_server {
         return true;
     }
 
-    void finish(long long ticket, Done &&d) {
-        std::lock_guard<std::mutex> lk(mu);
-        done[ticket] = std::move(d);
-        cv_done.notify_all();
+    // Results of the request(s) a worker is serving, kept in the worker
+    // thread's outbox and published after the worker has returned the
+    // request's reservation to the budget, so a client that sees its result
+    // also sees the memory released.
+    void finish(long long ticket, Done &&d) { outbox().emplace_back(ticket, std::move(d)); }
+    static std::vector<std::pair<long long, Done>> &outbox() {
+        thread_local std::vector<std::pair<long long, Done>> box;
+        return box;
     }
 
     void worker(int slot);
     void run_small(std::vector<Job> &jobs, hipStream_t s, double t_start);
     void run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, hipStream_t s1, double t_start);
-    void run_ooc(Job &j, hipStream_t s, double t_start);
+    void run_ooc(Job &j, size_t reserved, hipStream_t s, double t_start);
 };
 
 namespace {
@@ -203,40 +209,66 @@ void oocdag_server::worker(int slot) {
         {
             std::unique_lock<std::mutex> lk(mu);
             for (;;) {
+                // requests that cannot fit even in an empty budget fail at once
+                for (auto it = pending.begin(); it != pending.end();) {
+                    if (never_fits(it->rq)) {
+                        Done d;
+                        d.r.status = OOCDAG_EBUDGET;
+                        d.r.route = is_small(it->rq) ? 0 : 2;
+                        fill_common(d.r, *it, now(), now());
+                        done[it->ticket] = std::move(d);
+                        cv_done.notify_all();
+                        it = pending.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
                 if (stop && pending.empty()) break;
-                // first admissible request in arrival order
-                for (auto it = pending.begin(); it != pending.end(); ++it) {
-                    const oocdag_request &r = it->rq;
-                    if (is_small(r)) {
-                        // continuous batching: take every pending small request that fits
-                        size_t need = 0;
-                        for (auto jt = pending.begin(); jt != pending.end() && (int)small.size() < MAX_BATCH;) {
-                            if (is_small(jt->rq) && make_room(need + small_bytes(jt->rq), s)) {
-                                need += small_bytes(jt->rq);
-                                small.push_back(*jt);
-                                jt = pending.erase(jt);
-                            } else {
-                                ++jt;
-                            }
+                // continuous batching: every pending small request that fits
+                {
+                    size_t need = 0;
+                    for (auto jt = pending.begin(); jt != pending.end() && (int)small.size() < MAX_BATCH;) {
+                        if (is_small(jt->rq) && make_room(need + small_bytes(jt->rq), s)) {
+                            need += small_bytes(jt->rq);
+                            small.push_back(*jt);
+                            jt = pending.erase(jt);
+                        } else {
+                            ++jt;
                         }
-                        if (!small.empty()) {
-                            used += need;
-                            reserved = need;
-                        }
+                    }
+                    if (!small.empty()) {
+                        used += need;
+                        reserved = need;
                         break;
                     }
+                }
+                // else the first admissible large request in arrival order
+                for (auto it = pending.begin(); it != pending.end(); ++it) {
+                    const oocdag_request &r = it->rq;
+                    if (is_small(r)) continue;
                     CacheEntry *e = find(r.matrix_id);
                     if (e && !e->ready) continue;  // being factored by another slot
                     bool is_hit = e != nullptr;
                     size_t need = large_bytes(r, is_hit);
                     if (!is_hit && need > budget) {
+                        // out of core, inside a reservation of its own
+                        need = ooc_bytes(r);
+                        if (!make_room(need, s)) continue;
+                        used += need;
+                        reserved = need;
                         big = *it;
                         pending.erase(it);
                         have_big = true;
                         ooc_route = true;
                         break;
                     }
-                    if (!make_room(need, s)) continue;
+                    // pin a hit before making room, so the eviction cannot
+                    // pick the entry this request is about to read
+                    if (is_hit) e->users++;
+                    if (!make_room(need, s)) {
+                        if (is_hit) e->users--;
+                        continue;
+                    }
                     used += need;
                     reserved = need;
                     big = *it;
@@ -245,7 +277,6 @@ void oocdag_server::worker(int slot) {
                     hit = is_hit;
                     if (is_hit) {
                         ce = e;
-                        ce->users++;
                     } else {
                         cache.emplace_front();
                         ce = &cache.front();
@@ -265,7 +296,7 @@ void oocdag_server::worker(int slot) {
         if (!small.empty())
             run_small(small, s, t_start);
         else if (ooc_route)
-            run_ooc(big, s, t_start);
+            run_ooc(big, reserved, s, t_start);
         else
             run_large(big, ce, hit, s, s1, t_start);
         {
@@ -288,7 +319,10 @@ void oocdag_server::worker(int slot) {
                 }
             }
             --running;
+            for (auto &r : outbox()) done[r.first] = std::move(r.second);
+            outbox().clear();
             cv_work.notify_all();
+            cv_done.notify_all();
         }
     }
     (void)hipStreamSynchronize(s);
@@ -330,12 +364,13 @@ void oocdag_server::run_small(std::vector<Job> &jobs, hipStream_t s, double t_st
         if (j.rq.precision == 0) prec = 0;
         tol = std::min(tol, j.rq.tol);
     }
-    std::vector<double> berr(k);
+    std::vector<double> berr(k), lres(k);
     std::vector<int> iters(k), status(k);
     oocdag_batch_stats st;
     if (rc == OOCDAG_OK)
         rc = oocdag_batched_solve(k, n.data(), r.data(), a_off.data(), b_off.data(), A, B, X, prec,
-                                  tol, 10, 0, berr.data(), iters.data(), status.data(), &st);
+                                  tol, 10, 0, berr.data(), lres.data(), iters.data(),
+                                  status.data(), &st);
     // systems whose FP32 refinement did not reach tol: redo in FP64
     if (rc == OOCDAG_OK && prec == 1) {
         std::vector<int> redo;
@@ -343,7 +378,8 @@ void oocdag_server::run_small(std::vector<Job> &jobs, hipStream_t s, double t_st
             if (status[i] != 0) redo.push_back(i);
         if (!redo.empty()) {
             rc = oocdag_batched_solve(k, n.data(), r.data(), a_off.data(), b_off.data(), A, B, X, 0, tol,
-                                      10, 0, berr.data(), iters.data(), status.data(), &st);
+                                      10, 0, berr.data(), lres.data(), iters.data(),
+                                      status.data(), &st);
             for (int i : redo) out[i].r.fell_back = 1;
             if (!redo.empty()) prec = 0;
         }
@@ -367,6 +403,7 @@ void oocdag_server::run_small(std::vector<Job> &jobs, hipStream_t s, double t_st
         d.r.precision_used = prec;
         d.r.iterations = iters[i];
         d.r.backward_error = berr[i];
+        d.r.solve_resid = lres[i];
         d.x.resize((size_t)q.n * q.nrhs);
         double cs = 0;
         for (int c = 0; c < q.nrhs; ++c)
@@ -420,8 +457,8 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
         // cached factor: refinement solve only; a stall refactors in FP64
         const double ts = now();
         int it = 0;
-        double eta = 0;
-        int r = mp_solve(ce->f, B, q.nrhs, ldb, X, ldb, q.tol, 30, s, &it, &eta);
+        double eta = 0, lres = 0;
+        int r = mp_solve(ce->f, B, q.nrhs, ldb, X, ldb, q.tol, 30, s, &it, &eta, &lres);
         if (r == 1 && ce->f.precision != 0) {
             MpFactor f64 = ce->f;
             f64.W = f64.Z = f64.dinv = nullptr;
@@ -430,13 +467,14 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
             if (rc == OOCDAG_OK) {
                 mp_release(ce->f, s);
                 ce->f = f64;
-                r = mp_solve(ce->f, B, q.nrhs, ldb, X, ldb, q.tol, 30, s, &it, &eta);
+                r = mp_solve(ce->f, B, q.nrhs, ldb, X, ldb, q.tol, 30, s, &it, &eta, &lres);
                 st.fell_back = 1;
             }
         }
         if (r < 0) rc = r;
         st.iterations = it;
         st.backward_error = eta;
+        st.solve_resid = lres;
         st.precision_used = ce->f.precision;
         st.solve_seconds = now() - ts;
     }
@@ -455,6 +493,7 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
     d.r.fell_back = st.fell_back;
     d.r.iterations = st.iterations;
     d.r.backward_error = st.backward_error;
+    d.r.solve_resid = st.solve_resid;
     double cs = 0;
     for (double v : d.x) cs += v;
     d.r.x_checksum = cs;
@@ -467,159 +506,249 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
 
 namespace {
 
-__global__ void k_unpack_tiles_full(const double *Y, int n, int nb, int bb, int tj, double *col,
-                                    long long ldc) {
-    // tile column tj (tiles tj..bb-1) of the packed lower tiles -> rows
-    // [tj*nb, n) x cols [tj*nb, tj*nb + nb) of a column block (ld ldc)
-    const long long base = ((long long)tj * bb - ((long long)tj * (tj - 1)) / 2) * nb * nb;
-    const long long rows = (long long)(bb - tj) * nb;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < rows * nb;
-         idx += (long long)gridDim.x * blockDim.x) {
-        long long t = idx / ((long long)nb * nb), w = idx - t * nb * nb;
-        int r = (int)(w % nb), c = (int)(w / nb);
-        long long gr = (tj + t) * nb + r;
-        if (gr < n && tj * nb + c < n) col[gr - (long long)tj * nb + (long long)c * ldc] = Y[base + idx];
+// ||A||_inf row sums from one tile column Ac of a symmetric matrix (tiles
+// j..bb-1 of edge nb, consecutive, the diagonal tile full), rows offset to
+// the column's first row: the row part of every row of the column, then
+// the column sums below the diagonal tile, which are the row sums of A's
+// rows j*nb.. over the columns right of the tile.  Two launches on one
+// stream, one thread per row or column: a fixed summation order.
+__global__ void k_tilecol_rowabs(const double *Ac, int nb, int ntile, double *rowabs) {
+    const long long rows = (long long)ntile * nb;
+    for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < rows;
+         r += (long long)gridDim.x * blockDim.x) {
+        const long long t = r / nb, rt = r - t * nb;
+        const double *p = Ac + t * nb * nb + rt;
+        double s = 0.0;
+        for (int c = 0; c < nb; ++c) s += fabs(p[(long long)c * nb]);
+        rowabs[r] += s;
+    }
+}
+
+__global__ void k_tilecol_colabs(const double *Ac, int nb, int ntile, double *rowabs) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nb) return;
+    double s = 0.0;
+    for (int t = 1; t < ntile; ++t) {
+        const double *p = Ac + (long long)t * nb * nb + (long long)c * nb;
+        for (int r = 0; r < nb; ++r) s += fabs(p[r]);
+    }
+    rowabs[c] += s;
+}
+
+__global__ void k_rows_absmax_t(const double *Xt, long long ldx, int nrhs, long long N,
+                                double *out) {
+    const int k = blockIdx.x;
+    double m = 0.0;
+    for (long long c = threadIdx.x; c < N; c += blockDim.x) m = fmax(m, fabs(Xt[k + c * ldx]));
+    __shared__ double red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (threadIdx.x < s2) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s2]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+}
+
+// Parallel host copy (the matrix kept for the residual).
+void host_copy(double *dst, const double *src, size_t n) {
+    const size_t chunk = (size_t)1 << 20;
+    const long long nch = (long long)((n + chunk - 1) / chunk);
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < nch; ++c) {
+        const size_t o = (size_t)c * chunk;
+        memcpy(dst + o, src + o, std::min(chunk, n - o) * sizeof(double));
     }
 }
 
 }  // namespace
 
-void oocdag_server::run_ooc(Job &j, hipStream_t s, double t_start) {
+// A request too large for the budget: every phase keeps its device memory
+// inside `reserved` bytes.
+//   1. A's tile columns are generated one at a time on the device (G
+//      streamed in column blocks) into host packed tiles T, and copied to
+//      Ah for the residual;
+//   2. T is factored out of core in FP64 (oocdag_cholesky_host_tiles, the
+//      device budget = the reservation);
+//   3. L L^T X = B is solved on transposed right-hand sides X^T (nrhs x N)
+//      resident on the device, L streamed one tile column at a time:
+//      forward over j (X_j <- X_j L_jj^-T, then X_i -= X_j L_ij^T below),
+//      backward over j descending (X_j -= sum_i>j X_i L_ij, then
+//      X_j <- X_j L_jj^-1);
+//   4. R = B - A X and ||A||_inf with A's tile columns streamed from Ah
+//      (tile column j gives rows j.. through L-like products and, by
+//      symmetry, the rows of tile j through the tiles below it).
+void oocdag_server::run_ooc(Job &j, size_t reserved, hipStream_t s, double t_start) {
     const oocdag_request &q = j.rq;
-    const int n = q.n, nb = tile_nb(n), bb = (n + nb - 1) / nb, ldb = n + (n & 1);
+    const int n = q.n, nb = tile_nb(n), bb = (n + nb - 1) / nb;
+    const long long Np = (long long)bb * nb, ldt = q.nrhs + (q.nrhs & 1);
+    const int ldb = n + (n & 1);
+    const size_t tile = (size_t)nb * nb;
+    const size_t col_elems = (size_t)bb * tile;
     Done d;
     d.r.route = 2;
     d.r.batch_size = 1;
     d.r.precision_used = 0;
     int rc = OOCDAG_OK;
-    // 1. materialize A in pinned host memory (column-major, full symmetric)
-    double *Ah = nullptr, *Y = nullptr, *col = nullptr;
-    std::vector<double> diag(n);
-    if (hipHostMalloc((void **)&Ah, (size_t)n * n * 8, hipHostMallocDefault) != hipSuccess) rc = OOCDAG_ENOMEM;
-    if (rc == OOCDAG_OK && (hipMallocAsync((void **)&Y, tiles_bytes(n, nb), s) != hipSuccess ||
-                            hipMallocAsync((void **)&col, (size_t)n * nb * 8, s) != hipSuccess))
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    std::vector<double> T(oocdag_tiles_elems(n, nb)), Ah;
+    double *col = nullptr, *Gp = nullptr;
+    // 1. generation
+    if (!ok(hipMallocAsync((void **)&col, col_elems * 8, s)) ||
+        !ok(hipMallocAsync((void **)&Gp, col_elems * 8, s)))
         rc = OOCDAG_ENOMEM;
-    if (rc == OOCDAG_OK && gram_tiles(Y, n, nb, q.mu, q.seed, s) != hipSuccess) rc = OOCDAG_EHIP;
-    for (int tj = 0; tj < bb && rc == OOCDAG_OK; ++tj) {
-        const int c0 = tj * nb, w = std::min(nb, n - c0);
-        hipLaunchKernelGGL(k_unpack_tiles_full, dim3(4096), dim3(256), 0, s, Y, n, nb, bb, tj, col, (long long)n);
-        if (hipMemcpy2DAsync(Ah + (size_t)c0 * n + c0, (size_t)n * 8, col, (size_t)n * 8,
-                             (size_t)(n - c0) * 8, w, hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
+    for (int jt = 0; jt < bb && rc == OOCDAG_OK; ++jt) {
+        if (!ok(gram_tile_column(col, n, nb, jt, q.mu, q.seed, Gp, nb, s)) ||
+            !ok(hipMemcpyAsync(T.data() + (size_t)tile_index(jt, jt, bb, 1) * tile, col,
+                               (size_t)(bb - jt) * tile * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipStreamSynchronize(s)))
             rc = OOCDAG_EHIP;
     }
-    if (Y) (void)hipFreeAsync(Y, s);
+    if (Gp) (void)hipFreeAsync(Gp, s);
     if (col) (void)hipFreeAsync(col, s);
     (void)hipStreamSynchronize(s);
     if (rc == OOCDAG_OK) {
-        for (int c = 0; c < n; ++c) {
-            diag[c] = Ah[(size_t)c * n + c];
-            for (int r = c + 1; r < n; ++r) Ah[(size_t)r * n + c] = Ah[(size_t)c * n + r];
-        }
+        Ah.resize(T.size());
+        host_copy(Ah.data(), T.data(), T.size());
     }
     const double t_input = now() - t_start;
-    // 2. FP64 out-of-core factorization under this server's budget
+    // 2. FP64 out-of-core factorization inside the reservation
     if (rc == OOCDAG_OK) {
         int info = 0;
-        double t = oocdag_cholesky(Ah, n, nb, n, std::max(budget, (size_t)4 * bb * nb * nb * 8), &info);
+        double t = oocdag_cholesky_host_tiles(T.data(), n, nb, reserved, &info);
         if (t < 0 || info) rc = info ? info : OOCDAG_EHIP;
     }
-    // 3. solve by streaming the factor (oocdag_potrs stages it tile column
-    //    by tile column) and 4. FP64 residual against A (upper triangle +
-    //    saved diagonal), both on the device
-    std::vector<double> hb((size_t)n * q.nrhs), hx;
-    double *Bd = nullptr;
-    if (rc == OOCDAG_OK) {
-        if (hipMallocAsync((void **)&Bd, (size_t)ldb * q.nrhs * 8, s) != hipSuccess ||
-            normal_block(Bd, n, q.nrhs, ldb, q.rhs_seed, 2, s) != hipSuccess ||
-            hipMemcpy2DAsync(hb.data(), (size_t)n * 8, Bd, (size_t)ldb * 8, (size_t)n * 8, q.nrhs,
-                             hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            rc = OOCDAG_EHIP;
-        if (Bd) (void)hipFreeAsync(Bd, s);
+    // 3. streamed solve
+    double *Bd = nullptr, *Xt = nullptr, *Rt = nullptr, *dinv = nullptr, *nrm = nullptr,
+           *rowabs = nullptr;
+    const size_t vb = (size_t)(ldt * Np) * 8;
+    if (rc == OOCDAG_OK &&
+        (!ok(hipMallocAsync((void **)&Bd, (size_t)ldb * q.nrhs * 8, s)) ||
+         !ok(hipMallocAsync((void **)&Xt, vb, s)) || !ok(hipMallocAsync((void **)&Rt, vb, s)) ||
+         !ok(hipMallocAsync((void **)&dinv, (size_t)Np * 64 * 8, s)) ||
+         !ok(hipMallocAsync((void **)&nrm, (size_t)3 * ldt * 8, s)) ||
+         !ok(hipMallocAsync((void **)&rowabs, (size_t)Np * 8, s)) ||
+         !ok(hipMallocAsync((void **)&col, col_elems * 8, s))))
+        rc = OOCDAG_ENOMEM;
+    const GemmKind nt = q.nrhs <= 64 ? GEMM_RHS : GEMM_GENERAL;
+    const GemmKind tt = q.nrhs <= 64 ? GEMM_RHS_TB : GEMM_GENERAL_TB;
+    auto load_col = [&](const std::vector<double> &H, int jt) -> bool {
+        return ok(hipMemcpyAsync(col, H.data() + (size_t)tile_index(jt, jt, bb, 1) * tile,
+                                 (size_t)(bb - jt) * tile * 8, hipMemcpyHostToDevice, s));
+    };
+    if (rc == OOCDAG_OK &&
+        (!ok(normal_block(Bd, n, q.nrhs, ldb, q.rhs_seed, 2, s)) ||
+         !ok(hipMemsetAsync(Xt, 0, vb, s)) || !ok(hipMemsetAsync(rowabs, 0, (size_t)Np * 8, s)) ||
+         !ok(launch_transpose(Bd, ldb, Xt, ldt, n, q.nrhs, s)) ||
+         !ok(hipMemcpyAsync(Rt, Xt, vb, hipMemcpyDeviceToDevice, s))))
+        rc = OOCDAG_EHIP;
+    if (rc == OOCDAG_OK) {   // ||b_k||_inf, before X^T overwrites B^T
+        hipLaunchKernelGGL(k_rows_absmax_t, dim3(q.nrhs), dim3(256), 0, s, Xt, ldt, q.nrhs, Np,
+                           nrm + 2 * ldt);
+        if (!ok(hipGetLastError())) rc = OOCDAG_EHIP;
     }
-    if (rc == OOCDAG_OK) {
-        hx = hb;
-        rc = oocdag_potrs(Ah, n, n, hx.data(), q.nrhs, n);
-    }
-    if (rc == OOCDAG_OK) {
-        // A x from the symmetric copy (upper triangle + diagonal) with the
-        // device GEMM, one column block at a time
-        double *Ablk = nullptr, *Xd = nullptr, *R = nullptr, *rsum = nullptr;
-        const int ldr = n + (n & 1);
-        if (hipMallocAsync((void **)&rsum, (size_t)n * 8, s) != hipSuccess ||
-            hipMemsetAsync(rsum, 0, (size_t)n * 8, s) != hipSuccess ||
-            hipMallocAsync((void **)&Ablk, (size_t)ldr * nb * 8, s) != hipSuccess ||
-            hipMallocAsync((void **)&Xd, (size_t)ldr * q.nrhs * 8, s) != hipSuccess ||
-            hipMallocAsync((void **)&R, (size_t)ldr * q.nrhs * 8, s) != hipSuccess)
-            rc = OOCDAG_ENOMEM;
-        std::vector<double> blk((size_t)ldr * nb);
-        if (rc == OOCDAG_OK &&
-            (hipMemcpy2DAsync(Xd, (size_t)ldr * 8, hx.data(), (size_t)n * 8, (size_t)n * 8, q.nrhs,
-                              hipMemcpyHostToDevice, s) != hipSuccess ||
-             hipMemcpy2DAsync(R, (size_t)ldr * 8, hb.data(), (size_t)n * 8, (size_t)n * 8, q.nrhs,
-                              hipMemcpyHostToDevice, s) != hipSuccess))
+    for (int jt = 0; jt < bb && rc == OOCDAG_OK; ++jt) {   // forward
+        const Opnd Lc = panel(col, nb, bb - jt, 0);
+        if (!load_col(T, jt) || !ok(diag_inverses(Lc, nb, dinv + (size_t)jt * nb * 64, s)) ||
+            !ok(trsm_right_lower_t(plain(Xt, ldt, 0, jt * nb), q.nrhs, nb, Lc,
+                                   dinv + (size_t)jt * nb * 64, s))) {
             rc = OOCDAG_EHIP;
-        for (int c0 = 0; c0 < n && rc == OOCDAG_OK; c0 += nb) {
-            const int w = std::min(nb, n - c0);
-            // A[:, c0:c0+w] rebuilt from the strict upper triangle and diagonal
-            for (int c = 0; c < w; ++c) {
-                const int gc = c0 + c;
-                double *dst = blk.data() + (size_t)c * ldr;
-                for (int r = 0; r < gc; ++r) dst[r] = Ah[(size_t)gc * n + r];
-                dst[gc] = diag[gc];
-                for (int r = gc + 1; r < n; ++r) dst[r] = Ah[(size_t)r * n + gc];
-                for (int r = n; r < ldr; ++r) dst[r] = 0.0;
-            }
-            if (hipMemcpyAsync(Ablk, blk.data(), (size_t)ldr * w * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
-                rc = OOCDAG_EHIP;
-                break;
-            }
-            // R -= A[:, c0:c0+w] X[c0:c0+w, :]   (C = A B^T with B = X^T rows)
-            // R -= A[:, c0:c0+w] X[c0:c0+w, :]: operand B(rhs, k) = X(c0 + k, rhs),
-            // i.e. X read transposed
+            break;
+        }
+        if (jt + 1 < bb) {
             Gemm g;
-            g.A = plain(Ablk, ldr);
-            g.B = plain(Xd, ldr, c0, 0);
-            g.C = plain(R, ldr);
-            g.M = n; g.N = q.nrhs; g.K = w;
+            g.A = plain(Xt, ldt, 0, jt * nb);
+            g.B = Lc; g.B.r0 = nb;
+            g.C = plain(Xt, ldt, 0, (jt + 1) * nb);
+            g.M = q.nrhs; g.N = (bb - jt - 1) * nb; g.K = nb;
             g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
-            if (launch_gemm(g, GEMM_GENERAL_TB, s) != hipSuccess) rc = OOCDAG_EHIP;
-            hipLaunchKernelGGL(k_rowabs_block, dim3(64), dim3(256), 0, s, Ablk, (long long)ldr, n, w, rsum);
-            if (hipStreamSynchronize(s) != hipSuccess) rc = OOCDAG_EHIP;
+            if (!ok(launch_gemm(g, nt, s))) rc = OOCDAG_EHIP;
         }
-        std::vector<double> hr((size_t)n * q.nrhs), hrs(n);
-        if (rc == OOCDAG_OK &&
-            (hipMemcpy2DAsync(hr.data(), (size_t)n * 8, R, (size_t)ldr * 8, (size_t)n * 8, q.nrhs,
-                              hipMemcpyDeviceToHost, s) != hipSuccess ||
-             hipMemcpyAsync(hrs.data(), rsum, (size_t)n * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-             hipStreamSynchronize(s) != hipSuccess))
-            rc = OOCDAG_EHIP;
-        if (rc == OOCDAG_OK) {
-            const double an = *std::max_element(hrs.begin(), hrs.end());  // ||A||_inf
-            double eta = 0;
-            for (int k = 0; k < q.nrhs; ++k) {
-                double rn = 0, xn = 0, bn = 0;
-                for (int r = 0; r < n; ++r) {
-                    rn = std::max(rn, std::fabs(hr[r + (size_t)k * n]));
-                    xn = std::max(xn, std::fabs(hx[r + (size_t)k * n]));
-                    bn = std::max(bn, std::fabs(hb[r + (size_t)k * n]));
-                }
-                eta = std::max(eta, rn / (an * xn + bn));
-            }
-            d.r.backward_error = eta;
-        }
-        for (double *p : {Ablk, Xd, R, rsum})
-            if (p) (void)hipFreeAsync(p, s);
-        (void)hipStreamSynchronize(s);
     }
-    if (Ah) (void)hipHostFree(Ah);
+    for (int jt = bb - 1; jt >= 0 && rc == OOCDAG_OK; --jt) {   // backward
+        const Opnd Lc = panel(col, nb, bb - jt, 0);
+        if (!load_col(T, jt)) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        if (jt + 1 < bb) {
+            Gemm g;
+            g.A = plain(Xt, ldt, 0, (jt + 1) * nb);
+            g.B = Lc; g.B.r0 = nb;   // L[(jt+1)*nb.., jt], read transposed
+            g.C = plain(Xt, ldt, 0, jt * nb);
+            g.M = q.nrhs; g.N = nb; g.K = (bb - jt - 1) * nb;
+            g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
+            if (!ok(launch_gemm(g, tt, s))) rc = OOCDAG_EHIP;
+        }
+        if (rc == OOCDAG_OK &&
+            (!ok(trsm_right_lower_n(plain(Xt, ldt, 0, jt * nb), q.nrhs, nb, Lc,
+                                    dinv + (size_t)jt * nb * 64, s)) ||
+             !ok(hipStreamSynchronize(s))))
+            rc = OOCDAG_EHIP;
+    }
+    // 4. residual R^T = B^T - X^T A and ||A||_inf, A streamed from Ah
+    for (int jt = 0; jt < bb && rc == OOCDAG_OK; ++jt) {
+        const Opnd Ac = panel(col, nb, bb - jt, 0);
+        if (!load_col(Ah, jt)) {
+            rc = OOCDAG_EHIP;
+            break;
+        }
+        Gemm g;   // rows jt.. of A (columns of tile jt): R_i -= A_ij X_j
+        g.A = plain(Xt, ldt, 0, jt * nb);
+        g.B = Ac;
+        g.C = plain(Rt, ldt, 0, jt * nb);
+        g.M = q.nrhs; g.N = (bb - jt) * nb; g.K = nb;
+        g.alpha = -1.0; g.beta = 1.0; g.lower = 0;
+        if (!ok(launch_gemm(g, nt, s))) rc = OOCDAG_EHIP;
+        if (rc == OOCDAG_OK && jt + 1 < bb) {   // R_j -= sum_i>j A_ij^T X_i
+            Gemm h;
+            h.A = plain(Xt, ldt, 0, (jt + 1) * nb);
+            h.B = Ac; h.B.r0 = nb;
+            h.C = plain(Rt, ldt, 0, jt * nb);
+            h.M = q.nrhs; h.N = nb; h.K = (bb - jt - 1) * nb;
+            h.alpha = -1.0; h.beta = 1.0; h.lower = 0;
+            if (!ok(launch_gemm(h, tt, s))) rc = OOCDAG_EHIP;
+        }
+        hipLaunchKernelGGL(k_tilecol_rowabs, dim3(256), dim3(256), 0, s, col, nb, bb - jt,
+                           rowabs + (size_t)jt * nb);
+        hipLaunchKernelGGL(k_tilecol_colabs, dim3((nb + 255) / 256), dim3(256), 0, s, col, nb,
+                           bb - jt, rowabs + (size_t)jt * nb);
+        if (rc == OOCDAG_OK && (!ok(hipGetLastError()) || !ok(hipStreamSynchronize(s))))
+            rc = OOCDAG_EHIP;
+    }
+    std::vector<double> hx, hxn(q.nrhs), hrn(q.nrhs), hbn(q.nrhs), hra(Np);
+    if (rc == OOCDAG_OK) {
+        hipLaunchKernelGGL(k_rows_absmax_t, dim3(q.nrhs), dim3(256), 0, s, Xt, ldt, q.nrhs, Np, nrm);
+        hipLaunchKernelGGL(k_rows_absmax_t, dim3(q.nrhs), dim3(256), 0, s, Rt, ldt, q.nrhs, Np,
+                           nrm + ldt);
+        hx.resize((size_t)n * q.nrhs);
+        if (!ok(launch_transpose(Xt, ldt, Bd, ldb, q.nrhs, n, s)) ||
+            !ok(hipMemcpy2DAsync(hx.data(), (size_t)n * 8, Bd, (size_t)ldb * 8, (size_t)n * 8,
+                                 q.nrhs, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipMemcpyAsync(hxn.data(), nrm, q.nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipMemcpyAsync(hrn.data(), nrm + ldt, q.nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipMemcpyAsync(hbn.data(), nrm + 2 * ldt, q.nrhs * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipMemcpyAsync(hra.data(), rowabs, (size_t)Np * 8, hipMemcpyDeviceToHost, s)) ||
+            !ok(hipStreamSynchronize(s)))
+            rc = OOCDAG_EHIP;
+    }
+    if (rc == OOCDAG_OK) {
+        const double an = *std::max_element(hra.begin(), hra.begin() + n);   // ||A||_inf
+        double eta = 0;
+        for (int k = 0; k < q.nrhs; ++k) eta = std::max(eta, hrn[k] / (an * hxn[k] + hbn[k]));
+        d.r.backward_error = eta;
+        d.r.solve_resid = pdlaschk_resid(hrn.data(), hxn.data(), q.nrhs, an, n);
+    }
+    for (double *p : {Bd, Xt, Rt, dinv, nrm, rowabs, col})
+        if (p) (void)hipFreeAsync(p, s);
+    (void)hipStreamSynchronize(s);
     d.r.status = rc;
     d.x = std::move(hx);
     double cs = 0;
     for (double v : d.x) cs += v;
     d.r.x_checksum = cs;
     fill_common(d.r, j, t_start, now());
+    d.r.input_seconds = t_input;
     finish(j.ticket, std::move(d));
 }
 
@@ -630,11 +759,36 @@ extern "C" oocdag_server *oocdag_server_create(size_t hbm_budget_bytes, int slot
     if (slots < 1 || hbm_budget_bytes == 0) return nullptr;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return nullptr;
+    // the FP32/BF16 MFMA row-order probes allocate and synchronize once per
+    // process: run them here, not inside the first request that needs them
+    {
+        hipStream_t ps = nullptr;
+        if (hipStreamCreateWithFlags(&ps, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        const bool probed = mfma_rowmap(0, ps) >= 0 && mfma_rowmap(1, ps) >= 0;
+        (void)hipStreamDestroy(ps);
+        if (!probed) return nullptr;
+    }
     oocdag_server *srv = new oocdag_server();
     srv->budget = hbm_budget_bytes;
     srv->nslots = slots;
     for (int i = 0; i < slots; ++i) srv->workers.emplace_back([srv, i] { srv->worker(i); });
     return srv;
+}
+
+extern "C" int oocdag_server_estimate(const oocdag_request *rq, size_t *hit_bytes,
+                                      size_t *miss_bytes, size_t *cached_bytes,
+                                      size_t *ooc_min_bytes) {
+    if (!rq || rq->n < 1 || rq->nrhs < 1) return OOCDAG_EARG;
+    const bool small = is_small(*rq);
+    const int nb = tile_nb(rq->n);
+    if (hit_bytes) *hit_bytes = small ? oocdag_server::small_bytes(*rq)
+                                      : oocdag_server::large_bytes(*rq, true);
+    if (miss_bytes) *miss_bytes = small ? oocdag_server::small_bytes(*rq)
+                                        : oocdag_server::large_bytes(*rq, false);
+    if (cached_bytes)
+        *cached_bytes = small ? 0 : tiles_bytes(rq->n, nb) + mp_factor_bytes(rq->n, nb);
+    if (ooc_min_bytes) *ooc_min_bytes = small ? 0 : oocdag_server::ooc_min_bytes(*rq);
+    return OOCDAG_OK;
 }
 
 extern "C" long long oocdag_server_submit(oocdag_server *srv, const oocdag_request *rq) {

@@ -74,3 +74,97 @@ def test_server_routes_over_budget_to_out_of_core(gpu):
     big.close()
     assert res2["route"] == 1
     assert np.abs(x - x2).max() / np.abs(x2).max() <= 1e-10
+
+
+def gram_dense(ooc, n, mu, seed):
+    nb = 256 if n < 4096 else 512
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, mu, seed=seed)
+    A = np.zeros((n, n), order="F")
+    ooc.tiles_to_host(buf, A, nb)
+    buf.free()
+    return np.tril(A) + np.tril(A, -1).T
+
+
+def rhs(ooc, n, nrhs, seed):
+    d = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(d, n, nrhs, n, seed=seed, stream_id=2)
+    return d.to_array((n, nrhs))
+
+
+def test_server_out_of_core_route_inside_its_reservation(gpu):
+    """n = 6000 (NB = 512): the packed matrix (163 MB) alone exceeds a
+    96 MiB budget.  The route generates A tile column by tile column, factors
+    it out of core inside its reservation, solves and checks it with streamed
+    tile columns: the solution matches numpy's on the same matrix, the
+    backward error is FP64-level, the PDLASCHK residual <= 10."""
+    ooc = gpu
+    r = dict(matrix_id=3, n=6000, nrhs=4, mu=0.1, seed=91, rhs_seed=92)
+    est = ooc.Server.estimate(**r)
+    budget = 96 << 20
+    assert est["miss"] > budget and est["ooc_min"] <= budget
+    srv = ooc.Server(budget, slots=2)
+    (res, x), = run(srv, [r], prec="fp64")
+    st = srv.stats()
+    srv.close()
+    assert res["route"] == 2 and res["status"] == 0, res
+    assert res["backward_error"] <= 4e-16 and res["solve_resid"] <= 10.0, res
+    assert st["used_bytes"] == 0
+    A = gram_dense(ooc, r["n"], r["mu"], r["seed"])
+    B = rhs(ooc, r["n"], r["nrhs"], r["rhs_seed"])
+    xref = np.linalg.solve(A, B)
+    assert np.abs(x - xref).max() / np.abs(xref).max() <= 1e-10
+
+
+def test_server_cache_hit_is_not_evicted_by_its_own_admission(gpu):
+    """Two slots, a budget that holds matrix 1's cached factor plus a miss
+    for matrix 2 but not also a hit on matrix 1: the hit, admitted while the
+    miss holds its reservation, must neither evict nor read a freed factor.
+    It waits for room and returns the same x as when run alone."""
+    ooc = gpu
+    r1 = dict(matrix_id=1, n=5000, nrhs=2, mu=0.1, seed=11, rhs_seed=12)
+    r2 = dict(matrix_id=2, n=5000, nrhs=2, mu=0.1, seed=21, rhs_seed=22)
+    r3 = dict(r1, rhs_seed=13)
+    e1, e2, e3 = (ooc.Server.estimate(**r) for r in (r1, r2, r3))
+    budget = e1["cached"] + e2["miss"] + e3["hit"] // 2
+    assert e1["miss"] <= budget
+    srv = ooc.Server(budget, slots=2)
+    run(srv, [r1])
+    assert srv.stats()["used_bytes"] == e1["cached"]
+    (res2, _), (res3, x3) = run(srv, [r2, r3])
+    srv.close()
+    assert res2["status"] == 0 and res3["status"] == 0
+    assert res3["cache_hit"] == 1 and res3["route"] == 1
+    alone = ooc.Server(64 * GB, slots=1)
+    run(alone, [r1])
+    (res3a, x3a), = run(alone, [r3])
+    alone.close()
+    assert res3a["cache_hit"] == 1
+    assert np.abs(x3 - x3a).max() / np.abs(x3a).max() <= 1e-12
+
+
+def test_server_fails_requests_that_never_fit(gpu):
+    """A request above the whole budget on every route completes with
+    OOCDAG_EBUDGET instead of blocking the queue; later requests still run
+    and the server shuts down."""
+    ooc = gpu
+    srv = ooc.Server(2 << 20, slots=1)   # 2 MiB
+    t_big = srv.submit(precision="fp64", tol=TOL, matrix_id=1, n=1000, nrhs=8, mu=0.1, seed=1,
+                       rhs_seed=2)
+    t_ok = srv.submit(precision="fp64", tol=TOL, matrix_id=2, n=60, nrhs=1, mu=0.1, seed=3,
+                      rhs_seed=4)
+    (rb, _), (rk, _) = srv.wait(t_big), srv.wait(t_ok)
+    srv.close()
+    assert rb["status"] == ooc.EBUDGET
+    assert rk["status"] == 0 and rk["backward_error"] <= TOL
+
+
+def test_server_results_carry_solve_resid(gpu):
+    ooc = gpu
+    reqs = [dict(matrix_id=0, n=300, nrhs=3, mu=0.1, seed=5, rhs_seed=6),
+            dict(matrix_id=1, n=2500, nrhs=2, mu=1e-4, seed=7, rhs_seed=8)]
+    srv = ooc.Server(16 * GB, slots=2)
+    out = run(srv, reqs)
+    srv.close()
+    for (res, x), r in zip(out, reqs):
+        assert res["status"] == 0 and 0 < res["solve_resid"] <= 10.0, res
