@@ -353,6 +353,17 @@ int oocdag_dtrsm_rltn(int m, int n, const double *L, int ldl, double *B, int ldb
  * even lda in place, otherwise through a padded copy); strict upper
  * triangle untouched.  Synchronous; *info LAPACK-style. */
 int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream);
+/* oocdag_dpotrf_l that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (the 64 x 64 diagonal-block inverses and the info word, 8 MiB) is
+ * a fixed device array of the library, one per GPU, held until the process
+ * ends; the caller passes only A.  Limits: n % 64 == 0, n <= 16384, even
+ * lda (other shapes: OOCDAG_EARG; oocdag_dpotrf_l takes them through a
+ * padded copy, which would need scratch).  Every call on one GPU shares that
+ * array: calls that overlap on a GPU race.  Same results as
+ * oocdag_dpotrf_l, bit for bit. */
+int oocdag_dpotrf_l_noalloc(int n, double *A, int lda, int *info, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Device memory helpers for bindings without a GPU array library.

@@ -68,6 +68,7 @@ SIGNATURES = {
     "oocdag_dtrsm_rltn": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                     C.c_void_p]),
     "oocdag_dpotrf_l": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _ip, C.c_void_p]),
+    "oocdag_dpotrf_l_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _ip, C.c_void_p]),
     "oocdag_potrs": (C.c_int, [_dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int]),
     "oocdag_potrs_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                      C.c_void_p]),

@@ -1444,7 +1444,39 @@ struct DevTmp {
     }
 };
 
-int dpotrf_aligned(int n, double *A, int lda, int *info, hipStream_t s) {
+// ---- the allocation-free variant of oocdag_dpotrf_l ----------------------
+// Its scratch -- the 64 x 64 diagonal-block inverses of the left-looking
+// sweep and the info word -- lives in one file-scope device array of a fixed
+// size instead of hipMalloc'ed memory: kNoAllocMaxN / 64 inverse blocks,
+// then the info word.  Each device holds its own copy; its address is read
+// once per device with hipGetSymbolAddress and only that address is handed
+// to kernels and copies (the array's name is a host-side stand-in).  Every
+// call on one device shares the array: calls that overlap on a device race.
+static constexpr int kNoAllocMaxN = 16384;
+static constexpr size_t kNoAllocInvDoubles = (size_t)(kNoAllocMaxN / 64) * 64 * 64;
+static __device__ __align__(256) double scratch[kNoAllocInvDoubles + 32];
+
+static std::mutex g_scratch_mu;
+static std::unordered_map<int, double *> g_scratch_ptr;   // device -> address of its copy
+
+static double *scratch_on_current_device() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    double *p = nullptr;
+    if (hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+    return p;
+}
+
+static __global__ void k_set_int(int *p, int v) { *p = v; }
+
+static int dpotrf_aligned(int n, double *A, int lda, int *info, hipStream_t s) {
     Scratch sc;
     // the left-looking sweep keeps one 64x64 inverse per block column
     HIPTRY(hipMalloc(&sc.dinv, (size_t)(n / 64) * 64 * 64 * sizeof(double)));
@@ -1453,6 +1485,34 @@ int dpotrf_aligned(int n, double *A, int lda, int *info, hipStream_t s) {
     HIPTRY(hipMemcpyAsync(sc.dinfo, &big, sizeof(int), hipMemcpyHostToDevice, s));
     HIPTRY(factor_column(plain(A, lda), 0, n, n, 0, sc.dinv, sc.dinfo, s));
     return read_info(sc.dinfo, s, n, info);
+}
+
+int oocdag_dpotrf_l_noalloc(int n, double *A, int lda, int *info, void *stream) {
+    g_err.clear();
+    int dummy;
+    if (!info) info = &dummy;
+    *info = 0;
+    if (!A || n < 0 || lda < std::max(1, n)) {
+        set_err("dpotrf_l_noalloc: invalid n or lda");
+        return OOCDAG_EARG;
+    }
+    if (n > kNoAllocMaxN || n % 64 || (lda & 1)) {
+        set_err("dpotrf_l_noalloc: needs n %% 64 == 0, n <= %d and an even lda (n=%d lda=%d); "
+                "use oocdag_dpotrf_l for other shapes", kNoAllocMaxN, n, lda);
+        return OOCDAG_EARG;
+    }
+    if (n == 0) return OOCDAG_OK;
+    double *ws = scratch_on_current_device();
+    if (!ws) {
+        set_err("dpotrf_l_noalloc: the device scratch array is not available on this device");
+        return OOCDAG_EHIP;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int *dinfo = reinterpret_cast<int *>(ws + kNoAllocInvDoubles);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, dinfo, INT_MAX);
+    HIPTRY(hipGetLastError());
+    HIPTRY(factor_column(plain(A, lda), 0, n, n, 0, ws, dinfo, s));
+    return read_info(dinfo, s, n, info);
 }
 
 int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream) {
@@ -1484,7 +1544,8 @@ int oocdag_dpotrf_l(int n, double *A, int lda, int *info, void *stream) {
     return OOCDAG_OK;
 }
 
-int dtrsm_aligned(int m, int n, const double *L, int ldl, double *B, int ldb, hipStream_t s) {
+static int dtrsm_aligned(int m, int n, const double *L, int ldl, double *B, int ldb,
+                         hipStream_t s) {
     // Invert L's 64x64 diagonal blocks in LDS (L itself is not modified),
     // then sweep block columns left to right:
     //   B[:, c] -= B[:, :c] L[c, :c]^T ;  B[:, c] <- B[:, c] D_c^T
