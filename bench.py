@@ -692,7 +692,7 @@ def main() -> int:
                      "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": round(alg_bytes),
-                     "kernel": "k_dgemm_dtv<1,0,true> panel update",
+                     "kernel": "k_dgemm_dtv<1,0,*> panel update (K inside one tile / across tiles)",
                      "launches": res["prof_launches"],
                      "kernel_seconds": round(res["prof_seconds"], 4),
                      "avg_launch_ms": round(res["prof_seconds"] / max(res["prof_launches"], 1) * 1e3,
