@@ -404,11 +404,12 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
 
 // ---------------------------------------------------------------------------
 // Left-looking variant for larger systems.  Block columns of LT = 64:
-//   U. C_j = A[k0:, k0:k0+64] - L[k0:, :k0] L[k0:k0+64, :k0]^T as one LDS-tiled
-//      MFMA GEMM per 128-row tile (4 waves as 2 x 2; every element of the
-//      trailing matrix is read and written once over the whole
-//      factorization, instead of once per block step as in the
-//      right-looking kernel),
+//   U. C_j = A[k0:, k0:k0+64] - L[k0:, :k0] L[k0:k0+64, :k0]^T as one MFMA
+//      GEMM per 128-row tile (4 waves as 2 x 2, operands streamed from
+//      global memory straight into MFMA registers with a ring of loads in
+//      flight; every element of the trailing matrix is read and written
+//      once over the whole factorization, instead of once per block step as
+//      in the right-looking kernel),
 //   D. the 64 x 64 diagonal block of C_j factored in LDS (16-column sub-
 //      blocks: one wave factors, all threads solve and update) and inverted
 //      by 16 x 16 blocks (diagonal blocks first, then one block row of
@@ -420,19 +421,12 @@ __global__ __launch_bounds__(NTB, 1) void k_batched(BatchArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int LT = 64;
 constexpr int LS = LT + 1;
-constexpr int LBK = 16;
-constexpr int LSA = LT + 8;    // staging row stride (elements)
 constexpr int LRT = 128;       // rows per update tile
-constexpr int LSA2 = LRT + 8;
 constexpr int LRMAX = 8;
 
 template <typename FT>
 struct SmemLL {
     union {
-        struct {
-            FT A[1][LBK * LSA2];
-            FT B[1][LBK * LSA];
-        } g;
         struct {
             double D[LT * LS];
             double Di[LT * LS];
@@ -492,7 +486,18 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
             double sk[LRMAX];
 #pragma unroll
             for (int k = 0; k < LRMAX; ++k) sk[k] = 0.0;
-            for (int c = 0; c < w; ++c) {
+            int c = 0;
+            for (; c + 8 <= w; c += 8) {   // 8 independent loads, then the products
+                double wv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) wv[u] = (double)W[i + (long long)(k0 + c + u) * ld];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int k = 0; k < LRMAX; ++k)
+                        if (k < nrhs) sk[k] += wv[u] * y[k * NMAX + k0 + c + u];
+            }
+            for (; c < w; ++c) {
                 const double wv = (double)W[i + (long long)(k0 + c) * ld];
 #pragma unroll
                 for (int k = 0; k < LRMAX; ++k)
@@ -524,7 +529,18 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
             double sk[LRMAX];
 #pragma unroll
             for (int k = 0; k < LRMAX; ++k) sk[k] = 0.0;
-            for (int r = 0; r < w; ++r) {
+            int r = 0;
+            for (; r + 8 <= w; r += 8) {
+                double wv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) wv[u] = (double)col[r + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int k = 0; k < LRMAX; ++k)
+                        if (k < nrhs) sk[k] += wv[u] * y[k * NMAX + k0 + r + u];
+            }
+            for (; r < w; ++r) {
                 const double wv = (double)col[r];
 #pragma unroll
                 for (int k = 0; k < LRMAX; ++k)
@@ -579,38 +595,53 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-                // single-buffered K loop (the other workgroup on the CU
-                // overlaps its loads with these MFMAs)
-                for (int kk = 0; kk < k0; kk += LBK) {
+                // Operands straight from global memory into MFMA registers,
+                // no LDS and no barriers: lane (il, kl) of wave (wm, wn)
+                // feeds MFMA block i with row r0 + wm*64 + i*16 + il of L
+                // and block j with row k0 + wn*32 + j*16 + il, at column
+                // k + kl of the current 4-wide K slab.  A ring of LRING
+                // slabs keeps LAHEAD slabs of loads in flight ahead of the
+                // MFMAs; rows past n are clamped to row n-1 (in bounds; their
+                // products land in rows the epilogue does not write) and
+                // past the last slab the loads repeat it (never used).
+                if (k0 > 0) {
+                    constexpr int LRING = 4, LAHEAD = 3;
+                    int ra_[4], rb_[2];
 #pragma unroll
-                    for (int qd = 0; qd < 8; ++qd) {
-                        const int idx = tid + NTB * qd;
-                        const int r = idx & (LRT - 1), kc = idx >> 7;
-                        S.u.g.A[0][kc * LSA2 + r] =
-                            (r0 + r < n) ? W[(r0 + r) + (long long)(kk + kc) * ld] : (FT)0;
+                    for (int i = 0; i < 4; ++i) ra_[i] = min(r0 + wm * 64 + i * 16 + il, n - 1);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) rb_[j] = min(k0 + wn * 32 + j * 16 + il, n - 1);
+                    const FT *colp = W + (long long)kl * ld;   // column kl of the current slab
+                    const long long step = 4LL * ld;
+                    const int ns = k0 >> 2;                     // slabs (a multiple of 16)
+                    FT va[LRING][4], vb[LRING][2];
+                    auto ld_slab = [&](FT (&xa)[4], FT (&xb)[2]) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) xa[i] = colp[ra_[i]];
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) xb[j] = colp[rb_[j]];
+                    };
+#pragma unroll
+                    for (int q = 0; q < LAHEAD; ++q) {
+                        ld_slab(va[q], vb[q]);
+                        colp += step;
+                        __builtin_amdgcn_sched_barrier(0);
                     }
+                    for (int s0 = 0; s0 < ns; s0 += LRING) {
 #pragma unroll
-                    for (int qd = 0; qd < 4; ++qd) {
-                        const int idx = tid + NTB * qd;
-                        const int r = idx & (LT - 1), kc = idx >> 6;
-                        S.u.g.B[0][kc * LSA + r] =
-                            (k0 + r < n) ? W[(k0 + r) + (long long)(kk + kc) * ld] : (FT)0;
+                        for (int q = 0; q < LRING; ++q) {
+                            const int nx = (q + LAHEAD) & (LRING - 1);
+                            ld_slab(va[nx], vb[nx]);
+                            colp += (s0 + q + LAHEAD + 1 < ns) ? step : 0;
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                for (int j = 0; j < 2; ++j)
+                                    acc[i][j] = mfma4(va[q][i], vb[q][j], acc[i][j]);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
                     }
-                    __syncthreads();
-#pragma unroll
-                    for (int ks = 0; ks < LBK / 4; ++ks) {
-                        const int kq = ks * 4 + kl;
-                        FT fa[4], fb[2];
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) fa[i] = S.u.g.A[0][kq * LSA2 + wm * 64 + i * 16 + il];
-#pragma unroll
-                        for (int j = 0; j < 2; ++j) fb[j] = S.u.g.B[0][kq * LSA + wn * 32 + j * 16 + il];
-#pragma unroll
-                        for (int i = 0; i < 4; ++i)
-#pragma unroll
-                            for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[i], fb[j], acc[i][j]);
-                    }
-                    __syncthreads();
                 }
                 // C = A - acc (lower part only in the diagonal block)
 #pragma unroll
@@ -820,13 +851,27 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                             S.tmp[k * LT + jj] = (jj < jw && k < nrhs) ? X[j0 + jj + (long long)k * ld] : 0.0;
                         }
                         __syncthreads();
-                        if (i < n)
-                            for (int jj = 0; jj < jw; ++jj) {
+                        if (i < n) {
+                            // 8 independent loads of the row before their products
+                            int jj = 0;
+                            for (; jj + 8 <= jw; jj += 8) {
+                                double av[8];
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) av[u] = A[i + (long long)(j0 + jj + u) * ld];
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) {
+                                    as += fabs(av[u]);
+#pragma unroll
+                                    for (int k = 0; k < LRMAX; ++k) rs[k] -= av[u] * S.tmp[k * LT + jj + u];
+                                }
+                            }
+                            for (; jj < jw; ++jj) {
                                 const double aij = A[i + (long long)(j0 + jj) * ld];
                                 as += fabs(aij);
 #pragma unroll
                                 for (int k = 0; k < LRMAX; ++k) rs[k] -= aij * S.tmp[k * LT + jj];
                             }
+                        }
                         __syncthreads();
                     }
                     if (i < n) {
@@ -841,34 +886,33 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                         }
                     }
                 }
-                double e = 0.0, lr = 0.0;
-                S.red[tid] = amax;
-                __syncthreads();
-                for (int s2 = NTB / 2; s2 > 0; s2 >>= 1) {
-                    if (tid < s2) S.red[tid] = fmax(S.red[tid], S.red[tid + s2]);
-                    __syncthreads();
-                }
-                const double An = S.red[0];
-                __syncthreads();
+                // the 1 + 3 * LRMAX maxima: a butterfly inside each wave, one
+                // value at a time, then the four waves' partials from LDS
+                // (max is exact: every thread gets the same values)
+                auto wave_max = [&](double x, int slot) {
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+                    if (lane == 0) S.red[wave * 32 + slot] = x;
+                };
+                wave_max(amax, 0);
 #pragma unroll
                 for (int k = 0; k < LRMAX; ++k) {
-                    if (k >= nrhs) break;
-                    double vals[3] = {rmax[k], xmax[k], bmax[k]};
-                    double out[3];
-#pragma unroll
-                    for (int v = 0; v < 3; ++v) {
-                        S.red[tid] = vals[v];
-                        __syncthreads();
-                        for (int s2 = NTB / 2; s2 > 0; s2 >>= 1) {
-                            if (tid < s2) S.red[tid] = fmax(S.red[tid], S.red[tid + s2]);
-                            __syncthreads();
-                        }
-                        out[v] = S.red[0];
-                        __syncthreads();
-                    }
-                    e = fmax(e, out[0] / (An * out[1] + out[2]));
-                    lr = fmax(lr, out[0] == 0.0 ? 0.0 : out[0] / (An * out[1] * kEps * n));
+                    wave_max(rmax[k], 1 + 3 * k);
+                    wave_max(xmax[k], 2 + 3 * k);
+                    wave_max(bmax[k], 3 + 3 * k);
                 }
+                __syncthreads();
+                auto red4 = [&](int v) {
+                    return fmax(fmax(S.red[v], S.red[32 + v]), fmax(S.red[64 + v], S.red[96 + v]));
+                };
+                const double An = red4(0);
+                double e = 0.0, lr = 0.0;
+                for (int k = 0; k < nrhs; ++k) {
+                    const double rn = red4(1 + 3 * k), xn = red4(2 + 3 * k), bn = red4(3 + 3 * k);
+                    e = fmax(e, rn / (An * xn + bn));
+                    lr = fmax(lr, rn == 0.0 ? 0.0 : rn / (An * xn * kEps * n));
+                }
+                __syncthreads();
                 prev = eta;
                 eta = e;
                 lres = lr;
