@@ -88,6 +88,13 @@ struct Smem {
 
 // Orders LDS traffic between the lanes of one wave (the wave runs in lock
 // step; this only stops the compiler from moving LDS accesses across it).
+// lane l's value of v, broadcast to the whole wave (two v_readlane_b32)
+__device__ __forceinline__ double bcast_lane(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -471,16 +478,26 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
         const FT *Di = Dinv + (long long)kb * LT * LT;   // Di(c, d) at c + d*LT
         for (int idx = tid; idx < w * nrhs; idx += NTB) {
             const int c = idx % w, k = idx / w;
+            const double *yk = y + k * NMAX + k0;
             double s = 0.0;
-            for (int d = 0; d <= c; ++d) s += (double)Di[c + d * LT] * y[k * NMAX + k0 + d];
+            int d = 0;
+            for (; d + 8 <= c + 1; d += 8) {   // 8 independent loads, then the products
+                double dv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) dv[u] = (double)Di[c + (d + u) * LT];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += dv[u] * yk[d + u];
+            }
+            for (; d <= c; ++d) s += (double)Di[c + d * LT] * yk[d];
             S.tmp[k * LT + c] = s;
         }
         __syncthreads();
+        // the block's new values (S.tmp) go to y while the rows below are
+        // updated from S.tmp: disjoint parts of y, one barrier
         for (int idx = tid; idx < w * nrhs; idx += NTB) {
             const int c = idx % w, k = idx / w;
             y[k * NMAX + k0 + c] = S.tmp[k * LT + c];
         }
-        __syncthreads();
         // rows below: one thread per row, all right-hand sides (L read once)
         for (int i = k0 + w + tid; i < n; i += NTB) {
             double sk[LRMAX];
@@ -495,13 +512,13 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
                 for (int u = 0; u < 8; ++u)
 #pragma unroll
                     for (int k = 0; k < LRMAX; ++k)
-                        if (k < nrhs) sk[k] += wv[u] * y[k * NMAX + k0 + c + u];
+                        if (k < nrhs) sk[k] += wv[u] * S.tmp[k * LT + c + u];
             }
             for (; c < w; ++c) {
                 const double wv = (double)W[i + (long long)(k0 + c) * ld];
 #pragma unroll
                 for (int k = 0; k < LRMAX; ++k)
-                    if (k < nrhs) sk[k] += wv * y[k * NMAX + k0 + c];
+                    if (k < nrhs) sk[k] += wv * S.tmp[k * LT + c];
             }
 #pragma unroll
             for (int k = 0; k < LRMAX; ++k)
@@ -514,8 +531,18 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
         const FT *Di = Dinv + (long long)kb * LT * LT;
         for (int idx = tid; idx < w * nrhs; idx += NTB) {
             const int c = idx % w, k = idx / w;
+            const double *yk = y + k * NMAX + k0;
+            const FT *dc = Di + c * LT;   // Di(d, c), d = c..w-1, contiguous
             double s = 0.0;
-            for (int d = c; d < w; ++d) s += (double)Di[d + c * LT] * y[k * NMAX + k0 + d];
+            int d = c;
+            for (; d + 8 <= w; d += 8) {
+                double dv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) dv[u] = (double)dc[d + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += dv[u] * yk[d + u];
+            }
+            for (; d < w; ++d) s += (double)dc[d] * yk[d];
             S.tmp[k * LT + c] = s;
         }
         __syncthreads();
@@ -523,7 +550,6 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
             const int c = idx % w, k = idx / w;
             y[k * NMAX + k0 + c] = S.tmp[k * LT + c];
         }
-        __syncthreads();
         for (int i = tid; i < k0; i += NTB) {
             const FT *col = W + (long long)i * ld + k0;   // L(k0.., i), contiguous
             double sk[LRMAX];
@@ -538,13 +564,13 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
                 for (int u = 0; u < 8; ++u)
 #pragma unroll
                     for (int k = 0; k < LRMAX; ++k)
-                        if (k < nrhs) sk[k] += wv[u] * y[k * NMAX + k0 + r + u];
+                        if (k < nrhs) sk[k] += wv[u] * S.tmp[k * LT + r + u];
             }
             for (; r < w; ++r) {
                 const double wv = (double)col[r];
 #pragma unroll
                 for (int k = 0; k < LRMAX; ++k)
-                    if (k < nrhs) sk[k] += wv * y[k * NMAX + k0 + r];
+                    if (k < nrhs) sk[k] += wv * S.tmp[k * LT + r];
             }
 #pragma unroll
             for (int k = 0; k < LRMAX; ++k)
@@ -552,6 +578,53 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
         }
         __syncthreads();
     }
+}
+
+// The 16 x 16 diagonal sub-block at (s0, s0) of the LDS block D (column
+// stride LS; rows past sw are padding) factored in registers by one wave:
+// lane r holds row s0 + r (padded with an identity), right-looking, the
+// pivot and column j read from the owning lanes (v_readlane, no LDS round
+// trips): the same subtraction order per element as a left-looking dot
+// product, the column scaled by the reciprocal pivot as LAPACK dpotf2 does.
+// Then lane c forms column c of the factor's inverse by forward
+// substitution into Li (Li(i, c) at i + 16c).  Returns the first bad pivot
+// (1-based inside the sub-block) or 0.  Kept out of line: its 48 live
+// doubles must not add to the register pressure of the caller's other phases.
+__device__ __noinline__ int factor16(double *D, int s0, int sw, double *Li, int lane) {
+    double av[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+        av[c] = (lane < sw && c < sw && c <= lane) ? D[(s0 + c) * LS + s0 + lane]
+                                                   : (c == lane ? 1.0 : 0.0);
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const double d = bcast_lane(av[j], j);
+        if (bad == 0 && !(d > 0.0)) bad = j + 1;
+        const double sd = sqrt(d), rd = 1.0 / sd;
+        av[j] = lane > j ? av[j] * rd : (lane == j ? sd : 0.0);
+#pragma unroll
+        for (int k = j + 1; k < 16; ++k) av[k] -= av[j] * bcast_lane(av[j], k);
+    }
+    double yv[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) yv[i] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        yv[k] *= 1.0 / bcast_lane(av[k], k);
+#pragma unroll
+        for (int i = k + 1; i < 16; ++i) yv[i] -= bcast_lane(av[k], i) * yv[k];
+    }
+    if (lane < sw) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+            if (c <= lane) D[(s0 + c) * LS + s0 + lane] = av[c];
+    }
+    if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Li[i + 16 * lane] = yv[i];
+    }
+    return bad;
 }
 
 template <typename FT>
@@ -670,30 +743,26 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
             __syncthreads();
             for (int s0 = 0; s0 < w; s0 += 16) {
                 const int sw = min(16, w - s0);
+                double *Li = S.u.f.T;   // inverse of the sub-block's factor, Li(i, c) at i + 16c
                 if (wave == 0) {
-                    for (int jj = 0; jj < sw; ++jj) {
-                        double v = 0.0;
-                        if (lane >= jj && lane < sw) {
-                            v = D[(s0 + jj) * LS + s0 + lane];
-                            for (int k = 0; k < jj; ++k) v -= D[(s0 + k) * LS + s0 + lane] * D[(s0 + k) * LS + s0 + jj];
-                        }
-                        if (lane == jj) {
-                            if (!(v > 0.0) && S.bad == 0) S.bad = k0 + s0 + jj + 1;
-                            D[(s0 + jj) * LS + s0 + jj] = sqrt(v);
-                        }
-                        wave_sync();
-                        if (lane > jj && lane < sw) D[(s0 + jj) * LS + s0 + lane] = v / D[(s0 + jj) * LS + s0 + jj];
-                        wave_sync();
-                    }
+                    const int bad = factor16(D, s0, sw, Li, lane);
+                    if (lane == 0 && bad && S.bad == 0) S.bad = k0 + s0 + bad;
                 }
                 __syncthreads();
                 if (S.bad) break;
-                // rows of the 64-block below the sub-block: x L_ss^T = row
+                // rows of the 64-block below the sub-block: x = row L_ss^-T,
+                // x_c = sum_{d <= c} row_d Li(c, d); one thread per row
                 for (int i = s0 + sw + tid; i < w; i += NTB) {
-                    for (int c = 0; c < sw; ++c) {
-                        double v = D[(s0 + c) * LS + i];
-                        for (int d = 0; d < c; ++d) v -= D[(s0 + d) * LS + i] * D[(s0 + d) * LS + s0 + c];
-                        D[(s0 + c) * LS + i] = v / D[(s0 + c) * LS + s0 + c];
+                    double rv[16];
+#pragma unroll
+                    for (int d = 0; d < 16; ++d) rv[d] = d < sw ? D[(s0 + d) * LS + i] : 0.0;
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        if (c >= sw) break;
+                        double x = 0.0;
+#pragma unroll
+                        for (int d = 0; d <= c; ++d) x += rv[d] * Li[c + 16 * d];
+                        D[(s0 + c) * LS + i] = x;
                     }
                 }
                 __syncthreads();
