@@ -197,7 +197,8 @@ int oocdag_solve_resid_tiles(const double *dA, int N, int NB, const double *dX, 
  * normwise backward error max_k ||b_k - A x_k|| / (||A|| ||x_k|| + ||b_k||)
  * (infinity norms) is <= tol; a low-precision factor that is not positive
  * definite, or refinement that stalls (error not halving) or exceeds
- * max_iter, falls back to the FP64 factorization.  dA must hold the full
+ * max_iter, falls back one level (BF16 -> FP32 -> FP64; a factor that meets
+ * a bad pivot skips its remaining low-precision updates).  dA must hold the full
  * diagonal tiles (as oocdag_gram_tiles / oocdag_pdmatgen_tiles write them).
  * NB % 128 == 0.  dB, dX: device, column-major N x nrhs.
  * ------------------------------------------------------------------------- */

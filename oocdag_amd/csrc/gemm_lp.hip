@@ -10,8 +10,10 @@
 //   FP32 : column-major, s = 1, v_mfma_f32_16x16x4_f32
 //   BF16 : row-major (each row's K values contiguous, so one 16-byte LDS read
 //          feeds a lane of v_mfma_f32_16x16x32_bf16), one power-of-two scale
-//          per tile row of the super-column so every row block is O(1)
-//          before rounding.
+//          per tile of the super-column so every tile is O(1) before
+//          rounding; the kernel rescales its FP32 accumulators by the
+//          (exact, power-of-two) ratio of the scale products at each tile
+//          boundary along K.
 // Accumulation is FP32 in both cases; the diagonal POTRF, the TRSM, the
 // residual and the solution update remain FP64 (factor.hip, solve.hip).
 //
@@ -23,6 +25,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cstdint>
 #include <cmath>
 #include <mutex>
@@ -128,7 +131,7 @@ __global__ void k_zero_u32(unsigned *p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-// grid: (16 chunks, sw, nrt); max |x| of each tile into amax[tile row]
+// grid: (16 chunks, sw, nrt); max |x| of tile (i, t) into amax[i * sw + t]
 // (non-negative floats order like their bit patterns)
 __global__ void k_absmax_super(const double *W, int nb, int ym, int j0, int i0, unsigned *amax) {
     const int t = blockIdx.y, i = blockIdx.z;
@@ -145,12 +148,12 @@ __global__ void k_absmax_super(const double *W, int nb, int ym, int j0, int i0, 
         if (threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicMax(&amax[i], __float_as_uint((float)red[0]));
+    if (threadIdx.x == 0) atomicMax(&amax[(long long)i * gridDim.y + t], __float_as_uint((float)red[0]));
 }
 
-__global__ void k_scale_super(const unsigned *amax, int nrt, float *scale) {
+__global__ void k_scale_super(const unsigned *amax, int n, float *scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrt) return;
+    if (i >= n) return;
     const float m = __uint_as_float(amax[i]);
     int e = 0;
     if (m > 0.f) frexpf(m, &e);
@@ -189,9 +192,10 @@ __global__ void k_convert_region_f32(const double *W, int nb, int ym, int gr0, i
 }
 
 // BF16: one 64 x 64 patch per block, transposed through LDS so both the
-// column-major FP64 reads and the row-major BF16 writes are contiguous
+// column-major FP64 reads and the row-major BF16 writes are contiguous;
+// tile (ti, tk) of the super-column is divided by scale[ti * sw + tk]
 __global__ __launch_bounds__(256) void k_convert_super_bf16(const double *W, int nb, int ym, int j0,
-                                                            int i0, const float *scale,
+                                                            int i0, int sw, const float *scale,
                                                             unsigned short *X, long long ldx) {
     __shared__ float T[64][65];
     const int pr = blockIdx.x, pk = blockIdx.y;   // patch row / column (64 each)
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(256) void k_convert_super_bf16(const double *W, int
     const int ti = r0 / nb, tk = k0 / nb;
     const double *tile = W + tix(i0 + ti, j0 + tk, ym, 1) * (long long)nb * nb;
     const int rr = r0 - ti * nb, kk = k0 - tk * nb;
-    const double inv = 1.0 / (double)scale[ti];
+    const double inv = 1.0 / (double)scale[(long long)ti * sw + tk];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -224,17 +228,30 @@ struct LpArgs {
     int r0;             // global row/col of the region's top-left (diagonal)
     const void *X;      // low-precision operand, row 0 = global row r0
     long long ldx;      // FP32: column stride; BF16: row stride (elements)
-    const float *scale; // BF16: scale of the row block holding local row r: scale[r / nb]
+    const float *scale; // BF16: scale of tile (local row r, K tile t): scale[(r / nb) * kts + t]
+    int kts;            // BF16: K tiles (K / nb), the scale row length
     int M, N, K;        // region rows, cols, depth
     int mb, nbk;        // block grid
     long long nblk;
     int rowmap;         // 0: row = 4*(lane/16) + r, 1: row = lane/16 + 4*r
+    const int *stop;    // INT_MAX while the factorization is good; else every
+                        // workgroup returns at once (the result is discarded)
 };
+
+// Workgroup-uniform early exit (the flag is read once and broadcast through
+// LDS, so no wave leaves while others wait at a barrier).
+__device__ __forceinline__ bool stopped_wg(const int *stop) {
+    __shared__ int s_stop;
+    if (threadIdx.x == 0) s_stop = stop ? *stop : INT_MAX;
+    __syncthreads();
+    return s_stop != INT_MAX;
+}
 
 constexpr int BK32 = 32;            // floats per K step
 constexpr int S32 = BM + 4;         // LDS row stride (floats)
 
 __global__ __launch_bounds__(NT, 2) void k_update_f32(LpArgs a) {
+    if (stopped_wg(a.stop)) return;
     __shared__ float As[2][BK32 * S32];
     __shared__ float Bs[2][BK32 * S32];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -332,6 +349,8 @@ __global__ __launch_bounds__(NT, 2) void k_update_f32(LpArgs a) {
 template <int RING, int AHEAD, int ROWMAP>
 __global__ __launch_bounds__(NT, 2) void k_update_f32_dtv(LpArgs a) {
     static_assert((RING & (RING - 1)) == 0 && AHEAD < RING, "ring");
+    // no barriers here: a wave-uniform read of the flag is enough
+    if (a.stop && __builtin_amdgcn_readfirstlane(*a.stop) != INT_MAX) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     long long L = xcd_remap(blockIdx.x, a.nblk);
@@ -422,6 +441,7 @@ constexpr int S16 = BK16 + 8;       // LDS row stride (bf16), rows are m
 // 4*(lane/16) + r, four consecutive rows of one column; 1: rows lane/16 + 4r).
 template <int ROWMAP>
 __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
+    if (stopped_wg(a.stop)) return;
     __shared__ unsigned short As[2][BM * S16];
     __shared__ unsigned short Bs[2][BM * S16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -432,6 +452,10 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
     const int m0 = mbi * BM, n0 = nbi * BM;
     const unsigned short *X = (const unsigned short *)a.X;
     const long long per = (long long)a.nb * a.nb;
+    // scales of the block's A rows and B rows (one tile row each: nb % BM == 0)
+    const float *sa = a.scale + (long long)(m0 / a.nb) * a.kts;
+    const float *sb = a.scale + (long long)(n0 / a.nb) * a.kts;
+    const int spt = a.nb / BK16;   // K steps per tile
     // row-major X: element (r, k) at r * ldx + k.
     // q = tid + 256 s -> row q / 8 (32 rows per pass), 8-wide k chunk q % 8
     const int kc = tid & 7, rr0 = tid >> 3;
@@ -467,6 +491,18 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) gload((kt + 1) * BK16);
+        if (kt && kt % spt == 0) {
+            // entering K tile t: acc holds sum_u (s_u / s_{t-1}) Y_u; the
+            // ratio of power-of-two scale products is exact
+            const int t = kt / spt;
+            const float q = (sa[t - 1] / sa[t]) * (sb[t - 1] / sb[t]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[i][j][e] *= q;
+        }
 #pragma unroll
         for (int ks = 0; ks < BK16 / 32; ++ks) {
             const int kof = ks * 32 + 8 * (lane >> 4);
@@ -487,7 +523,7 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
         if (more) lstore(cur ^ 1);
         __syncthreads();
     }
-    // Epilogue, per 16-row block i: all 16 FP64 loads of the lane first (as
+    // Epilogue (C -= s_last * acc), per 16-row block i: all 16 FP64 loads of the lane first (as
     // 16-byte pairs for ROWMAP 0), then the updates and stores, so the
     // loads' latencies overlap instead of alternating with stores the
     // compiler cannot reorder them across.  Elements above the diagonal
@@ -495,7 +531,6 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int mb16 = m0 + wm * 64 + i * 16;
-        const float si = a.scale[mb16 / a.nb];
         const int gr0 = a.r0 + mb16;
         double *cp[4];
         double old[4][4];
@@ -520,7 +555,7 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int n = n0 + wn * 64 + j * 16 + (lane & 15);
-            const double s = (double)si * (double)a.scale[n / a.nb];
+            const double s = (double)sa[a.kts - 1] * (double)sb[a.kts - 1];
             const int gc = a.r0 + n;
             double v[4];
 #pragma unroll
@@ -589,11 +624,12 @@ hipError_t convert_super(const double *W, int nb, int ym, int j0, int sw, int i0
         return hipGetLastError();
     }
     if (nb % 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_zero_u32, dim3((nrt + 255) / 256), dim3(256), 0, s, amax, nrt);
+    const int nt = nrt * sw;   // tiles of the super-column below row i0
+    hipLaunchKernelGGL(k_zero_u32, dim3((nt + 255) / 256), dim3(256), 0, s, amax, nt);
     hipLaunchKernelGGL(k_absmax_super, dim3(16, sw, nrt), dim3(256), 0, s, W, nb, ym, j0, i0, amax);
-    hipLaunchKernelGGL(k_scale_super, dim3((nrt + 255) / 256), dim3(256), 0, s, amax, nrt, scale);
+    hipLaunchKernelGGL(k_scale_super, dim3((nt + 255) / 256), dim3(256), 0, s, amax, nt, scale);
     hipLaunchKernelGGL(k_convert_super_bf16, dim3(nrt * nb / 64, sw * nb / 64), dim3(256), 0, s, W,
-                       nb, ym, j0, i0, (const float *)scale, (unsigned short *)X,
+                       nb, ym, j0, i0, sw, (const float *)scale, (unsigned short *)X,
                        (long long)sw * nb);
     return hipGetLastError();
 }
@@ -610,9 +646,10 @@ hipError_t convert_region_f32(const double *W, int nb, int ym, int gr0, int gc0,
 
 hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const void *X,
                             long long ldx, int K, const float *scale, int M, int N, int rowmap,
-                            hipStream_t s) {
+                            hipStream_t s, const int *stop) {
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
     if (nb % BM || M % BM || N % BM || r0 % BM || M < N) return hipErrorInvalidValue;
+    if (mode != 0 && (K % nb || !scale)) return hipErrorInvalidValue;
     if (K % (mode == 0 ? BK32 : BK16)) return hipErrorInvalidValue;
     if (mode == 0 ? (ldx % 4) != 0 : (ldx % 8) != 0) return hipErrorInvalidValue;
     LpArgs a;
@@ -623,6 +660,8 @@ hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const
     a.X = X;
     a.ldx = ldx;
     a.scale = scale;
+    a.kts = K / nb;
+    a.stop = stop;
     a.M = M;
     a.N = N;
     a.K = K;

@@ -11,8 +11,10 @@
 //   x = W^-1 b;  repeat  r = b - A x,  eta = max_k ||r_k|| / (||A|| ||x_k|| + ||b_k||),
 //                        stop if eta <= tol,  x += W^-1 r
 // A low-precision factor that is not positive definite, or refinement that
-// stalls (eta not halving) or runs out of iterations, falls back to the
-// plain FP64 factorization.
+// stalls (eta not halving) or runs out of iterations, falls back one level:
+// BF16 -> FP32 -> FP64.  A factor that meets a bad pivot stops its queued
+// low-precision updates at once, so a failed attempt costs little more than
+// the columns up to that pivot.
 #include "ooc_internal.h"
 #include "../../include/oocdag.h"
 
@@ -29,7 +31,7 @@ hipError_t convert_super(const double *W, int nb, int ym, int j0, int sw, int i0
                          unsigned *amax, float *scale, void *X, hipStream_t s);
 hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const void *X,
                             long long ldx, int K, const float *scale, int M, int N, int rowmap,
-                            hipStream_t s);
+                            hipStream_t s, const int *stop);
 hipError_t convert_region_f32(const double *W, int nb, int ym, int gr0, int gc0, int R, int K,
                               float *X, hipStream_t s);
 
@@ -539,7 +541,10 @@ int mp_super_width(int NB, int bb) { return std::max(1, std::min(bb, 2048 / NB))
 //   narrow: the next super-column        (low precision, K = sw*NB, stream s)
 //   bulk  : everything to its right      (low precision, stream s1)
 // so the next super-column's FP64 work overlaps the bulk update.  lpbuf
-// holds two converted super-columns (by parity), scale 2*bb floats, amax bb.
+// holds two converted super-columns (by parity), scale and amax 2*bb*sw and
+// bb*sw entries (BF16: one per tile).  Once a diagonal block reports a bad
+// pivot (dinfo < INT_MAX) the low-precision updates still queued return at
+// once: the factor is discarded.
 int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int *dinfo,
                  void *lpbuf, float *scale, unsigned *amax, float *colbuf, hipStream_t s,
                  hipStream_t s1, hipEvent_t w_ready) {
@@ -567,7 +572,7 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
     const size_t esz = mode == 0 ? 4 : 2;
     const size_t lp_bytes = (size_t)bb * NB * sw * NB * esz;
     void *lp[2] = {lpbuf, (char *)lpbuf + lp_bytes};
-    float *sc[2] = {scale, scale + bb};
+    float *sc[2] = {scale, scale + (size_t)bb * sw};   // BF16: per tile of a super-column
 
     hipEvent_t ev_fact, ev_bulk[2];
     if (hipEventCreateWithFlags(&ev_fact, hipEventDisableTiming) != hipSuccess) return -4;
@@ -598,7 +603,8 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
         if (e != hipSuccess) return e;
         const int r0 = c0 + h, M = bb * NB - r0;
         if ((e = convert_region_f32(W, NB, bb, r0, c0, M, h, colbuf, s)) != hipSuccess ||
-            (e = launch_update_lp(0, W, bb, NB, r0, colbuf, M, h, nullptr, M, h, rowmap32, s)) !=
+            (e = launch_update_lp(0, W, bb, NB, r0, colbuf, M, h, nullptr, M, h, rowmap32, s,
+                                  dinfo)) !=
                 hipSuccess)
             return e;
         return factor_cols(r0, h, dinv_c + (size_t)(h / 64) * 64 * 64);
@@ -620,7 +626,7 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
                 const int nrt = bb - t - 1;
                 MP_TRY(convert_super(W, NB, bb, t, 1, t + 1, nrt, 0, nullptr, nullptr, colbuf, s));
                 MP_TRY(launch_update_lp(0, W, bb, NB, (t + 1) * NB, colbuf, (long long)nrt * NB, NB,
-                                        nullptr, nrt * NB, (j1 - t - 1) * NB, rowmap32, s));
+                                        nullptr, nrt * NB, (j1 - t - 1) * NB, rowmap32, s, dinfo));
             }
         }
         if (j1 >= bb) break;
@@ -635,7 +641,7 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
         if (J == 0 && w_ready) MP_TRY(hipStreamWaitEvent(s, w_ready, 0));
         const int k1 = std::min(j1 + sw, bb);
         MP_TRY(launch_update_lp(mode, W, bb, NB, j1 * NB, lp[J & 1], ldx, K, sc[J & 1],
-                                (bb - j1) * NB, (k1 - j1) * NB, rowmap, s));
+                                (bb - j1) * NB, (k1 - j1) * NB, rowmap, s, dinfo));
         if (k1 < bb) {
             const long long roff = (long long)(k1 - j1) * NB;   // X row of global row k1*NB
             const void *Xb = mode == 0 ? (const void *)((const float *)lp[J & 1] + roff)
@@ -646,8 +652,10 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
                 MP_TRY(hipStreamWaitEvent(s1, ev_fact, 0));
             }
             hipStream_t sb = overlap ? s1 : s;
-            MP_TRY(launch_update_lp(mode, W, bb, NB, k1 * NB, Xb, ldx, K, sc[J & 1] + (k1 - j1),
-                                    (bb - k1) * NB, (bb - k1) * NB, rowmap, sb));
+            // BF16 scales: one row of (j1 - j0) per tile row, from row j1
+            MP_TRY(launch_update_lp(mode, W, bb, NB, k1 * NB, Xb, ldx, K,
+                                    sc[J & 1] + (size_t)(k1 - j1) * (j1 - j0), (bb - k1) * NB,
+                                    (bb - k1) * NB, rowmap, sb, dinfo));
             MP_TRY(hipEventRecord(ev_bulk[J & 1], sb));
             any_bulk = overlap;
         }
@@ -725,9 +733,9 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
         return OOCDAG_ENOMEM;
     const size_t sw = (size_t)mp_super_width(NB, bb);
     const size_t lp_bytes = precision > 0 ? 2 * (size_t)bb * NB * sw * NB * (precision == 1 ? 4 : 2) : 8;
-    if (!ok(salloc(&lp, lp_bytes, s0)) || !ok(salloc(&scale, (size_t)2 * bb * 4, s0)) ||
+    if (!ok(salloc(&lp, lp_bytes, s0)) || !ok(salloc(&scale, (size_t)2 * bb * sw * 4, s0)) ||
         (precision > 0 && !ok(salloc(&colbuf, (size_t)bb * NB * NB * 4, s0))) ||
-        !ok(salloc(&amax, (size_t)bb * 4, s0)) || !ok(salloc(&info, sizeof(int), s0)) ||
+        !ok(salloc(&amax, (size_t)bb * sw * 4, s0)) || !ok(salloc(&info, sizeof(int), s0)) ||
         !ok(salloc(&anorm, (size_t)Np * 8, s0)))
         return OOCDAG_ENOMEM;
     hipEvent_t w_ready = nullptr;
@@ -923,7 +931,7 @@ int mp_factor_solve(const double *dA, int N, int NB, const double *dB, int nrhs,
     const double t0 = now();
     MpFactor f;
     int prec = precision, rc = OOCDAG_OK;
-    for (int attempt = 0; attempt < 2; ++attempt) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
         rc = mp_factor(f, dA, N, NB, prec, s, s1);
         if (rc) break;
         st->factor_seconds += f.factor_seconds;
@@ -949,7 +957,7 @@ int mp_factor_solve(const double *dA, int N, int NB, const double *dB, int nrhs,
             break;
         }
         st->fell_back = 1;
-        prec = 0;
+        prec = prec == 2 ? 1 : 0;   // BF16 -> FP32 -> FP64
     }
     if (keep && rc == OOCDAG_OK && !st->info)
         *keep = f;

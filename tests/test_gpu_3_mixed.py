@@ -211,16 +211,65 @@ def test_mpir_update_variants_agree(gpu, prec):
         assert np.array_equal(x, out[0][1])
 
 
-def test_mpir_non_spd_reports_info(gpu):
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_mpir_non_spd_reports_info(gpu, prec):
+    """An indefinite matrix: every attempt of the cascade (BF16 -> FP32 ->
+    FP64) meets a bad pivot, the FP64 attempt reports LAPACK's info (the
+    first column whose pivot is not positive, as numpy's Cholesky fails)."""
     ooc = gpu
-    n, nb = 512, 128
+    n, nb = 3072, 128   # BF16: two super-columns, narrow and bulk updates
     buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
     ooc.gram_tiles(buf, n, nb, -10.0, seed=1)   # shifted far below zero: indefinite
     dB = ooc.DeviceBuffer(n * 8)
     ooc.normal_fill(dB, n, 1, n)
     dX = ooc.DeviceBuffer(n * 8)
-    st = ooc.mpir_solve_tiles(buf, n, nb, dB, 1, dX, "fp32")
-    assert st["info"] > 0 and st["fell_back"] == 1
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, 1, dX, prec)
+    assert st["info"] == 1 and st["fell_back"] == 1 and st["precision_used"] == 0
+
+
+def test_mpir_bf16_cascades_through_fp32(gpu):
+    """kappa ~ 4e5 (mu = 1e-5): kappa * 2^-8 >> 1, so the BF16 attempt cannot
+    refine; the next attempt is FP32 (kappa * 2^-24 < 1), not FP64."""
+    ooc = gpu
+    n, nb, nrhs = 4096, 256, 2
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, 1e-5, seed=4)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=4, stream_id=2)
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, "bf16", tol=8 * EPS, max_iter=30)
+    assert st["fell_back"] == 1 and st["precision_used"] == 1, st
+    assert st["backward_error"] <= 8 * EPS and st["info"] == 0, st
+    A = gram_host(ooc, buf, n, nb)
+    X, B = dX.to_array((n, nrhs)), dB.to_array((n, nrhs))
+    Xref = np.linalg.solve(A, B)
+    assert np.abs(X - Xref).max() / np.abs(Xref).max() <= 1e2 * np.linalg.cond(A, 1) * EPS
+
+
+def test_mpir_bf16_per_tile_scales(gpu):
+    """A = L' L'^T with L' = L C, C = diag(2^(t mod 3)) by tile column t (L the
+    Cholesky factor of a Gram matrix): the tiles of one tile row of a
+    super-column get different power-of-two scales, so the BF16 update's
+    accumulator rescaling at each tile boundary along K is exercised (a wrong
+    ratio or scale index breaks the factor and the refinement falls back).
+    n = 5120, NB = 512: narrow and bulk updates, the bulk's scale rows offset."""
+    ooc = gpu
+    n, nb, nrhs = 5120, 512, 3
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, 0.1, seed=5)
+    L = np.linalg.cholesky(gram_host(ooc, buf, n, nb))
+    L *= 2.0 ** (np.arange(n) // nb % 3)[None, :]
+    A = L @ L.T
+    A = np.asfortranarray((A + A.T) / 2)
+    ooc.tiles_from_host(buf, A, nb)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=5, stream_id=2)
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, "bf16", tol=8 * EPS, max_iter=30)
+    assert st["fell_back"] == 0 and st["precision_used"] == 2 and st["iterations"] <= 16, st
+    X, B = dX.to_array((n, nrhs)), dB.to_array((n, nrhs))
+    Xref = np.linalg.solve(A, B)
+    assert np.abs(X - Xref).max() / np.abs(Xref).max() <= 1e2 * np.linalg.cond(A, 1) * EPS
 
 
 @pytest.mark.parametrize("prec", ["fp64", "bf16"])
