@@ -71,11 +71,13 @@ double oocdag_cholesky_nx(double *A, int N, int NB, int LDA, size_t memsize, int
  * Factorization, host packed tiles in / out (out of core).
  * Replaces: the same Cholesky() panel loop (oocdag.cpp:18-112; staging
  *   A2Y/A2X/D2H oocquark.cpp:16-63,253-319) for a caller that keeps A on the
- *   host as packed lower tiles -- tile (i,j), i >= j, NB x NB column-major at
- *   element offset (i + j*bb - j(j+1)/2)*NB*NB, bb = ceil(N/NB), padded past N
- *   with zeros and a unit diagonal (the device layout; oocdag_tiles_elems
- *   gives the size).  Every tile-column copy is then one contiguous DMA and
- *   nothing is packed on the host: T in pinned memory (oocdag_host_alloc) is
+ *   host as packed lower tiles -- tile (i,j), i >= j, in a column-major slot
+ *   of edge TS = oocdag_tile_edge(NB) at element offset
+ *   (i + j*bb - j(j+1)/2)*TS*TS, bb = ceil(N/NB), the NB x NB tile in its
+ *   top-left corner, padded past N and past NB with zeros and a unit
+ *   diagonal (the device layout; oocdag_tiles_elems gives the size).  Every
+ *   tile-column copy is then one contiguous DMA and nothing is packed on the
+ *   host: T in pinned memory (oocdag_host_alloc) is
  *   copied directly; pageable T is staged through the library's pinned
  *   double-buffered slots with a parallel memcpy.  Same panel schedule,
  *   budget, return value and *info as oocdag_cholesky; the tiles are
