@@ -586,12 +586,17 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
 // pivot and column j read from the owning lanes (v_readlane, no LDS round
 // trips): the same subtraction order per element as a left-looking dot
 // product, the column scaled by the reciprocal pivot as LAPACK dpotf2 does.
-// Then lane c forms column c of the factor's inverse by forward
-// substitution into Li (Li(i, c) at i + 16c).  Returns the first bad pivot
-// (1-based inside the sub-block) or 0.  Kept out of line: its 48 live
-// doubles must not add to the register pressure of the caller's other phases.
-__device__ __noinline__ int factor16(double *D, int s0, int sw, double *Li, int lane) {
-    double av[16];
+// The reciprocal pivots are kept, so the inverse needs no divisions: lane c
+// forms column c of the factor's inverse by forward substitution into Li
+// (Li(i, c) at i + 16c) and into the diagonal block of the 64 x 64 inverse
+// Di (column stride LS).  (Reading the factor back from LDS instead of
+// v_readlane made the compiler keep the loads live: 256 VGPRs and a stack
+// of ~700 bytes per lane.)  Returns the
+// first bad pivot (1-based inside the sub-block) or 0.  Kept out of line:
+// its 48 live doubles must not add to the register pressure of the
+// caller's other phases.
+__device__ __noinline__ int factor16(double *D, int s0, int sw, double *Li, double *Di, int lane) {
+    double av[16], rdv[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c)
         av[c] = (lane < sw && c < sw && c <= lane) ? D[(s0 + c) * LS + s0 + lane]
@@ -602,6 +607,7 @@ __device__ __noinline__ int factor16(double *D, int s0, int sw, double *Li, int 
         const double d = bcast_lane(av[j], j);
         if (bad == 0 && !(d > 0.0)) bad = j + 1;
         const double sd = sqrt(d), rd = 1.0 / sd;
+        rdv[j] = rd;
         av[j] = lane > j ? av[j] * rd : (lane == j ? sd : 0.0);
 #pragma unroll
         for (int k = j + 1; k < 16; ++k) av[k] -= av[j] * bcast_lane(av[j], k);
@@ -611,7 +617,7 @@ __device__ __noinline__ int factor16(double *D, int s0, int sw, double *Li, int 
     for (int i = 0; i < 16; ++i) yv[i] = (i == lane) ? 1.0 : 0.0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        yv[k] *= 1.0 / bcast_lane(av[k], k);
+        yv[k] *= rdv[k];
 #pragma unroll
         for (int i = k + 1; i < 16; ++i) yv[i] -= bcast_lane(av[k], i) * yv[k];
     }
@@ -623,6 +629,11 @@ __device__ __noinline__ int factor16(double *D, int s0, int sw, double *Li, int 
     if (lane < 16) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) Li[i + 16 * lane] = yv[i];
+    }
+    if (lane < sw) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < sw) Di[(s0 + lane) * LS + s0 + i] = yv[i];
     }
     return bad;
 }
@@ -716,7 +727,23 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                         }
                     }
                 }
-                // C = A - acc (lower part only in the diagonal block)
+                // C = A - acc (lower part only in the diagonal block).  All
+                // loads of A first: a store to W may alias A for the
+                // compiler, so interleaved they would be 32 round trips.
+                double aval[4][2][4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        const int c = wn * 32 + j * 16 + il;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int rr = wm * 64 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
+                            const int gr = r0 + rr;
+                            const bool ok = gr < n && c < w && gr >= k0 + c;
+                            aval[i][j][r] = ok ? A[gr + (long long)(k0 + c) * ld] : 0.0;
+                        }
+                    }
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -727,8 +754,7 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                             const int rr = wm * 64 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
                             const int gr = r0 + rr;
                             if (gr >= n || c >= w || gr < k0 + c) continue;
-                            const long long o = gr + (long long)(k0 + c) * ld;
-                            W[o] = (FT)(A[o] - (double)acc[i][j][r]);
+                            W[gr + (long long)(k0 + c) * ld] = (FT)(aval[i][j][r] - (double)acc[i][j][r]);
                         }
                     }
                 __syncthreads();
@@ -739,30 +765,46 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 const int r = idx & (LT - 1), c = idx >> 6;
                 D[c * LS + r] = (r < w && c < w && r >= c) ? (double)W[(k0 + r) + (long long)(k0 + c) * ld]
                                                            : (r == c ? 1.0 : 0.0);
+                Di[c * LS + r] = (r == c) ? 1.0 : 0.0;
             }
             __syncthreads();
             for (int s0 = 0; s0 < w; s0 += 16) {
                 const int sw = min(16, w - s0);
                 double *Li = S.u.f.T;   // inverse of the sub-block's factor, Li(i, c) at i + 16c
                 if (wave == 0) {
-                    const int bad = factor16(D, s0, sw, Li, lane);
+                    // also the diagonal block of Di
+                    const int bad = factor16(D, s0, sw, Li, Di, lane);
                     if (lane == 0 && bad && S.bad == 0) S.bad = k0 + s0 + bad;
                 }
                 __syncthreads();
                 if (S.bad) break;
                 // rows of the 64-block below the sub-block: x = row L_ss^-T,
-                // x_c = sum_{d <= c} row_d Li(c, d); one thread per row
-                for (int i = s0 + sw + tid; i < w; i += NTB) {
-                    double rv[16];
+                // x_c = sum_{d <= c} row_d Li(c, d); four threads per row,
+                // four columns each, written back after every row is read
+                {
+                    const int nrow = w - s0 - sw;   // <= 48: at most 192 threads
+                    const int i = s0 + sw + (tid >> 2), g = tid & 3;
+                    const bool act = tid < 4 * nrow;
+                    double xo[4];
+                    if (act) {
+                        double rv[16];
 #pragma unroll
-                    for (int d = 0; d < 16; ++d) rv[d] = d < sw ? D[(s0 + d) * LS + i] : 0.0;
+                        for (int d = 0; d < 16; ++d) rv[d] = (d < sw && d < 4 * g + 4) ? D[(s0 + d) * LS + i] : 0.0;
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) {
-                        if (c >= sw) break;
-                        double x = 0.0;
+                        for (int cc = 0; cc < 4; ++cc) {
+                            const int c = 4 * g + cc;
+                            double x = 0.0;
 #pragma unroll
-                        for (int d = 0; d <= c; ++d) x += rv[d] * Li[c + 16 * d];
-                        D[(s0 + c) * LS + i] = x;
+                            for (int d = 0; d < 16; ++d)
+                                if (d <= c) x += rv[d] * Li[c + 16 * d];
+                            xo[cc] = x;
+                        }
+                    }
+                    __syncthreads();
+                    if (act) {
+#pragma unroll
+                        for (int cc = 0; cc < 4; ++cc)
+                            if (4 * g + cc < sw) D[(s0 + 4 * g + cc) * LS + i] = xo[cc];
                     }
                 }
                 __syncthreads();
@@ -779,28 +821,10 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 __syncthreads();
             }
             if (S.bad) break;
-            // inverse, blocked by 16: diagonal blocks first (wave b inverts
-            // block b, lane = column), then block rows i = 1..3 from
-            // X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj (two barriers per row)
+            // inverse, blocked by 16: the diagonal blocks X_ii are the
+            // sub-block inverses factor16 left in Di; block rows i = 1..3
+            // from X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj (two barriers per row)
             {
-                for (int idx = tid; idx < LT * LT; idx += NTB) {
-                    const int r = idx & (LT - 1), c = idx >> 6;
-                    Di[c * LS + r] = (r == c) ? 1.0 : 0.0;
-                }
-                __syncthreads();
-                if (lane < 16 && 16 * wave < w) {
-                    const int b0 = 16 * wave, c = b0 + lane;
-                    const int bw = min(16, w - b0);
-                    if (lane < bw) {
-                        for (int k = lane; k < bw; ++k) {
-                            const double yk = Di[c * LS + b0 + k] / D[(b0 + k) * LS + b0 + k];
-                            Di[c * LS + b0 + k] = yk;
-                            for (int i = k + 1; i < bw; ++i)
-                                Di[c * LS + b0 + i] -= D[(b0 + k) * LS + b0 + i] * yk;
-                        }
-                    }
-                }
-                __syncthreads();
                 double *Tb = S.u.f.T;
                 for (int bi = 1; bi * 16 < w; ++bi) {
                     // T_bj = sum_{k=j}^{bi-1} L_(bi,k) X_(k,j), j < bi: element per thread
@@ -840,18 +864,24 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 int rows[2];
 #pragma unroll
                 for (int i = 0; i < 2; ++i) rows[i] = min(r0 + wm * 32 + i * 16 + il, n - 1);
-#pragma unroll 4
+                // all 32 loads of the tile's rows first (one round trip),
+                // then the MFMAs with Di from LDS
+                FT fa[LT / 4][2];
+#pragma unroll
+                for (int ks = 0; ks < LT / 4; ++ks)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) fa[ks][i] = W[rows[i] + (long long)(k0 + ks * 4 + kl) * ld];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
                 for (int ks = 0; ks < LT / 4; ++ks) {
                     const int kk = ks * 4 + kl;
-                    FT fa[2], fb[2];
-#pragma unroll
-                    for (int i = 0; i < 2; ++i) fa[i] = W[rows[i] + (long long)(k0 + kk) * ld];
+                    FT fb[2];
 #pragma unroll
                     for (int j = 0; j < 2; ++j) fb[j] = (FT)Di[kk * LS + wn * 32 + j * 16 + il];
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
-                        for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[i], fb[j], acc[i][j]);
+                        for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[ks][i], fb[j], acc[i][j]);
                 }
                 __syncthreads();   // every wave has read its rows before any is overwritten
 #pragma unroll
@@ -887,9 +917,13 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
             const double *B = B0 + (long long)c0 * ld;
             double *X = X0 + (long long)c0 * ld;
             double *y = S.u.v.y;
+            // two loops: a store to X may alias B for the compiler
             for (int idx = tid; idx < n * nrhs; idx += NTB) {
                 const int i = idx % n, k = idx / n;
                 y[k * NMAX + i] = B[i + (long long)k * ld];
+            }
+            for (int idx = tid; idx < n * nrhs; idx += NTB) {
+                const int i = idx % n, k = idx / n;
                 X[i + (long long)k * ld] = 0.0;
             }
             __syncthreads();
@@ -897,9 +931,21 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
             double eta = INFINITY, prev = INFINITY, lres = INFINITY;
             for (;;) {
                 tri_solve_ll<FT>(S, W, Dinv, n, ld, nrhs);
-                for (int idx = tid; idx < n * nrhs; idx += NTB) {
-                    const int i = idx % n, k = idx / n;
-                    X[i + (long long)k * ld] += y[k * NMAX + i];
+                // x += d, eight loads of x before their stores
+                for (int idx0 = tid; idx0 < n * nrhs; idx0 += 8 * NTB) {
+                    double xv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int idx = idx0 + u * NTB;
+                        const int i = idx % n, k = idx / n;
+                        xv[u] = idx < n * nrhs ? X[i + (long long)k * ld] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int idx = idx0 + u * NTB;
+                        const int i = idx % n, k = idx / n;
+                        if (idx < n * nrhs) X[i + (long long)k * ld] = xv[u] + y[k * NMAX + i];
+                    }
                 }
                 __syncthreads();
                 double rmax[LRMAX], xmax[LRMAX], bmax[LRMAX];
