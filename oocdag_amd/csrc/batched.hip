@@ -504,12 +504,12 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
 #pragma unroll
             for (int k = 0; k < LRMAX; ++k) sk[k] = 0.0;
             int c = 0;
-            for (; c + 8 <= w; c += 8) {   // 8 independent loads, then the products
-                double wv[8];
+            for (; c + 16 <= w; c += 16) {   // 16 independent loads, then the products
+                double wv[16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) wv[u] = (double)W[i + (long long)(k0 + c + u) * ld];
+                for (int u = 0; u < 16; ++u) wv[u] = (double)W[i + (long long)(k0 + c + u) * ld];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
+                for (int u = 0; u < 16; ++u)
 #pragma unroll
                     for (int k = 0; k < LRMAX; ++k)
                         if (k < nrhs) sk[k] += wv[u] * S.tmp[k * LT + c + u];
@@ -556,12 +556,12 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
 #pragma unroll
             for (int k = 0; k < LRMAX; ++k) sk[k] = 0.0;
             int r = 0;
-            for (; r + 8 <= w; r += 8) {
-                double wv[8];
+            for (; r + 16 <= w; r += 16) {
+                double wv[16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) wv[u] = (double)col[r + u];
+                for (int u = 0; u < 16; ++u) wv[u] = (double)col[r + u];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
+                for (int u = 0; u < 16; ++u)
 #pragma unroll
                     for (int k = 0; k < LRMAX; ++k)
                         if (k < nrhs) sk[k] += wv[u] * S.tmp[k * LT + r + u];
@@ -948,17 +948,19 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                     }
                 }
                 __syncthreads();
-                double rmax[LRMAX], xmax[LRMAX], bmax[LRMAX];
-#pragma unroll
-                for (int k = 0; k < LRMAX; ++k) rmax[k] = xmax[k] = bmax[k] = 0.0;
+                // r = b - A x: x staged through LDS 64 entries at a time, each
+                // chunk once for two of this thread's rows (tid + g NTB, g in
+                // a pair), 2 x 8 loads before their products
                 double amax = 0.0;
-                // r = b - A x with x staged through LDS 64 rows at a time;
-                // one pass per row slot (rows tid, tid + NTB, ...)
-                for (int i0 = 0; i0 < n; i0 += NTB) {
-                    const int i = i0 + tid;
-                    double rs[LRMAX], as = 0.0;
+                for (int g0 = 0; g0 * NTB < n; g0 += 2) {
+                    const int i0 = g0 * NTB + tid, i1 = i0 + NTB;
+                    const bool v0 = i0 < n, v1 = i1 < n;
+                    double rs0[LRMAX], rs1[LRMAX], as0 = 0.0, as1 = 0.0;
 #pragma unroll
-                    for (int k = 0; k < LRMAX; ++k) rs[k] = (i < n && k < nrhs) ? B[i + (long long)k * ld] : 0.0;
+                    for (int k = 0; k < LRMAX; ++k) {
+                        rs0[k] = (v0 && k < nrhs) ? B[i0 + (long long)k * ld] : 0.0;
+                        rs1[k] = (v1 && k < nrhs) ? B[i1 + (long long)k * ld] : 0.0;
+                    }
                     for (int j0 = 0; j0 < n; j0 += LT) {
                         const int jw = min(LT, n - j0);
                         for (int idx = tid; idx < LT * LRMAX; idx += NTB) {
@@ -966,39 +968,66 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                             S.tmp[k * LT + jj] = (jj < jw && k < nrhs) ? X[j0 + jj + (long long)k * ld] : 0.0;
                         }
                         __syncthreads();
-                        if (i < n) {
-                            // 8 independent loads of the row before their products
+                        if (v0) {
+                            const double *a0 = A + i0 + (long long)j0 * ld, *a1 = A + i1 + (long long)j0 * ld;
                             int jj = 0;
                             for (; jj + 8 <= jw; jj += 8) {
-                                double av[8];
-#pragma unroll
-                                for (int u = 0; u < 8; ++u) av[u] = A[i + (long long)(j0 + jj + u) * ld];
+                                double av0[8], av1[8];
 #pragma unroll
                                 for (int u = 0; u < 8; ++u) {
-                                    as += fabs(av[u]);
+                                    av0[u] = a0[(long long)(jj + u) * ld];
+                                    av1[u] = v1 ? a1[(long long)(jj + u) * ld] : 0.0;
+                                }
 #pragma unroll
-                                    for (int k = 0; k < LRMAX; ++k) rs[k] -= av[u] * S.tmp[k * LT + jj + u];
+                                for (int u = 0; u < 8; ++u) {
+                                    as0 += fabs(av0[u]);
+                                    as1 += fabs(av1[u]);
+#pragma unroll
+                                    for (int k = 0; k < LRMAX; ++k) {
+                                        const double t = S.tmp[k * LT + jj + u];
+                                        rs0[k] -= av0[u] * t;
+                                        rs1[k] -= av1[u] * t;
+                                    }
                                 }
                             }
                             for (; jj < jw; ++jj) {
-                                const double aij = A[i + (long long)(j0 + jj) * ld];
-                                as += fabs(aij);
+                                const double b0 = a0[(long long)jj * ld];
+                                const double b1 = v1 ? a1[(long long)jj * ld] : 0.0;
+                                as0 += fabs(b0);
+                                as1 += fabs(b1);
 #pragma unroll
-                                for (int k = 0; k < LRMAX; ++k) rs[k] -= aij * S.tmp[k * LT + jj];
+                                for (int k = 0; k < LRMAX; ++k) {
+                                    const double t = S.tmp[k * LT + jj];
+                                    rs0[k] -= b0 * t;
+                                    rs1[k] -= b1 * t;
+                                }
                             }
                         }
                         __syncthreads();
                     }
-                    if (i < n) {
-                        amax = fmax(amax, as);
+                    if (v0) {
+                        amax = fmax(amax, as0);
 #pragma unroll
-                        for (int k = 0; k < LRMAX; ++k) {
-                            if (k >= nrhs) break;
-                            y[k * NMAX + i] = rs[k];
-                            rmax[k] = fmax(rmax[k], fabs(rs[k]));
-                            xmax[k] = fmax(xmax[k], fabs(X[i + (long long)k * ld]));
-                            bmax[k] = fmax(bmax[k], fabs(B[i + (long long)k * ld]));
-                        }
+                        for (int k = 0; k < LRMAX; ++k)
+                            if (k < nrhs) y[k * NMAX + i0] = rs0[k];
+                    }
+                    if (v1) {
+                        amax = fmax(amax, as1);
+#pragma unroll
+                        for (int k = 0; k < LRMAX; ++k)
+                            if (k < nrhs) y[k * NMAX + i1] = rs1[k];
+                    }
+                }
+                double rmax[LRMAX], xmax[LRMAX], bmax[LRMAX];
+#pragma unroll
+                for (int k = 0; k < LRMAX; ++k) rmax[k] = xmax[k] = bmax[k] = 0.0;
+                for (int i = tid; i < n; i += NTB) {   // this thread's own rows of y
+#pragma unroll
+                    for (int k = 0; k < LRMAX; ++k) {
+                        if (k >= nrhs) break;
+                        rmax[k] = fmax(rmax[k], fabs(y[k * NMAX + i]));
+                        xmax[k] = fmax(xmax[k], fabs(X[i + (long long)k * ld]));
+                        bmax[k] = fmax(bmax[k], fabs(B[i + (long long)k * ld]));
                     }
                 }
                 // the 1 + 3 * LRMAX maxima: a butterfly inside each wave, one
