@@ -744,6 +744,10 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                             aval[i][j][r] = ok ? A[gr + (long long)(k0 + c) * ld] : 0.0;
                         }
                     }
+                // the diagonal block (first tile, waves wm = 0) goes straight
+                // to the LDS block D of the factorization below (as the FT
+                // value W would hold); L_D reaches W once factored
+                const bool to_d = rt == 0 && wm == 0;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -754,20 +758,38 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                             const int rr = wm * 64 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
                             const int gr = r0 + rr;
                             if (gr >= n || c >= w || gr < k0 + c) continue;
-                            W[gr + (long long)(k0 + c) * ld] = (FT)(aval[i][j][r] - (double)acc[i][j][r]);
+                            const FT v = (FT)(aval[i][j][r] - (double)acc[i][j][r]);
+                            if (to_d)
+                                S.u.f.D[c * LS + rr] = (double)v;
+                            else
+                                W[gr + (long long)(k0 + c) * ld] = v;
                         }
                     }
                 __syncthreads();
             }
             // ---- D: factor and invert the diagonal block in LDS (double)
+            // D's lower part came from the update; the rest is the identity
             double *D = S.u.f.D, *Di = S.u.f.Di;
             for (int idx = tid; idx < LT * LT; idx += NTB) {
                 const int r = idx & (LT - 1), c = idx >> 6;
-                D[c * LS + r] = (r < w && c < w && r >= c) ? (double)W[(k0 + r) + (long long)(k0 + c) * ld]
-                                                           : (r == c ? 1.0 : 0.0);
+                if (!(r < w && c < w && r >= c)) D[c * LS + r] = (r == c) ? 1.0 : 0.0;
                 Di[c * LS + r] = (r == c) ? 1.0 : 0.0;
             }
             __syncthreads();
+            // trailing update of the 64-block from the panel at columns
+            // p0 .. p0+pw-1 (lower part, rows and columns from p0+pw):
+            // either only the next sub-block's 16 x 16 diagonal block, or
+            // everything else, by threads t0, t0 + nt, ...
+            auto trail = [&](int p0, int pw, bool diag_only, int t0, int nt) {
+                const int b0 = p0 + pw, mt = w - b0, md = diag_only ? min(16, mt) : mt;
+                for (int idx = t0; idx < md * md; idx += nt) {
+                    const int i = idx % md, k = idx / md;
+                    if (i < k || (!diag_only && i < 16 && k < 16)) continue;
+                    double v = D[(b0 + k) * LS + b0 + i];
+                    for (int c = 0; c < pw; ++c) v -= D[(p0 + c) * LS + b0 + i] * D[(p0 + c) * LS + b0 + k];
+                    D[(b0 + k) * LS + b0 + i] = v;
+                }
+            };
             for (int s0 = 0; s0 < w; s0 += 16) {
                 const int sw = min(16, w - s0);
                 double *Li = S.u.f.T;   // inverse of the sub-block's factor, Li(i, c) at i + 16c
@@ -775,6 +797,10 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                     // also the diagonal block of Di
                     const int bad = factor16(D, s0, sw, Li, Di, lane);
                     if (lane == 0 && bad && S.bad == 0) S.bad = k0 + s0 + bad;
+                } else if (s0 > 0) {
+                    // meanwhile the rest of the previous panel's trailing update
+                    // (disjoint from this sub-block, which it already updated)
+                    trail(s0 - 16, 16, false, tid - 64, NTB - 64);
                 }
                 __syncthreads();
                 if (S.bad) break;
@@ -808,16 +834,10 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                     }
                 }
                 __syncthreads();
-                const int mt = w - s0 - sw;
-                for (int idx = tid; idx < mt * mt; idx += NTB) {
-                    int i = idx % mt, k = idx / mt;
-                    if (i < k) continue;
-                    i += s0 + sw;
-                    k += s0 + sw;
-                    double v = D[k * LS + i];
-                    for (int c = 0; c < sw; ++c) v -= D[(s0 + c) * LS + i] * D[(s0 + c) * LS + k];
-                    D[k * LS + i] = v;
-                }
+                // the next sub-block's diagonal block now; the rest of this
+                // update beside the next factor16 (a sub-block narrower than
+                // 16 is the last: nothing is left to update after it)
+                trail(s0, sw, true, tid, NTB);
                 __syncthreads();
             }
             if (S.bad) break;
