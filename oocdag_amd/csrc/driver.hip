@@ -1355,27 +1355,78 @@ int oocdag_pdmatgen(char aform, char diag, int M, int N, int MB, int NB, double 
     g.M = M;
     g.N = N;
     g.mode = (af == 'F') ? 3 : symm ? 2 : tran ? 1 : 0;
-    // column chunks of ~128 MiB
-    int chunk = (int)std::max<long long>(1, (128ll << 20) / 8 / std::max(1, M));
-    chunk = std::min(chunk, N);
-    double *d = nullptr;
-    HIPTRY(hipMalloc(&d, (size_t)M * chunk * sizeof(double)));
-    struct F {
-        double *p;
-        ~F() { (void)hipFree(p); }
-    } fd{d};
-    for (int c0 = 0; c0 < N; c0 += chunk) {
+    // column chunks of ~1 GiB generated on the device into two alternating
+    // windows and copied down while the next chunk is generated; pinned A
+    // takes the copies directly, pageable A goes through two pinned staging
+    // buffers that the host threads empty column by column
+    const int chunk = (int)std::min<long long>(
+        N, std::max<long long>(1, (1ll << 30) / 8 / std::max(1, M)));
+    const size_t cap = (size_t)M * chunk;
+    const bool direct = host_pinned(A);
+    double *win[2] = {nullptr, nullptr}, *pin[2] = {nullptr, nullptr};
+    hipStream_t s[2] = {nullptr, nullptr};
+    struct Free {
+        double **w, **p;
+        hipStream_t *s;
+        ~Free() {
+            for (int i = 0; i < 2; ++i) {
+                if (s[i]) (void)hipStreamSynchronize(s[i]);
+                if (w[i]) (void)hipFree(w[i]);
+                pinned_put(p[i]);
+                if (s[i]) (void)hipStreamDestroy(s[i]);
+            }
+        }
+    } fr{win, pin, s};
+    for (int i = 0; i < 2; ++i) {
+        HIPTRY(hipStreamCreateWithFlags(&s[i], hipStreamNonBlocking));
+        HIPTRY(hipMalloc(&win[i], cap * sizeof(double)));
+        if (!direct && !(pin[i] = pinned_get(cap * sizeof(double)))) {
+            set_err("pinned host allocation of %zu bytes failed", cap * sizeof(double));
+            return OOCDAG_ENOMEM;
+        }
+    }
+    // columns [c0, c0 + nc) of pinned buffer b into A
+    auto drain = [&](int b, int c0, int nc) {
+        if (LDA == M) {
+            par_memcpy(A + (size_t)c0 * LDA, pin[b], (size_t)M * nc);
+            return;
+        }
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < nc; ++c)
+            memcpy(A + (size_t)(c0 + c) * LDA, pin[b] + (size_t)c * M, (size_t)M * sizeof(double));
+    };
+    int k = 0, prev_c0 = 0, prev_nc = 0;
+    for (int c0 = 0; c0 < N; c0 += chunk, ++k) {
+        const int b = k & 1;
         g.col0 = c0;
         g.ncols = std::min(chunk, N - c0);
-        g.D = d;
+        g.D = win[b];
         long long total = (long long)M * g.ncols;
         int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
-        hipLaunchKernelGGL(k_gen_cols, dim3(blocks), dim3(256), 0, 0, g);
+        hipLaunchKernelGGL(k_gen_cols, dim3(blocks), dim3(256), 0, s[b], g);
         HIPTRY(hipGetLastError());
-        HIPTRY(hipMemcpy2D(A + (size_t)c0 * LDA, (size_t)LDA * sizeof(double), d,
-                           (size_t)M * sizeof(double), (size_t)M * sizeof(double), g.ncols,
-                           hipMemcpyDeviceToHost));
+        if (direct)
+            HIPTRY(hipMemcpy2DAsync(A + (size_t)c0 * LDA, (size_t)LDA * sizeof(double), win[b],
+                                    (size_t)M * sizeof(double), (size_t)M * sizeof(double),
+                                    g.ncols, hipMemcpyDeviceToHost, s[b]));
+        else
+            HIPTRY(hipMemcpyAsync(pin[b], win[b], (size_t)M * g.ncols * sizeof(double),
+                                  hipMemcpyDeviceToHost, s[b]));
+        if (!direct && k >= 1) {
+            // chunk k-1 has landed in its pinned buffer: move it into A while
+            // chunk k is generated; chunk k+1 reuses that buffer only after
+            // this copy (it is queued by this thread afterwards)
+            HIPTRY(hipStreamSynchronize(s[b ^ 1]));
+            drain(b ^ 1, prev_c0, prev_nc);
+        }
+        prev_c0 = c0;
+        prev_nc = g.ncols;
     }
+    if (k >= 1) {
+        HIPTRY(hipStreamSynchronize(s[(k - 1) & 1]));
+        if (!direct) drain((k - 1) & 1, prev_c0, prev_nc);
+    }
+    HIPTRY(hipStreamSynchronize(s[k & 1]));
     // LIN/pdmatgen.f:516-543; AFORM 'F' overwrites the diagonal afterwards
     // (:546-626), so DIAG has no effect on it
     if (dg == 'D' && MB == NB && af != 'F') {
