@@ -21,7 +21,7 @@ from test_gpu_2_cholesky import L_TOL, expected_traffic, rel, slot
 pytestmark = pytest.mark.gpu
 
 
-def test_cfg1_reference_schedule(gpu, ora):
+def test_n4096_nb256_28mib_reference_schedule(gpu, ora):
     """n = 4096, NB = 256, memsize = 28 MiB (memBlock = 56 tiles) with one X
     column: the reference's four panels [2, 3, 5, 6], the reference's A2Y/A2X
     copy volume (oocquark.cpp:253-319) and the oracle's L."""

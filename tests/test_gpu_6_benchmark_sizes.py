@@ -31,7 +31,7 @@ GB = 1 << 30
 
 
 @pytest.mark.timeout(900)
-def test_cfg_out_of_core_n196608(gpu):
+def test_out_of_core_n196608_budget64g(gpu):
     ooc = gpu
     n, nb, mem = 196608, 1024, 64 << 30
     plan = ooc.panel_plan(n, nb, mem)
@@ -90,7 +90,7 @@ def dense_symmetric(ooc, buf, n, nb):
 
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("mu", [1e-2, 1e-6])
-def test_cfg_mixed_precision_n65536(gpu, mu):
+def test_mixed_precision_n65536(gpu, mu):
     ooc = gpu
     n, nb, nrhs = 65536, 512, 16
     buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
@@ -135,7 +135,7 @@ def test_cfg_mixed_precision_n65536(gpu, mu):
 
 
 @pytest.mark.timeout(600)
-def test_cfg_batched_10000(gpu):
+def test_batched_10000_systems(gpu):
     ooc = gpu
     rng = np.random.default_rng(2024)
     n = np.exp(rng.uniform(np.log(32), np.log(1024), size=10000)).astype(np.int32)
@@ -177,7 +177,7 @@ def poisson_workload(count, seed=7):
 
 
 @pytest.mark.timeout(600)
-def test_cfg_served_poisson_stream(gpu):
+def test_served_poisson_stream(gpu):
     """40 requests arriving as a Poisson stream at 4 requests/s into a
     3-slot server with a 64 GiB budget: every result at FP64 backward error,
     and sampled results (a fresh matrix, a cache hit, batched ones) equal to
