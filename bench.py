@@ -82,7 +82,7 @@ def verify_small_ooc(ooc) -> float:
 
 
 def cpu_baseline(n: int = 4096, nb: int = 256, memsize: int = 28 << 20,
-                 min_runs: int = 5, target_s: float = 3.0) -> dict:
+                 min_runs: int = 5, target_s: float = 10.0) -> dict:
     """The reference driver restated on the CPU (oracle/oracle.c: the
     reference's panel planner find_Yn, A2Y/A2X staging into a host buffer of
     memBlock tiles and the same tile-task order), one task per tile on all
@@ -101,7 +101,7 @@ def cpu_baseline(n: int = 4096, nb: int = 256, memsize: int = 28 << 20,
     plan = oracle.panel_plan(n, nb, memsize)
     times = []
     t_begin = time.perf_counter()
-    while len(times) < min_runs or (time.perf_counter() - t_begin < target_s and len(times) < 201):
+    while len(times) < min_runs or (time.perf_counter() - t_begin < target_s and len(times) < 1001):
         A = A0.copy(order="F")
         t, info = oracle.cholesky(A, nb, memsize)
         if info != 0:
