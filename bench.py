@@ -332,7 +332,8 @@ def bench_batched(ooc, n, r, reps: int, cpu_rec=None) -> dict:
         st["mean_iterations"] = float(iters.mean())
         out[prec] = st
         log(f"[batched] {prec}: {st['systems_per_second']:.0f} systems/s ({st['seconds'] * 1e3:.1f} ms"
-            f"/batch), variant T={st['variant_T']} RB={st['variant_RB']}, max berr "
+            f"/batch), variant T={st['variant_T']} RB={st['variant_RB']}, block-column launches "
+            f"for {st['split_systems']} systems (n >= {st['split_min_n']}), max berr "
             f"{st['max_backward_error']:.2e}, failed {st['failed']}, mean iters {st['mean_iterations']:.2f}")
     if cpu_rec:
         out["cpu_baseline"] = cpu_rec

@@ -235,10 +235,11 @@ int oocdag_normal_fill(double *dB, int rows, int cols, int ld, unsigned long lon
  * device arenas laid out by oocdag_batch_layout (A_s n x n and B_s/X_s
  * n x nrhs, column-major with leading dimension n rounded up to even).  One
  * persistent kernel (work queue over the systems, largest first; each
- * workgroup factors and solves one system with LDS-staged panels) replayed
- * through a captured hipGraph `reps` times; its block size and register
- * tiling are picked by timing the compiled variants on a sample of the
- * batch.  precision 0: FP64 factor; 1: FP32 factor + FP64 refinement to the
+ * workgroup factors and solves one system with LDS-staged panels), after
+ * block-column launches that factor the largest systems across the whole
+ * GPU when that is faster (oocdag_batch_set_split), replayed through a
+ * captured hipGraph `reps` times; its block size and register tiling are
+ * picked by timing the compiled variants on a sample of the batch.  precision 0: FP64 factor; 1: FP32 factor + FP64 refinement to the
  * backward error tol.  Per system (optional arrays): berr, the normwise
  * backward error max_k ||r_k|| / (||A|| ||x_k|| + ||b_k||); solve_resid,
  * PDLASCHK's max_k ||r_k|| / (||A|| ||x_k|| eps n) (LIN/pdlaschk.f:186,295);
@@ -253,6 +254,8 @@ typedef struct {
     int variant_T, variant_RB;  /* chosen block size and register tile */
     double tune_seconds;
     double max_solve_resid;     /* PDLASCHK residual, max over systems with status 0 */
+    int split_min_n;            /* systems with n >= this (0: none) were factored by */
+    int split_systems;          /* the block-column launches; how many */
 } oocdag_batch_stats;
 
 long long oocdag_batch_layout(int nsys, const int *n, const int *nrhs, long long *a_off,
@@ -270,6 +273,18 @@ int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, const long lon
  * (T, RB): right-looking with T x T diagonal blocks and RB x RB register
  * tiles, and the left-looking MFMA kernel (T = 64, RB = 0). */
 int oocdag_batch_set_variant(int precision, int variant);
+/* Fixes which systems oocdag_batched_solve factors by block-column launches
+ * instead of inside one workgroup each: those with n >= min_n (0: none).
+ * -1 restores the default: a call with reps > 0 times the whole batch with
+ * no split and with each candidate threshold (769, 513, 385, 257) and keeps
+ * the fastest; a call with reps <= 0 splits the systems of n >= 513 when
+ * there are at most two per CU of them.  Split systems take one launch per
+ * step of 64 columns for the update, the diagonal block and the rows below,
+ * across all of them at once (so a few large systems use the whole GPU);
+ * the left-looking kernel then solves them.  The factor is bit-identical
+ * either way.  Only the left-looking variant splits.  Returns OOCDAG_OK or
+ * OOCDAG_EARG. */
+int oocdag_batch_set_split(int precision, int min_n);
 
 /* Chooses how the mixed-precision factorization (oocdag_mpir_solve_tiles and
  * the server) runs its low-precision trailing updates, as bits: 1 = the

@@ -121,7 +121,8 @@ class BatchStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("systems_per_second", C.c_double),
                 ("max_backward_error", C.c_double), ("failed", C.c_int), ("variant_T", C.c_int),
                 ("variant_RB", C.c_int), ("tune_seconds", C.c_double),
-                ("max_solve_resid", C.c_double)]
+                ("max_solve_resid", C.c_double), ("split_min_n", C.c_int),
+                ("split_systems", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -133,6 +134,7 @@ SIGNATURES.update({
     "oocdag_batch_generate": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
                                         C.c_double, C.c_ulonglong]),
     "oocdag_batch_set_variant": (C.c_int, [C.c_int, C.c_int]),
+    "oocdag_batch_set_split": (C.c_int, [C.c_int, C.c_int]),
     "oocdag_mp_set_update_variant": (C.c_int, [C.c_int]),
     "oocdag_batched_solve": (C.c_int, [C.c_int, _ip, _ip, _llp, _llp, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp,
@@ -515,6 +517,14 @@ class Batch:
         if rc < 0:
             raise OocdagError(f"batch_set_variant({precision}, {variant}) failed")
         return rc
+
+    @staticmethod
+    def set_split(precision: str, min_n: int) -> None:
+        """Factor the systems with n >= min_n by block-column launches (0:
+        none, -1: autotune the threshold)."""
+        k = {"fp64": 0, "fp32": 1}[precision]
+        if lib().oocdag_batch_set_split(k, min_n) != 0:
+            raise OocdagError(f"batch_set_split({precision}, {min_n}) failed")
 
     def solve(self, precision="fp64", tol=None, max_iter=10, reps=1, with_resid=False):
         """Returns (stats, berr, iters, status), plus the per-system PDLASCHK

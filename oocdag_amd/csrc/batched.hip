@@ -19,10 +19,13 @@
 // report status != 0 so the caller can rerun them in FP64.
 //
 // A left-looking variant (64-wide block columns, MFMA updates, two
-// workgroups per CU) covers the same steps for larger systems.
+// workgroups per CU) covers the same steps for larger systems; its phases
+// also run as block-column launches across all the largest systems at once
+// (a few large systems then use the whole GPU instead of one CU each).
 //
 // Variants (T, RB, FT) are compiled ahead; the host autotuner times them on
-// a sample of the batch and launches the fastest through a captured hipGraph.
+// a sample of the batch, and the block-column threshold on the whole batch,
+// and launches the fastest through a captured hipGraph.
 #include "ooc_internal.h"
 
 #include <algorithm>
@@ -66,6 +69,8 @@ struct BatchArgs {
     double tol;
     int max_iter;
     int rowmap;   // FP32 MFMA result rows: 0: 4*(lane/16) + r, 1: lane/16 + 4r
+    int prefact;  // leading queue entries already factored (k_batched_ll solves them only)
+    int diag_k0;  // the diagonal-block step's column (k_batched_ll<FT, false>)
 };
 
 template <typename FT, int T>
@@ -594,8 +599,16 @@ __device__ void tri_solve_ll(SmemLL<FT> &S, const FT *W, const FT *Dinv, int n, 
 // of ~700 bytes per lane.)  Returns the
 // first bad pivot (1-based inside the sub-block) or 0.  Kept out of line:
 // its 48 live doubles must not add to the register pressure of the
-// caller's other phases.
-__device__ __noinline__ int factor16(double *D, int s0, int sw, double *Li, double *Di, int lane) {
+// caller's other phases.  The blocks come as LDS (address space 3)
+// pointers: from a generic pointer into the kernel's dynamic LDS an
+// out-of-line function finds the block's address through a per-kernel
+// offset table, one scalar memory load per access (~4x the factor's time).
+typedef __attribute__((address_space(3))) double lds_double;
+
+__device__ __forceinline__ lds_double *to_lds(double *p) { return (lds_double *)p; }
+
+__device__ __noinline__ int factor16(lds_double *D, int s0, int sw, lds_double *Li, lds_double *Di,
+                                     int lane) {
     double av[16], rdv[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c)
@@ -638,20 +651,185 @@ __device__ __noinline__ int factor16(double *D, int s0, int sw, double *Li, doub
     return bad;
 }
 
+// ---- U: rows r0 .. r0 + 127 (one 128-row tile) of block column
+// [k0, k0 + w): C = A[:, k0:k0+w] - L[:, :k0] L[k0:k0+w, :k0]^T, the lower
+// part only in the diagonal block; 4 waves as 2 x 2, each 64 rows x 32
+// columns.  With to_d the diagonal block (waves wm = 0 of the tile at
+// r0 = k0) goes straight to the LDS block D of the factorization (as the FT
+// value W would hold) instead of W.
 template <typename FT>
-__global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
+__device__ __forceinline__ void ll_update_tile(const double *A, FT *W, int n, int ld, int k0, int w,
+                                               int r0, bool to_d, double *D, int rowmap) {
     typedef typename AccT<FT>::type acc_t;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    SmemLL<FT> &S = *reinterpret_cast<SmemLL<FT> *>(smem_raw);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int il = lane & 15, kl = lane >> 4;
+    acc_t acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    // Operands straight from global memory into MFMA registers, no LDS and
+    // no barriers: lane (il, kl) of wave (wm, wn) feeds MFMA block i with
+    // row r0 + wm*64 + i*16 + il of L and block j with row
+    // k0 + wn*32 + j*16 + il, at column k + kl of the current 4-wide K slab.
+    // A ring of LRING slabs keeps LAHEAD slabs of loads in flight ahead of
+    // the MFMAs; rows past n are clamped to row n-1 (in bounds; their
+    // products land in rows the epilogue does not write) and past the last
+    // slab the loads repeat it (never used).
+    if (k0 > 0) {
+        constexpr int LRING = 4, LAHEAD = 3;
+        int ra_[4], rb_[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ra_[i] = min(r0 + wm * 64 + i * 16 + il, n - 1);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) rb_[j] = min(k0 + wn * 32 + j * 16 + il, n - 1);
+        const FT *colp = W + (long long)kl * ld;   // column kl of the current slab
+        const long long step = 4LL * ld;
+        const int ns = k0 >> 2;                     // slabs (a multiple of 16)
+        FT va[LRING][4], vb[LRING][2];
+        auto ld_slab = [&](FT (&xa)[4], FT (&xb)[2]) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xa[i] = colp[ra_[i]];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) xb[j] = colp[rb_[j]];
+        };
+#pragma unroll
+        for (int q = 0; q < LAHEAD; ++q) {
+            ld_slab(va[q], vb[q]);
+            colp += step;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        for (int s0 = 0; s0 < ns; s0 += LRING) {
+#pragma unroll
+            for (int q = 0; q < LRING; ++q) {
+                const int nx = (q + LAHEAD) & (LRING - 1);
+                ld_slab(va[nx], vb[nx]);
+                colp += (s0 + q + LAHEAD + 1 < ns) ? step : 0;
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(va[q][i], vb[q][j], acc[i][j]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    // C = A - acc (lower part only in the diagonal block).  All loads of A
+    // first: a store to W may alias A for the compiler, so interleaved they
+    // would be 32 round trips.
+    double aval[4][2][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = wn * 32 + j * 16 + il;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = wm * 64 + i * 16 + acc_row<FT>(lane, r, rowmap);
+                const int gr = r0 + rr;
+                const bool ok = gr < n && c < w && gr >= k0 + c;
+                aval[i][j][r] = ok ? A[gr + (long long)(k0 + c) * ld] : 0.0;
+            }
+        }
+    const bool dd = to_d && wm == 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = wn * 32 + j * 16 + il;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = wm * 64 + i * 16 + acc_row<FT>(lane, r, rowmap);
+                const int gr = r0 + rr;
+                if (gr >= n || c >= w || gr < k0 + c) continue;
+                const FT v = (FT)(aval[i][j][r] - (double)acc[i][j][r]);
+                if (dd)
+                    D[c * LS + rr] = (double)v;
+                else
+                    W[gr + (long long)(k0 + c) * ld] = v;
+            }
+        }
+}
+
+// ---- T: rows r0 .. r0 + 63 below the diagonal block of block column k0
+// (64 wide), X = C Di^T on MFMA.  Di(c, d) comes from the LDS block
+// (DiL[d * LS + c], double) or from the block's slot of the inverse store
+// (DiG[c + d * LT], FT): the same FT operand values either way.
+template <typename FT, bool DI_GLOBAL>
+__device__ __forceinline__ void ll_trsm_tile(FT *W, const double *DiL, const FT *DiG, int n, int ld,
+                                             int k0, int r0, int rowmap) {
+    typedef typename AccT<FT>::type acc_t;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int il = lane & 15, kl = lane >> 4;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    int rows[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) rows[i] = min(r0 + wm * 32 + i * 16 + il, n - 1);
+    // all 32 loads of the tile's rows first (one round trip), then the
+    // MFMAs with Di
+    FT fa[LT / 4][2];
+#pragma unroll
+    for (int ks = 0; ks < LT / 4; ++ks)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) fa[ks][i] = W[rows[i] + (long long)(k0 + ks * 4 + kl) * ld];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ks = 0; ks < LT / 4; ++ks) {
+        const int kk = ks * 4 + kl;
+        FT fb[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = wn * 32 + j * 16 + il;
+            fb[j] = DI_GLOBAL ? DiG[c + kk * LT] : (FT)DiL[kk * LS + c];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[ks][i], fb[j], acc[i][j]);
+    }
+    __syncthreads();   // every wave has read its rows before any is overwritten
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = wn * 32 + j * 16 + il;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = r0 + wm * 32 + i * 16 + acc_row<FT>(lane, r, rowmap);
+                if (gr >= n) continue;
+                W[gr + (long long)(k0 + c) * ld] = (FT)acc[i][j][r];
+            }
+        }
+    __syncthreads();
+}
+
+// FULL: the persistent kernel.  !FULL: the diagonal-block step of the
+// block-column launches (k_bl_update, k_bl_trsm below): workgroup b factors
+// and inverts the diagonal block at column a.diag_k0 of system a.order[b],
+// whose update is already in W, records a bad pivot in a.status and returns
+// -- the same D phase code as the persistent kernel's.
+template <typename FT, bool FULL = true>
+__global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    SmemLL<FT> &S = *reinterpret_cast<SmemLL<FT> *>(smem_raw);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (;;) {
-        if (tid == 0) S.sid = atomicAdd(a.counter, 1);
-        __syncthreads();
-        const int q = S.sid;
-        __syncthreads();
-        if (q >= a.nsys) break;
+        int q = blockIdx.x;
+        if (FULL) {
+            if (tid == 0) S.sid = atomicAdd(a.counter, 1);
+            __syncthreads();
+            q = S.sid;
+            __syncthreads();
+            if (q >= a.nsys) break;
+        } else if (a.status[a.order[q]] != 0) {
+            return;
+        }
         const int sidx = a.order[q];
         const SysDesc sd = a.sys[sidx];
         const int n = sd.n, nrhs_all = sd.nrhs, ld = sd.ld;
@@ -662,113 +840,43 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
         FT *Dinv = (FT *)a.Dinv + sd.d;
         if (n > PMAX || nrhs_all > RTOTAL) {
             if (tid == 0) a.status[sidx] = -1;
+            if (!FULL) return;   // one system per workgroup: never loop back
             continue;
         }
-        if (tid == 0) S.bad = 0;
-        __syncthreads();
-        const int nbk = (n + LT - 1) / LT;
-        for (int jb = 0; jb < nbk; ++jb) {
-            const int k0 = jb * LT, w = min(LT, n - k0), m = n - k0;
-            // ---- U: left-looking update of block column jb, 128-row tiles
-            // (4 waves as 2 x 2, each 64 rows x 32 columns)
-            const int nrt = (m + LRT - 1) / LRT;
-            for (int rt = 0; rt < nrt; ++rt) {
-                const int r0 = k0 + rt * LRT;
-                acc_t acc[4][2];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-                // Operands straight from global memory into MFMA registers,
-                // no LDS and no barriers: lane (il, kl) of wave (wm, wn)
-                // feeds MFMA block i with row r0 + wm*64 + i*16 + il of L
-                // and block j with row k0 + wn*32 + j*16 + il, at column
-                // k + kl of the current 4-wide K slab.  A ring of LRING
-                // slabs keeps LAHEAD slabs of loads in flight ahead of the
-                // MFMAs; rows past n are clamped to row n-1 (in bounds; their
-                // products land in rows the epilogue does not write) and
-                // past the last slab the loads repeat it (never used).
-                if (k0 > 0) {
-                    constexpr int LRING = 4, LAHEAD = 3;
-                    int ra_[4], rb_[2];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) ra_[i] = min(r0 + wm * 64 + i * 16 + il, n - 1);
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) rb_[j] = min(k0 + wn * 32 + j * 16 + il, n - 1);
-                    const FT *colp = W + (long long)kl * ld;   // column kl of the current slab
-                    const long long step = 4LL * ld;
-                    const int ns = k0 >> 2;                     // slabs (a multiple of 16)
-                    FT va[LRING][4], vb[LRING][2];
-                    auto ld_slab = [&](FT (&xa)[4], FT (&xb)[2]) {
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) xa[i] = colp[ra_[i]];
-#pragma unroll
-                        for (int j = 0; j < 2; ++j) xb[j] = colp[rb_[j]];
-                    };
-#pragma unroll
-                    for (int q = 0; q < LAHEAD; ++q) {
-                        ld_slab(va[q], vb[q]);
-                        colp += step;
-                        __builtin_amdgcn_sched_barrier(0);
-                    }
-                    for (int s0 = 0; s0 < ns; s0 += LRING) {
-#pragma unroll
-                        for (int q = 0; q < LRING; ++q) {
-                            const int nx = (q + LAHEAD) & (LRING - 1);
-                            ld_slab(va[nx], vb[nx]);
-                            colp += (s0 + q + LAHEAD + 1 < ns) ? step : 0;
-                            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                                for (int j = 0; j < 2; ++j)
-                                    acc[i][j] = mfma4(va[q][i], vb[q][j], acc[i][j]);
-                            __builtin_amdgcn_sched_barrier(0);
-                        }
-                    }
-                }
-                // C = A - acc (lower part only in the diagonal block).  All
-                // loads of A first: a store to W may alias A for the
-                // compiler, so interleaved they would be 32 round trips.
-                double aval[4][2][4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        const int c = wn * 32 + j * 16 + il;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int rr = wm * 64 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
-                            const int gr = r0 + rr;
-                            const bool ok = gr < n && c < w && gr >= k0 + c;
-                            aval[i][j][r] = ok ? A[gr + (long long)(k0 + c) * ld] : 0.0;
-                        }
-                    }
-                // the diagonal block (first tile, waves wm = 0) goes straight
-                // to the LDS block D of the factorization below (as the FT
-                // value W would hold); L_D reaches W once factored
-                const bool to_d = rt == 0 && wm == 0;
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        const int c = wn * 32 + j * 16 + il;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int rr = wm * 64 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
-                            const int gr = r0 + rr;
-                            if (gr >= n || c >= w || gr < k0 + c) continue;
-                            const FT v = (FT)(aval[i][j][r] - (double)acc[i][j][r]);
-                            if (to_d)
-                                S.u.f.D[c * LS + rr] = (double)v;
-                            else
-                                W[gr + (long long)(k0 + c) * ld] = v;
-                        }
-                    }
-                __syncthreads();
+        // the first a.prefact queue entries were factored by the block-column
+        // launches: their status holds 0 or the first bad pivot
+        const bool factored = FULL && q < a.prefact;
+        if (tid == 0) S.bad = factored ? a.status[sidx] : 0;
+        const int jb0 = FULL ? 0 : a.diag_k0 / LT;
+        const int jb1 = FULL ? (factored ? 0 : (n + LT - 1) / LT) : jb0 + 1;
+        if (!FULL) {
+            // the updated diagonal block's lower part from W (the FT values
+            // the update stored); the D phase sets the rest
+            const int k0 = a.diag_k0, w = min(LT, n - k0);
+            for (int idx = tid; idx < LT * LT; idx += NTB) {
+                const int r = idx & (LT - 1), c = idx >> 6;
+                if (r < w && c < w && r >= c)
+                    S.u.f.D[c * LS + r] = (double)W[(k0 + r) + (long long)(k0 + c) * ld];
             }
-            // ---- D: factor and invert the diagonal block in LDS (double)
-            // D's lower part came from the update; the rest is the identity
+        }
+        __syncthreads();
+        for (int jb = jb0; jb < jb1; ++jb) {
+            const int k0 = jb * LT, w = min(LT, n - k0), m = n - k0;
+            if (FULL) {
+                // ---- U: left-looking update of block column jb, 128-row
+                // tiles; the diagonal block goes straight to the LDS block D
+                const int nrt = (m + LRT - 1) / LRT;
+                for (int rt = 0; rt < nrt; ++rt) {
+                    ll_update_tile<FT>(A, W, n, ld, k0, w, k0 + rt * LRT, rt == 0, S.u.f.D, a.rowmap);
+                    __syncthreads();
+                }
+            }
+            // ---- D: factor and invert the diagonal block in LDS (double).
+            // D's lower part came from the update; the rest is the identity.
+            // On exit L_D is in W and the inverse Di in the block's slot of
+            // Dinv, or S.bad holds the first bad pivot (1-based, global row).
+            // (Inline here: moved into a function of its own, this phase
+            // pushes the kernel past 256 VGPRs into scratch spills.)
             double *D = S.u.f.D, *Di = S.u.f.Di;
             for (int idx = tid; idx < LT * LT; idx += NTB) {
                 const int r = idx & (LT - 1), c = idx >> 6;
@@ -780,13 +888,30 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
             // p0 .. p0+pw-1 (lower part, rows and columns from p0+pw):
             // either only the next sub-block's 16 x 16 diagonal block, or
             // everything else, by threads t0, t0 + nt, ...
+            // Only lower elements (i >= k) are enumerated: element t of the
+            // packed lower triangle, t = i (i + 1) / 2 + k, from t = 136 (past
+            // the leading 16 x 16 triangle) unless diag_only.  A full panel
+            // (pw = 16) loads its 32 operands before the products, which keep
+            // the order of the one-at-a-time loop.
             auto trail = [&](int p0, int pw, bool diag_only, int t0, int nt) {
                 const int b0 = p0 + pw, mt = w - b0, md = diag_only ? min(16, mt) : mt;
-                for (int idx = t0; idx < md * md; idx += nt) {
-                    const int i = idx % md, k = idx / md;
-                    if (i < k || (!diag_only && i < 16 && k < 16)) continue;
+                const int tend = md * (md + 1) / 2;
+                for (int t = (diag_only ? 0 : 136) + t0; t < tend; t += nt) {
+                    int i, k;
+                    tri_lower_decode(t, i, k);
                     double v = D[(b0 + k) * LS + b0 + i];
-                    for (int c = 0; c < pw; ++c) v -= D[(p0 + c) * LS + b0 + i] * D[(p0 + c) * LS + b0 + k];
+                    if (pw == 16) {
+                        double x[16], y[16];
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) {
+                            x[c] = D[(p0 + c) * LS + b0 + i];
+                            y[c] = D[(p0 + c) * LS + b0 + k];
+                        }
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) v -= x[c] * y[c];
+                    } else {
+                        for (int c = 0; c < pw; ++c) v -= D[(p0 + c) * LS + b0 + i] * D[(p0 + c) * LS + b0 + k];
+                    }
                     D[(b0 + k) * LS + b0 + i] = v;
                 }
             };
@@ -795,7 +920,7 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 double *Li = S.u.f.T;   // inverse of the sub-block's factor, Li(i, c) at i + 16c
                 if (wave == 0) {
                     // also the diagonal block of Di
-                    const int bad = factor16(D, s0, sw, Li, Di, lane);
+                    const int bad = factor16(to_lds(D), s0, sw, to_lds(Li), to_lds(Di), lane);
                     if (lane == 0 && bad && S.bad == 0) S.bad = k0 + s0 + bad;
                 } else if (s0 > 0) {
                     // meanwhile the rest of the previous panel's trailing update
@@ -814,13 +939,13 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                     double xo[4];
                     if (act) {
                         double rv[16];
-#pragma unroll
+        #pragma unroll
                         for (int d = 0; d < 16; ++d) rv[d] = (d < sw && d < 4 * g + 4) ? D[(s0 + d) * LS + i] : 0.0;
-#pragma unroll
+        #pragma unroll
                         for (int cc = 0; cc < 4; ++cc) {
                             const int c = 4 * g + cc;
                             double x = 0.0;
-#pragma unroll
+        #pragma unroll
                             for (int d = 0; d < 16; ++d)
                                 if (d <= c) x += rv[d] * Li[c + 16 * d];
                             xo[cc] = x;
@@ -828,7 +953,7 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                     }
                     __syncthreads();
                     if (act) {
-#pragma unroll
+        #pragma unroll
                         for (int cc = 0; cc < 4; ++cc)
                             if (4 * g + cc < sw) D[(s0 + 4 * g + cc) * LS + i] = xo[cc];
                     }
@@ -852,7 +977,18 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                         const int bj = idx >> 8, e = idx & 255, r = e & 15, c = e >> 4;
                         const int gr = bi * 16 + r, gc = bj * 16 + c;
                         double t = 0.0;
-                        for (int k = gc; k < bi * 16; ++k) t += D[k * LS + gr] * Di[gc * LS + k];
+                        int k = gc;
+                        for (; k + 8 <= bi * 16; k += 8) {   // 16 loads, then the products in order
+                            double dv[8], iv[8];
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) {
+                                dv[u] = D[(k + u) * LS + gr];
+                                iv[u] = Di[gc * LS + k + u];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) t += dv[u] * iv[u];
+                        }
+                        for (; k < bi * 16; ++k) t += D[k * LS + gr] * Di[gc * LS + k];
                         Tb[bj * 256 + c * 16 + r] = t;
                     }
                     __syncthreads();
@@ -860,7 +996,18 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                         const int bj = idx >> 8, e = idx & 255, r = e & 15, c = e >> 4;
                         const int gr = bi * 16 + r, gc = bj * 16 + c;
                         double x = 0.0;
-                        for (int k = 0; k <= r; ++k) x -= Di[(bi * 16 + k) * LS + gr] * Tb[bj * 256 + c * 16 + k];
+                        int k = 0;
+                        for (; k + 8 <= r + 1; k += 8) {
+                            double dv[8], tv[8];
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) {
+                                dv[u] = Di[(bi * 16 + k + u) * LS + gr];
+                                tv[u] = Tb[bj * 256 + c * 16 + k + u];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) x -= dv[u] * tv[u];
+                        }
+                        for (; k <= r; ++k) x -= Di[(bi * 16 + k) * LS + gr] * Tb[bj * 256 + c * 16 + k];
                         Di[gc * LS + gr] = x;
                     }
                     __syncthreads();
@@ -873,51 +1020,17 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
                 Dinv[(long long)jb * LT * LT + idx] = (FT)((r >= c) ? Di[c * LS + r] : 0.0);
             }
             __syncthreads();
-            // ---- T: rows below, X = C Di^T (Di(c, d) = Di[d * LS + c])
-            const int mb = m - w;
-            for (int r0 = k0 + w; r0 < k0 + w + mb; r0 += LT) {
-                acc_t acc[2][2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-                int rows[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) rows[i] = min(r0 + wm * 32 + i * 16 + il, n - 1);
-                // all 32 loads of the tile's rows first (one round trip),
-                // then the MFMAs with Di from LDS
-                FT fa[LT / 4][2];
-#pragma unroll
-                for (int ks = 0; ks < LT / 4; ++ks)
-#pragma unroll
-                    for (int i = 0; i < 2; ++i) fa[ks][i] = W[rows[i] + (long long)(k0 + ks * 4 + kl) * ld];
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int ks = 0; ks < LT / 4; ++ks) {
-                    const int kk = ks * 4 + kl;
-                    FT fb[2];
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) fb[j] = (FT)Di[kk * LS + wn * 32 + j * 16 + il];
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j) acc[i][j] = mfma4(fa[ks][i], fb[j], acc[i][j]);
-                }
-                __syncthreads();   // every wave has read its rows before any is overwritten
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        const int c = wn * 32 + j * 16 + il;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int gr = r0 + wm * 32 + i * 16 + acc_row<FT>(lane, r, a.rowmap);
-                            if (gr >= n) continue;
-                            W[gr + (long long)(k0 + c) * ld] = (FT)acc[i][j][r];
-                        }
-                    }
-                __syncthreads();
+            if (S.bad) break;
+            if (FULL) {
+                // ---- T: rows below, X = C Di^T
+                const int mb = m - w;
+                for (int r0 = k0 + w; r0 < k0 + w + mb; r0 += LT)
+                    ll_trsm_tile<FT, false>(W, S.u.f.Di, nullptr, n, ld, k0, r0, a.rowmap);
             }
+        }
+        if (!FULL) {
+            if (tid == 0 && S.bad) a.status[sidx] = S.bad;
+            return;
         }
         if (S.bad) {
             if (tid == 0) {
@@ -1099,6 +1212,45 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Block-column launches for the largest systems of a batch (queue entries
+// a.order[0 .. cnt), sorted largest first, so the systems still active at a
+// step are a prefix).  Inside one workgroup a system's 64-wide block columns
+// form a serial chain (update, diagonal factor, rows below); here step k0 of
+// every large system runs as three launches -- one workgroup per 128-row
+// update tile (k_bl_update), one per system for the diagonal block
+// (k_batched_ll<FT, false>), one per 64-row tile below it (k_bl_trsm) -- and
+// k_batched_ll then only solves and refines them (a.prefact = cnt).  The
+// phases are k_batched_ll's own code, so the factor is the same bit for bit.
+// A system whose diagonal block meets a bad pivot records it in a.status and
+// is skipped by the later steps.
+// ---------------------------------------------------------------------------
+template <typename FT>
+__global__ __launch_bounds__(NTB, 2) void k_bl_update(BatchArgs a, const int2 *items, int k0) {
+    const int2 it = items[xcd_remap(blockIdx.x, gridDim.x)];   // (system, 128-row tile)
+    if (a.status[it.x] != 0) return;
+    const SysDesc sd = a.sys[it.x];
+    ll_update_tile<FT>(a.A + sd.a, (FT *)a.W + sd.w, sd.n, sd.ld, k0, min(LT, sd.n - k0),
+                       k0 + it.y * LRT, false, nullptr, a.rowmap);
+}
+
+template <typename FT>
+__global__ __launch_bounds__(NTB, 2) void k_bl_trsm(BatchArgs a, const int2 *items, int k0) {
+    const int2 it = items[xcd_remap(blockIdx.x, gridDim.x)];   // (system, 64-row tile below)
+    if (a.status[it.x] != 0) return;
+    const SysDesc sd = a.sys[it.x];
+    ll_trsm_tile<FT, true>((FT *)a.W + sd.w, nullptr,
+                           (const FT *)a.Dinv + sd.d + (long long)(k0 / LT) * LT * LT, sd.n, sd.ld,
+                           k0, k0 + LT + it.y * LT, a.rowmap);
+}
+
+// status of queue entries order[0 .. cnt) <- 0 (the block-column launches
+// record bad pivots there)
+__global__ void k_zero_status(BatchArgs a, int cnt) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x)
+        a.status[a.order[q]] = 0;
+}
+
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
 __global__ void k_add_diag_one(double *A, int n, int ld, double mu) {
@@ -1143,6 +1295,63 @@ Variant variants_fp32[] = {
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
         .count();
+}
+
+typedef void (*BlTileKernel)(BatchArgs, const int2 *, int);
+struct BlKernels {
+    BlTileKernel update, trsm;
+    BatchKernel diag;
+};
+const BlKernels bl_kernels[2] = {
+    {k_bl_update<double>, k_bl_trsm<double>, k_batched_ll<double, false>},
+    {k_bl_update<float>, k_bl_trsm<float>, k_batched_ll<float, false>},
+};
+
+// Work lists of the block-column launches for queue entries order[0 .. nl)
+// (n descending): per step k0 = 64 j the systems still active (n > k0, a
+// prefix of the order), their 128-row update tiles from row k0 and their
+// 64-row tiles below the diagonal block, as (system, tile) pairs.
+struct BlPlan {
+    int nl = 0;
+    std::vector<int> active;        // systems per step
+    std::vector<int> u_off, t_off;  // per step (+ 1 end entry) into items
+    std::vector<int2> items;
+};
+
+BlPlan bl_plan(const std::vector<int> &order, const int *n, int nl) {
+    BlPlan p;
+    p.nl = nl;
+    if (nl == 0) return p;
+    const int steps = (n[order[0]] + LT - 1) / LT;
+    for (int j = 0; j < steps; ++j) {
+        const int k0 = j * LT;
+        int act = 0;
+        p.u_off.push_back((int)p.items.size());
+        for (; act < nl && n[order[act]] > k0; ++act)
+            for (int t = 0; t < (n[order[act]] - k0 + LRT - 1) / LRT; ++t)
+                p.items.push_back(make_int2(order[act], t));
+        p.active.push_back(act);
+    }
+    p.u_off.push_back((int)p.items.size());
+    for (int j = 0; j < steps; ++j) {
+        const int k0 = j * LT;
+        p.t_off.push_back((int)p.items.size());
+        for (int q = 0; q < p.active[j]; ++q) {
+            const int mb = n[order[q]] - k0 - LT;   // rows below the diagonal block
+            for (int t = 0; t < (mb > 0 ? (mb + LT - 1) / LT : 0); ++t)
+                p.items.push_back(make_int2(order[q], t));
+        }
+    }
+    p.t_off.push_back((int)p.items.size());
+    return p;
+}
+
+// Leading entries of the order (n descending) with n >= min_n.
+int count_at_least(const std::vector<int> &order, const int *n, int min_n) {
+    if (min_n <= 0) return 0;
+    int c = 0;
+    while (c < (int)order.size() && n[order[c]] >= min_n) ++c;
+    return c;
 }
 
 }  // namespace
@@ -1217,6 +1426,7 @@ namespace {
 std::mutex g_tune_mu;
 int g_tuned[2] = {-1, -1};
 int g_forced[2] = {-1, -1};
+int g_split_forced[2] = {-1, -1};
 constexpr int kVariants[2] = {3, 5};   // entries of variants_fp64 / variants_fp32
 }  // namespace
 
@@ -1226,6 +1436,14 @@ extern "C" int oocdag_batch_set_variant(int precision, int variant) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_forced[precision] = variant;
     return kVariants[precision];
+}
+
+extern "C" int oocdag_batch_set_split(int precision, int min_n) {
+    if (precision != 0 && precision != 1) return OOCDAG_EARG;
+    if (min_n < -1) return OOCDAG_EARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_split_forced[precision] = min_n;
+    return OOCDAG_OK;
 }
 
 namespace ooc {
@@ -1301,15 +1519,16 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     int *dorder = nullptr, *dint = nullptr, *dcounter = nullptr;
     double *dberr = nullptr;
     void *dW = nullptr, *dD = nullptr;
+    int2 *ditems = nullptr;   // work lists of the block-column launches
     struct F {
         hipStream_t s;
-        void **p[7];
+        void **p[8];
         ~F() {
             for (auto q : p)
                 if (*q) (void)hipFreeAsync(*q, s);
         }
     } fr{s, {(void **)&dsd, (void **)&dorder, (void **)&dint, (void **)&dcounter, (void **)&dberr, &dW,
-          &dD}};
+          &dD, (void **)&ditems}};
     if (hipMallocAsync((void **)&dsd, nsys * sizeof(SysDesc), s) != hipSuccess ||
         hipMallocAsync((void **)&dorder, nsys * sizeof(int), s) != hipSuccess ||
         hipMallocAsync((void **)&dint, 2 * (size_t)nsys * sizeof(int), s) != hipSuccess ||
@@ -1349,11 +1568,52 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         a.tol = tol;
         a.max_iter = max_iter;
         a.rowmap = rowmap;
+        a.prefact = 0;
+        a.diag_k0 = 0;
         return a;
     };
-    auto run = [&](const Variant &v, const int *ord, int cnt) -> hipError_t {
-        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, dcounter, 0);
+    // the systems of the sorted order with n >= min_n (0: none) go through
+    // the block-column launches; their work lists are uploaded to ditems
+    BlPlan plan;
+    auto set_plan = [&](int min_n) -> hipError_t {
+        plan = bl_plan(order, n, count_at_least(order, n, min_n));
+        if (ditems) {
+            (void)hipFreeAsync(ditems, s);
+            ditems = nullptr;
+        }
+        if (plan.items.empty()) return hipSuccess;
+        hipError_t e = hipMallocAsync((void **)&ditems, plan.items.size() * sizeof(int2), s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ditems, plan.items.data(), plan.items.size() * sizeof(int2),
+                               hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e;
+    };
+    // one batch: the block-column steps of the plan's systems (when split,
+    // with the left-looking kernel only: the one that can solve systems
+    // factored elsewhere), then the persistent kernel over the queue
+    auto run = [&](const Variant &v, const int *ord, int cnt, bool split) -> hipError_t {
         BatchArgs a = make_args(ord, cnt);
+        if (split && v.RB == 0 && plan.nl > 0) {
+            const BlKernels &bk = bl_kernels[precision];
+            hipLaunchKernelGGL(k_zero_status, dim3((plan.nl + 255) / 256), dim3(256), 0, s, a,
+                               plan.nl);
+            for (size_t j = 0; j < plan.active.size(); ++j) {
+                const int k0 = (int)j * LT;
+                const int nu = plan.u_off[j + 1] - plan.u_off[j];
+                const int nt = plan.t_off[j + 1] - plan.t_off[j];
+                hipLaunchKernelGGL(bk.update, dim3(nu), dim3(NTB), 0, s, a,
+                                   (const int2 *)ditems + plan.u_off[j], k0);
+                a.diag_k0 = k0;
+                hipLaunchKernelGGL(bk.diag, dim3(plan.active[j]), dim3(NTB), sizeof(SmemLL<double>),
+                                   s, a);
+                if (nt > 0)
+                    hipLaunchKernelGGL(bk.trsm, dim3(nt), dim3(NTB), 0, s, a,
+                                       (const int2 *)ditems + plan.t_off[j], k0);
+            }
+            a.prefact = plan.nl;
+        }
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, dcounter, 0);
         hipLaunchKernelGGL(v.k, dim3(ncu * v.wgs), dim3(NTB), v.smem, s, a);
         return hipGetLastError();
     };
@@ -1363,11 +1623,14 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         if (vs[i].smem > 0 && vs[i].smem <= 160 * 1024)
             (void)hipFuncSetAttribute((const void *)vs[i].k,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)vs[i].smem);
-    int best = -1;
+    (void)hipFuncSetAttribute((const void *)bl_kernels[precision].diag,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(SmemLL<double>));
+    int best = -1, split_n = -1;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         if (g_forced[precision] >= 0) best = g_forced[precision];
         else if (reps <= 0) best = g_tuned[precision];
+        split_n = g_split_forced[precision];
     }
     // --- autotune on a sample of the sorted order (skipped when a tuned
     // variant is cached and the caller asked for a plain run)
@@ -1384,7 +1647,7 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         double tbest = 1e30;
         for (int rep = 0; rep < 2; ++rep) {
             double t0 = now_s();
-            if (run(vs[i], dsample, (int)sample.size()) != hipSuccess ||
+            if (run(vs[i], dsample, (int)sample.size(), false) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess) {
                 tbest = 1e30;
                 break;
@@ -1403,15 +1666,51 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
         g_tuned[precision] = best;
     }
     }
-    st->tune_seconds = now_s() - tt0;
     if (vs[best].smem == 0 || vs[best].smem > 160 * 1024) return OOCDAG_EARG;
+    // --- the block-column threshold.  A plain run (reps <= 0, the server)
+    // takes the measured rule: systems of n >= 513 split when there are at
+    // most two per CU of them (more keep every CU busy inside the persistent
+    // kernel, which then overlaps their phases better than the launches do);
+    // otherwise timed on the whole batch, no split and each candidate that
+    // moves a different number of systems
+    if (split_n < 0 && vs[best].RB == 0 && reps <= 0) {
+        const int nl = count_at_least(order, n, 513);
+        split_n = (nl > 0 && nl <= 2 * ncu) ? 513 : 0;
+    }
+    if (split_n < 0 && vs[best].RB == 0) {
+        double best_t = 1e30;
+        int last_nl = -1;
+        for (int cand : {0, 769, 513, 385, 257}) {
+            const int nl = count_at_least(order, n, cand);
+            if (nl == last_nl) continue;
+            last_nl = nl;
+            if (set_plan(cand) != hipSuccess) return OOCDAG_EHIP;
+            double tbest = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                const double t0 = now_s();
+                if (run(vs[best], dorder, nsys, true) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess)
+                    return OOCDAG_EHIP;
+                tbest = std::min(tbest, now_s() - t0);
+            }
+            if (tbest < best_t) {
+                best_t = tbest;
+                split_n = cand;
+            }
+        }
+    }
+    if (split_n < 0 || vs[best].RB != 0) split_n = 0;
+    if (set_plan(split_n) != hipSuccess) return OOCDAG_EHIP;
+    st->tune_seconds = now_s() - tt0;
+    st->split_min_n = split_n;
+    st->split_systems = plan.nl;
     st->variant_T = vs[best].T;
     st->variant_RB = vs[best].RB;
     int rc = OOCDAG_OK;
     if (reps <= 0) {
         // one direct launch on this call's stream (used by the server)
         const double t0 = now_s();
-        if (run(vs[best], dorder, nsys) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        if (run(vs[best], dorder, nsys, true) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
             return OOCDAG_EHIP;
         st->seconds = now_s() - t0;
     } else {
@@ -1419,7 +1718,7 @@ extern "C" int oocdag_batched_solve(int nsys, const int *n, const int *nrhs, con
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return OOCDAG_EHIP;
-    hipError_t ce = run(vs[best], dorder, nsys);
+    hipError_t ce = run(vs[best], dorder, nsys, true);
     if (hipStreamEndCapture(s, &graph) != hipSuccess || ce != hipSuccess) return OOCDAG_EHIP;
     if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
         (void)hipGraphDestroy(graph);

@@ -322,15 +322,6 @@ __device__ __forceinline__ void decode_block(long long L, int Mb, int Nb, int &m
     }
 }
 
-// Dispatch order -> logical order: XCD x (= blockIdx % 8) gets a contiguous
-// slice of logical ids.
-__device__ __forceinline__ long long xcd_remap(long long bid, long long G) {
-    const int NX = 8;
-    long long per = G / NX, rem = G % NX;
-    long long x = bid % NX, i = bid / NX;
-    return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
-}
-
 template <int BMT, int BN, int MODE, int KIND, int TB, int FULL>
 __global__ __launch_bounds__(NT, 2) void k_dgemm_nt(KArgs a) {
     constexpr int BM = BMT;         // block rows (128, or 32 for few right-hand sides)

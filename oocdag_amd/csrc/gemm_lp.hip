@@ -47,13 +47,6 @@ __device__ __forceinline__ long long tix(int ti, int tk, int ym, int tri) {
     return ti + (long long)tk * ym - (tri ? ((long long)tk * (tk + 1)) / 2 : 0);
 }
 
-__device__ __forceinline__ long long xcd_remap(long long bid, long long G) {
-    const int NX = 8;
-    long long per = G / NX, rem = G % NX;
-    long long x = bid % NX, i = bid / NX;
-    return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
-}
-
 // Blocks in rows [0, R) of a lower trapezoid with Nb block columns.
 __device__ __forceinline__ long long trap_rows(long long R, int Nb) {
     return R <= Nb ? R * (R + 1) / 2 : (long long)Nb * (Nb + 1) / 2 + (R - Nb) * Nb;

@@ -225,4 +225,14 @@ hipError_t normal_block(double *dB, int rows, int cols, long long ld, unsigned l
 hipError_t symm_t(const double *dA, int N, int NB, const double *Xt, double *Yt, long long ld,
                   int nrhs, double alpha, double beta, hipStream_t s);
 
+// Dispatch order -> logical order for a grid of G workgroups: XCD x
+// (= blockIdx % 8, round-robin dispatch) gets a contiguous slice of logical
+// ids, so neighbouring work shares that XCD's L2.  A bijection on [0, G).
+__device__ __forceinline__ long long xcd_remap(long long bid, long long G) {
+    const int NX = 8;
+    long long per = G / NX, rem = G % NX;
+    long long x = bid % NX, i = bid / NX;
+    return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
 }  // namespace ooc
