@@ -314,12 +314,15 @@ def batched_workload(nsys: int):
     return n, r
 
 
-def bench_batched(ooc, n, r, reps: int, cpu_rec=None) -> dict:
+def bench_batched(ooc, n, r, reps: int, cpu_rec=None, split: int = -1) -> dict:
     """10000 independent SPD systems, n log-uniform in [32, 1024], A = G G^T/n
     + 0.1 I, 1..8 N(0,1) right-hand sides; FP64 and FP32+IR; systems/s of the
-    graph-replayed persistent kernel and per-system backward error."""
+    graph-replayed persistent kernel and per-system backward error.  split:
+    the block-column threshold (-1: autotuned, 0: none, k: systems of n >= k)."""
     import numpy as np
     nsys = len(n)
+    for prec in ("fp64", "fp32"):
+        ooc.Batch.set_split(prec, split)
     bt = ooc.Batch(n, r)
     t0 = time.perf_counter()
     bt.generate(mu=0.1, seed=2024)
@@ -513,6 +516,12 @@ def main() -> int:
                     help="mpir mode: comma-separated subset of fp64,fp32,bf16")
     ap.add_argument("--mpir-mus", default="1e-2,1e-6",
                     help="mpir mode: comma-separated diagonal shifts mu")
+    ap.add_argument("--batch-uniform", default="", metavar="COUNT,N",
+                    help="batched mode: COUNT systems of order N with 4 right-hand sides "
+                         "instead of the 10000-system workload")
+    ap.add_argument("--batch-split", type=int, default=-1, metavar="MIN_N",
+                    help="batched mode: factor the systems of n >= MIN_N by block-column "
+                         "launches (0: none; -1: autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run device-to-device copies of BYTES (a plain copy kernel of "
                          "known size that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -524,7 +533,7 @@ def main() -> int:
         return 2
 
     cpu_batched = None
-    if args.mode == "batched" and not args.no_cpu:
+    if args.mode == "batched" and not args.no_cpu and not args.batch_uniform:
         # the CPU reference's worker processes fork before this process
         # touches the GPU
         cpu_batched = cpu_batched_baseline(*batched_workload(10000))
@@ -557,13 +566,19 @@ def main() -> int:
         return 0
 
     if args.mode == "batched":
-        res = bench_batched(ooc, *batched_workload(10000), max(1, args.steps) * 3,
-                            cpu_rec=cpu_batched)
+        workload = "10000 SPD systems n~logU[32,1024], 1-8 rhs"
+        sizes = batched_workload(10000)
+        if args.batch_uniform:
+            import numpy as np
+            count, order = (int(x) for x in args.batch_uniform.split(","))
+            sizes = (np.full(count, order, dtype=np.int32), np.full(count, 4, dtype=np.int32))
+            workload = f"{count} SPD systems of n={order}, 4 rhs"
+        res = bench_batched(ooc, *sizes, max(1, args.steps) * 3, cpu_rec=cpu_batched,
+                            split=args.batch_split)
         print(json.dumps({"metric": "batched small SPD factor+solve throughput", "value":
                           round(res["fp64"]["systems_per_second"], 1), "unit": "systems/s",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
-                          "config": {"workload": "10000 SPD systems n~logU[32,1024], 1-8 rhs",
-                                     "results": res}}), flush=True)
+                          "config": {"workload": workload, "results": res}}), flush=True)
         return 0
 
     if args.mode == "mpir":
