@@ -356,14 +356,12 @@ void oocdag_server::run_small(std::vector<Job> &jobs, hipStream_t s, double t_st
     }
     if (rc == OOCDAG_OK && hipStreamSynchronize(s) != hipSuccess) rc = OOCDAG_EHIP;
     const double t_input = now() - t_start;
-    // small systems: FP64 factor (FP32 + refinement does not pay at these
-    // orders); the batched kernel still refines when the error is above tol
-    int prec = 0;
+    // small systems: FP64 factor whatever precision was asked for (FP32 +
+    // refinement does not pay at these orders); the batched kernel still
+    // refines when the error is above tol, the tightest tol of the batch
+    const int prec = 0;
     double tol = 1.0;
-    for (auto &j : jobs) {
-        if (j.rq.precision == 0) prec = 0;
-        tol = std::min(tol, j.rq.tol);
-    }
+    for (auto &j : jobs) tol = std::min(tol, j.rq.tol);
     std::vector<double> berr(k), lres(k);
     std::vector<int> iters(k), status(k);
     oocdag_batch_stats st;
@@ -371,19 +369,6 @@ void oocdag_server::run_small(std::vector<Job> &jobs, hipStream_t s, double t_st
         rc = oocdag_batched_solve(k, n.data(), r.data(), a_off.data(), b_off.data(), A, B, X, prec,
                                   tol, 10, 0, berr.data(), lres.data(), iters.data(),
                                   status.data(), &st);
-    // systems whose FP32 refinement did not reach tol: redo in FP64
-    if (rc == OOCDAG_OK && prec == 1) {
-        std::vector<int> redo;
-        for (int i = 0; i < k; ++i)
-            if (status[i] != 0) redo.push_back(i);
-        if (!redo.empty()) {
-            rc = oocdag_batched_solve(k, n.data(), r.data(), a_off.data(), b_off.data(), A, B, X, 0, tol,
-                                      10, 0, berr.data(), lres.data(), iters.data(),
-                                      status.data(), &st);
-            for (int i : redo) out[i].r.fell_back = 1;
-            if (!redo.empty()) prec = 0;
-        }
-    }
     std::vector<double> hx(b_off[k]);
     if (rc == OOCDAG_OK &&
         (hipMemcpyAsync(hx.data(), X, b_off[k] * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -454,28 +439,48 @@ void oocdag_server::run_large(Job &j, CacheEntry *ce, bool hit, hipStream_t s, h
             ce->A = nullptr;
         }
     } else if (rc == OOCDAG_OK) {
-        // cached factor: refinement solve only; a stall refactors in FP64
+        // cached factor: refinement solve only.  A stall refactors in FP64
+        // into a factor of this request's own, inside extra budget reserved
+        // for it; other requests may be solving with the cached factor at
+        // the same time, so the entry takes the FP64 factor (and the
+        // low-precision one is freed) only when this request is its sole user.
         const double ts = now();
         int it = 0;
         double eta = 0, lres = 0;
         int r = mp_solve(ce->f, B, q.nrhs, ldb, X, ldb, q.tol, 30, s, &it, &eta, &lres);
+        st.precision_used = ce->f.precision;
         if (r == 1 && ce->f.precision != 0) {
-            MpFactor f64 = ce->f;
-            f64.W = f64.Z = f64.dinv = nullptr;
-            f64.W32 = f64.Z32 = nullptr;
-            rc = mp_factor(f64, ce->A, n, nb, 0, s, s1);
-            if (rc == OOCDAG_OK) {
-                mp_release(ce->f, s);
-                ce->f = f64;
-                r = mp_solve(ce->f, B, q.nrhs, ldb, X, ldb, q.tol, 30, s, &it, &eta, &lres);
-                st.fell_back = 1;
+            const size_t extra = mp_factor_bytes(n, nb);
+            bool got;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                got = make_room(extra, s);
+                if (got) used += extra;
+            }
+            if (!got) {
+                rc = OOCDAG_EBUDGET;
+            } else {
+                MpFactor f64 = ce->f;
+                f64.W = f64.Z = f64.dinv = nullptr;
+                f64.W32 = f64.Z32 = nullptr;
+                rc = mp_factor(f64, ce->A, n, nb, 0, s, s1);
+                if (rc == OOCDAG_OK) {
+                    r = mp_solve(f64, B, q.nrhs, ldb, X, ldb, q.tol, 30, s, &it, &eta, &lres);
+                    st.fell_back = 1;
+                    st.precision_used = 0;
+                }
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (rc == OOCDAG_OK && ce->users == 1) std::swap(ce->f, f64);
+                    used -= extra;
+                }
+                mp_release(f64, s);   // the replaced factor, or this request's own
             }
         }
         if (r < 0) rc = r;
         st.iterations = it;
         st.backward_error = eta;
         st.solve_resid = lres;
-        st.precision_used = ce->f.precision;
         st.solve_seconds = now() - ts;
     }
     if (rc == OOCDAG_OK) {

@@ -168,3 +168,30 @@ def test_server_results_carry_solve_resid(gpu):
     srv.close()
     for (res, x), r in zip(out, reqs):
         assert res["status"] == 0 and 0 < res["solve_resid"] <= 10.0, res
+
+
+def test_server_stalled_hit_refactors_without_touching_the_shared_factor(gpu):
+    """Cache hits whose refinement on the cached BF16 factor stalls (a target
+    no factor reaches) refactor in FP64 into a factor of their own: two such
+    hits run at once on two slots, each solves with its own FP64 factor, the
+    shared cached factor is only replaced when one request is its sole user,
+    and the memory reserved for the extra factor is returned."""
+    ooc = gpu
+    r1 = dict(matrix_id=7, n=8192, nrhs=2, mu=1e-2, seed=31, rhs_seed=32)
+    srv = ooc.Server(64 * GB, slots=2)
+    (res1, x1), = run(srv, [r1], prec="bf16")
+    assert res1["status"] == 0 and res1["precision_used"] == 2 and res1["cache_hit"] == 0
+    used_cached = srv.stats()["used_bytes"]
+    hits = [dict(r1, rhs_seed=40 + k) for k in range(2)]
+    tickets = [srv.submit(precision="bf16", tol=1e-300, **h) for h in hits]
+    out = [srv.wait(t, want_x=True) for t in tickets]
+    for res, x in out:
+        assert res["status"] == 0 and res["cache_hit"] == 1, res
+        assert res["fell_back"] == 1 and res["precision_used"] == 0, res
+        assert res["backward_error"] <= TOL, res   # FP64 reaches 8 eps, not 1e-300
+    assert srv.stats()["used_bytes"] == used_cached
+    # the entry still serves: a later hit at the ordinary target
+    (res3, x3), = run(srv, [dict(r1, rhs_seed=32)], prec="bf16")
+    srv.close()
+    assert res3["status"] == 0 and res3["cache_hit"] == 1 and res3["backward_error"] <= TOL
+    assert np.abs(x3 - x1).max() / np.abs(x1).max() <= 1e-10
