@@ -773,6 +773,23 @@ extern "C" oocdag_server *oocdag_server_create(size_t hbm_budget_bytes, int slot
         (void)hipStreamDestroy(ps);
         if (!probed) return nullptr;
     }
+    // Requests allocate stream-ordered (hipMallocAsync) from the device's
+    // default pool, which by default hands freed memory back to the driver at
+    // every synchronization, so each request's allocations went to the
+    // driver again (0.1-0.3 ms each).  The pool now keeps up to 1 GiB (or the
+    // budget, if smaller) of freed memory for reuse; larger blocks, such as
+    // an out-of-core panel, are still returned.  The threshold is raised,
+    // never lowered, and stays for the life of the process.
+    {
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = std::min<uint64_t>(hbm_budget_bytes, 1ull << 30), cur = 0;
+            if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &cur) == hipSuccess &&
+                cur < keep)
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    }
     oocdag_server *srv = new oocdag_server();
     srv->budget = hbm_budget_bytes;
     srv->nslots = slots;
