@@ -967,51 +967,31 @@ __global__ __launch_bounds__(NTB, 2) void k_batched_ll(BatchArgs a) {
             }
             if (S.bad) break;
             // inverse, blocked by 16: the diagonal blocks X_ii are the
-            // sub-block inverses factor16 left in Di; block rows i = 1..3
-            // from X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj (two barriers per row)
-            {
-                double *Tb = S.u.f.T;
-                for (int bi = 1; bi * 16 < w; ++bi) {
-                    // T_bj = sum_{k=j}^{bi-1} L_(bi,k) X_(k,j), j < bi: element per thread
-                    for (int idx = tid; idx < bi * 256; idx += NTB) {
-                        const int bj = idx >> 8, e = idx & 255, r = e & 15, c = e >> 4;
-                        const int gr = bi * 16 + r, gc = bj * 16 + c;
-                        double t = 0.0;
-                        int k = gc;
-                        for (; k + 8 <= bi * 16; k += 8) {   // 16 loads, then the products in order
-                            double dv[8], iv[8];
+            // sub-block inverses factor16 left in Di; block row bi = 1..3 from
+            // X_(bi,bj) = -X_(bi,bi) sum_{kb=bj}^{bi-1} L_(bi,kb) X_(kb,bj), one
+            // block per wave (bj = wave), the 16 x 16 products on MFMA.  The
+            // sum T as the accumulators hold it (row 4r + lane/16 of column
+            // lane%16 in t[r]) is the next product's B operand for k-step r
+            // as it stands.  One barrier per block row.
+            for (int bi = 1; bi * 16 < w; ++bi) {
+                if (wave < bi) {
+                    const int bj = wave, il = lane & 15, kl = lane >> 4;
+                    d4 t = d4{0, 0, 0, 0};
+                    for (int kb = bj; kb < bi; ++kb) {
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) {
-                                dv[u] = D[(k + u) * LS + gr];
-                                iv[u] = Di[gc * LS + k + u];
-                            }
-#pragma unroll
-                            for (int u = 0; u < 8; ++u) t += dv[u] * iv[u];
+                        for (int k0 = 0; k0 < 16; k0 += 4) {
+                            const int k = kb * 16 + k0 + kl;
+                            t = mfma4(D[k * LS + bi * 16 + il], Di[(bj * 16 + il) * LS + k], t);
                         }
-                        for (; k < bi * 16; ++k) t += D[k * LS + gr] * Di[gc * LS + k];
-                        Tb[bj * 256 + c * 16 + r] = t;
                     }
-                    __syncthreads();
-                    for (int idx = tid; idx < bi * 256; idx += NTB) {
-                        const int bj = idx >> 8, e = idx & 255, r = e & 15, c = e >> 4;
-                        const int gr = bi * 16 + r, gc = bj * 16 + c;
-                        double x = 0.0;
-                        int k = 0;
-                        for (; k + 8 <= r + 1; k += 8) {
-                            double dv[8], tv[8];
+                    d4 x = d4{0, 0, 0, 0};
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) {
-                                dv[u] = Di[(bi * 16 + k + u) * LS + gr];
-                                tv[u] = Tb[bj * 256 + c * 16 + k + u];
-                            }
+                    for (int r = 0; r < 4; ++r)
+                        x = mfma4(Di[(bi * 16 + 4 * r + kl) * LS + bi * 16 + il], t[r], x);
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) x -= dv[u] * tv[u];
-                        }
-                        for (; k <= r; ++k) x -= Di[(bi * 16 + k) * LS + gr] * Tb[bj * 256 + c * 16 + k];
-                        Di[gc * LS + gr] = x;
-                    }
-                    __syncthreads();
+                    for (int r = 0; r < 4; ++r) Di[(bj * 16 + il) * LS + bi * 16 + kl + 4 * r] = -x[r];
                 }
+                __syncthreads();
             }
             // L_D into W, Di into the inverse store
             for (int idx = tid; idx < LT * LT; idx += NTB) {
