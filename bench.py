@@ -516,6 +516,9 @@ def main() -> int:
                     help="mpir mode: comma-separated subset of fp64,fp32,bf16")
     ap.add_argument("--mpir-mus", default="1e-2,1e-6",
                     help="mpir mode: comma-separated diagonal shifts mu")
+    ap.add_argument("--mp-variant", type=int, default=0, metavar="BITS",
+                    help="mpir mode: oocdag_mp_set_update_variant bits (4: the FP32 route's "
+                         "super-column updates on FP32 MFMA instead of three BF16 pieces)")
     ap.add_argument("--batch-uniform", default="", metavar="COUNT,N",
                     help="batched mode: COUNT systems of order N with 4 right-hand sides "
                          "instead of the 10000-system workload")
@@ -585,13 +588,15 @@ def main() -> int:
         n = 65536 if args.n == 196608 else args.n
         precs = [p for p in args.mpir_precisions.split(",") if p]
         mus = [float(m) for m in args.mpir_mus.split(",") if m]
+        ooc.mp_set_update_variant(args.mp_variant)
         res = bench_mpir(ooc, n, 512, 16, mus, precs)
         best = max([v["speedup_vs_fp64"] for r in res.values() for k, v in r.items()
                     if "speedup_vs_fp64" in v and v["fell_back"] == 0] or [float("nan")])
         print(json.dumps({"metric": "mixed-precision+IR speedup over plain FP64 at FP64 backward "
                           "error", "value": round(best, 3), "unit": "x", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f64/f32/bf16",
-                          "config": {"workload": f"Gram SPD n={n} nb=512 nrhs=16", "results": res}}),
+                          "config": {"workload": f"Gram SPD n={n} nb=512 nrhs=16",
+                                     "mp_variant": args.mp_variant, "results": res}}),
               flush=True)
         return 0
 

@@ -193,8 +193,10 @@ int oocdag_solve_resid_tiles(const double *dA, int N, int NB, const double *dX, 
 /* ---------------------------------------------------------------------------
  * Mixed precision + iterative refinement (no reference counterpart: the
  * reference factors in FP64 only).  Factor a copy of the packed FP64 tiles
- * dA with the trailing updates in FP64 (precision 0), FP32 (1) or BF16 with
- * per-tile power-of-two scaling (2), FP32 accumulation; diagonal POTRF, TRSM,
+ * dA with the trailing updates in FP64 (precision 0), FP32 (1: FP32
+ * accuracy, the super-column updates on BF16 MFMAs from three BF16 pieces
+ * per element) or BF16 with per-tile power-of-two scaling (2), FP32
+ * accumulation; diagonal POTRF, TRSM,
  * residual r = b - A x and the update x += d stay in FP64.  Refine until the
  * normwise backward error max_k ||b_k - A x_k|| / (||A|| ||x_k|| + ||b_k||)
  * (infinity norms) is <= tol; a low-precision factor that is not positive
@@ -291,9 +293,11 @@ int oocdag_batch_set_split(int precision, int min_n);
  * LDS-staged FP32 update kernel instead of the one that loads operands
  * straight into MFMA registers; 2 = each super-column's bulk update runs
  * after the next super-column's FP64 factorization instead of beside it on a
- * second stream.  Every variant gives bit-identical results.  A test and
- * tuning hook with no reference counterpart; 0 restores the default; returns
- * the number of variants (4) or OOCDAG_EARG. */
+ * second stream; 4 = the FP32 route's super-column updates on FP32 MFMA
+ * instead of from three BF16 pieces per element.  Variants 0-3 give
+ * bit-identical results; bit 4 changes the FP32 factor's rounding (same
+ * accuracy).  A test and tuning hook with no reference counterpart; 0
+ * restores the default; returns the number of variants (8) or OOCDAG_EARG. */
 int oocdag_mp_set_update_variant(int variant);
 
 /* ---------------------------------------------------------------------------

@@ -471,7 +471,9 @@ def mp_set_update_variant(variant: int) -> int:
     """Choose how the mixed-precision factor runs its low-precision updates,
     as bits: 1 = LDS-staged FP32 kernel (else operands straight into MFMA
     registers), 2 = bulk update serialized after the next super-column (else
-    overlapped on a second stream); 0 is the default."""
+    overlapped on a second stream), 4 = the FP32 route's super-column updates
+    on FP32 MFMA (else from three BF16 pieces per element); 0 is the
+    default.  Returns the number of variants."""
     rc = lib().oocdag_mp_set_update_variant(variant)
     if rc < 0:
         raise OocdagError(f"mp_set_update_variant({variant}) failed")

@@ -14,7 +14,11 @@
 //          rounding; the kernel rescales its FP32 accumulators by the
 //          (exact, power-of-two) ratio of the scale products at each tile
 //          boundary along K.
-// Accumulation is FP32 in both cases; the diagonal POTRF, the TRSM, the
+//   SPLIT3: FP32 accuracy from BF16 MFMAs: each element as three BF16
+//          pieces, row-major, six products per pair of elements (the FP32
+//          route's super-column updates; FP32 MFMA peaks at 157 TFLOP/s,
+//          BF16 at 2.5 PFLOP/s).
+// Accumulation is FP32 in every case; the diagonal POTRF, the TRSM, the
 // residual and the solution update remain FP64 (factor.hip, solve.hip).
 //
 // The accumulator row order of the FP32-output MFMAs is read from a one-wave
@@ -208,6 +212,44 @@ __global__ __launch_bounds__(256) void k_convert_super_bf16(const double *W, int
     for (int q = 0; q < 16; ++q) {
         const int r = ty + 4 * q;
         X[(long long)(r0 + r) * ldx + k0 + tx] = f2bf(T[tx][r]);
+    }
+}
+
+// SPLIT3: x = h + m + l, each a BF16 (h = bf16(x), m = bf16(x - h),
+// l = bf16(x - h - m); every difference is exact in FP64), row-major with
+// the three pieces side by side in a row: element (r, k) piece p at
+// X[r * 3K + p * K + k].  Together they hold x to about 2^-24 relative, as
+// FP32 does.  No scales: BF16 has FP32's exponent range.
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+__global__ __launch_bounds__(256) void k_convert_super_split3(const double *W, int nb, int ym,
+                                                              int j0, int i0, unsigned short *X,
+                                                              long long K) {
+    __shared__ double T[64][65];
+    const int pr = blockIdx.x, pk = blockIdx.y;
+    const int r0 = pr * 64, k0 = pk * 64;
+    const int ti = r0 / nb, tk = k0 / nb;
+    const double *tile = W + tix(i0 + ti, j0 + tk, ym, 1) * (long long)nb * nb;
+    const int rr = r0 - ti * nb, kk = k0 - tk * nb;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = ty + 4 * q;
+        T[k][tx] = tile[(rr + tx) + (long long)(kk + k) * nb];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int r = ty + 4 * q;
+        const double x = T[tx][r];
+        const unsigned short h = f2bf((float)x);
+        const double r1 = x - (double)bf2f(h);
+        const unsigned short m = f2bf((float)r1);
+        const unsigned short l = f2bf((float)(r1 - (double)bf2f(m)));
+        unsigned short *p = X + (long long)(r0 + r) * 3 * K + k0 + tx;
+        p[0] = h;
+        p[K] = m;
+        p[2 * K] = l;
     }
 }
 
@@ -568,11 +610,173 @@ __global__ __launch_bounds__(NT, 2) void k_update_bf16(LpArgs a) {
     }
 }
 
+// FP32-accurate update from the three BF16 pieces of X (SPLIT3 layout):
+// X_m X_n^T = hh + hm + mh + hl + lh + mm with the pieces' products of
+// order 2^-24 and below (ml, lm, ll) left out, six v_mfma_f32_16x16x32_bf16
+// per 32-deep step: FP32 accuracy at 96 cycles per 16 x 16 x 32 against
+// eight FP32 16 x 16 x 4 MFMAs' 256.  The BF16 MFMA adds its 32 products to
+// the accumulator exactly only down to about 2^-27 of the accumulator's
+// magnitude and drops the bits below (32 products of 2^-28 each leave an
+// accumulator of 1.0 unchanged; the FP32 MFMA rounds each addition to
+// nearest instead).  So hh, the large term, goes into the running sum, and
+// the five small terms of each 32-deep step into a chain of their own that
+// starts from zero (what it drops is 2^-27 of its own, ~2^-8 smaller,
+// magnitude), added to the running sum with one FP32 add (round to nearest):
+// with every term in the running sum the dropped bits of the small products
+// biased the factor enough to triple the refinement steps at kappa ~ 1e6.
+// One LDS buffer per operand and piece (2 x 3 x 128 rows of 64 bytes, 48 KB,
+// two workgroups per CU); the next 32-deep slab is loaded into registers
+// while the current one is multiplied.  A row's four 16-byte chunks are
+// stored at chunk ^ ((row / 2) % 4): timed on gfx950 with this kernel's
+// access patterns, the fragment reads (16 rows x one chunk per 16 lanes)
+// ran 1.9x and the staging writes 1.2x faster than with 80-byte padded rows.
+constexpr int BKS = 32;             // bf16 per K step (one 64-byte LDS row)
+__device__ __forceinline__ int swz(int row) { return (row >> 1) & 3; }
+
+template <int ROWMAP>
+__global__ __launch_bounds__(NT, 2) void k_update_split3(LpArgs a) {
+    if (stopped_wg(a.stop)) return;
+    __shared__ unsigned short Ls[2][3][BM * BKS];   // [A rows / B rows][piece]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    long long L = xcd_remap(blockIdx.x, a.nblk);
+    int mbi, nbi;
+    tri_decode(L, a.mb, a.nbk, mbi, nbi);
+    const int m0 = mbi * BM, n0 = nbi * BM;
+    const unsigned short *X = (const unsigned short *)a.X;
+    const long long K = a.K;
+    // staging: q = tid + 256 s -> row q / 4 (64 rows per pass), 8-wide k chunk q % 4
+    const int kc = tid & 3, rr0 = tid >> 2;
+    const unsigned short *pa = X + (long long)(m0 + rr0) * a.ldx + 8 * kc;
+    const unsigned short *pb = X + (long long)(n0 + rr0) * a.ldx + 8 * kc;
+    const long long pass = 64 * a.ldx;
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    s8 ra[3][2], rb[3][2];
+    auto gload = [&](long long k0) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                ra[p][s] = *reinterpret_cast<const s8 *>(pa + s * pass + p * K + k0);
+                rb[p][s] = *reinterpret_cast<const s8 *>(pb + s * pass + p * K + k0);
+            }
+    };
+    const int sto = 8 * (kc ^ swz(rr0));   // rows rr0 + 64 s share swz(rr0)
+    auto lstore = [&]() {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int r = rr0 + 64 * s;
+                *reinterpret_cast<s8 *>(&Ls[0][p][r * BKS + sto]) = ra[p][s];
+                *reinterpret_cast<s8 *>(&Ls[1][p][r * BKS + sto]) = rb[p][s];
+            }
+    };
+    // fragment: 16 rows (lane % 16) x 8 k (8 * (lane / 16) ..) of tile t;
+    // the rows' swizzle depends on lane % 16 alone (tile rows start at
+    // multiples of 16)
+    const int kof = 8 * ((lane >> 4) ^ swz(lane & 15));
+    auto frag = [&](int side, int p, int t) -> bf8 {
+        const int row = (side ? wn : wm) * 64 + t * 16 + (lane & 15);
+        return __builtin_bit_cast(bf8, *reinterpret_cast<const s8 *>(&Ls[side][p][row * BKS + kof]));
+    };
+    const int nk = (int)(K / BKS);
+    gload(0);
+    for (int kt = 0; kt < nk; ++kt) {
+        lstore();
+        if (kt + 1 < nk) gload((long long)(kt + 1) * BKS);
+        __syncthreads();
+        bf8 bh[4], bm[4], bl[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bh[t] = frag(1, 0, t);
+            bm[t] = frag(1, 1, t);
+            bl[t] = frag(1, 2, t);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bf8 ah = frag(0, 0, i), am = frag(0, 1, i), al = frag(0, 2, i);
+            f4 sm[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                sm[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[j], f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                sm[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[j], sm[j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                sm[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[j], sm[j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                sm[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], sm[j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                sm[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], sm[j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] += sm[j];
+        }
+        __syncthreads();
+    }
+    // Epilogue (C -= acc), per 16-row block i: the lane's 16 FP64 loads
+    // first, then the updates and stores; elements above the diagonal
+    // (diagonal blocks only) are written back unchanged.
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gr0 = a.r0 + m0 + wm * 64 + i * 16;
+        double *cp[4];
+        double old[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gc = a.r0 + n0 + wn * 64 + j * 16 + (lane & 15);
+            const int ti = gr0 / a.nb, tk = gc / a.nb;
+            cp[j] = a.C + tix(ti, tk, a.c_ym, 1) * ((long long)a.nb * a.nb) + (gr0 - ti * a.nb) +
+                    (long long)(gc - tk * a.nb) * a.nb;
+            if (ROWMAP == 0) {
+                const d2 *q = reinterpret_cast<const d2 *>(cp[j] + 4 * (lane >> 4));
+                const d2 lo = q[0], hi = q[1];
+                old[j][0] = lo.x;
+                old[j][1] = lo.y;
+                old[j][2] = hi.x;
+                old[j][3] = hi.y;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) old[j][r] = cp[j][(lane >> 4) + 4 * r];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gc = a.r0 + n0 + wn * 64 + j * 16 + (lane & 15);
+            double v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = ROWMAP == 0 ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r;
+                v[r] = gr0 + rr < gc ? old[j][r] : old[j][r] - (double)acc[i][j][r];
+            }
+            if (ROWMAP == 0) {
+                d2 *q = reinterpret_cast<d2 *>(cp[j] + 4 * (lane >> 4));
+                q[0] = d2{v[0], v[1]};
+                q[1] = d2{v[2], v[3]};
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cp[j][(lane >> 4) + 4 * r] = v[r];
+            }
+        }
+    }
+}
+
 int g_rowmap[2] = {-1, -1};
 std::mutex g_probe_mu;
 // Mixed-precision variant bits: 1 = LDS-staged FP32 update kernel (else
 // operands straight to MFMA registers); 2 = bulk update on the critical-path
-// stream (else overlapped on the second stream).
+// stream (else overlapped on the second stream); 4 = the FP32 route's
+// super-column updates on FP32 MFMA (else from three BF16 pieces).
 std::atomic<int> g_mp_variant{0};
 
 }  // namespace
@@ -617,6 +821,11 @@ hipError_t convert_super(const double *W, int nb, int ym, int j0, int sw, int i0
         return hipGetLastError();
     }
     if (nb % 64) return hipErrorInvalidValue;
+    if (mode == 2) {
+        hipLaunchKernelGGL(k_convert_super_split3, dim3(nrt * nb / 64, sw * nb / 64), dim3(256), 0,
+                           s, W, nb, ym, j0, i0, (unsigned short *)X, (long long)sw * nb);
+        return hipGetLastError();
+    }
     const int nt = nrt * sw;   // tiles of the super-column below row i0
     hipLaunchKernelGGL(k_zero_u32, dim3((nt + 255) / 256), dim3(256), 0, s, amax, nt);
     hipLaunchKernelGGL(k_absmax_super, dim3(16, sw, nrt), dim3(256), 0, s, W, nb, ym, j0, i0, amax);
@@ -642,9 +851,10 @@ hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const
                             hipStream_t s, const int *stop) {
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
     if (nb % BM || M % BM || N % BM || r0 % BM || M < N) return hipErrorInvalidValue;
-    if (mode != 0 && (K % nb || !scale)) return hipErrorInvalidValue;
-    if (K % (mode == 0 ? BK32 : BK16)) return hipErrorInvalidValue;
+    if (mode == 1 && (K % nb || !scale)) return hipErrorInvalidValue;
+    if (K % (mode == 0 ? BK32 : mode == 1 ? BK16 : BKS)) return hipErrorInvalidValue;
     if (mode == 0 ? (ldx % 4) != 0 : (ldx % 8) != 0) return hipErrorInvalidValue;
+    if (mode == 2 && (ldx != 3LL * K || ((uintptr_t)X & 15))) return hipErrorInvalidValue;
     LpArgs a;
     a.C = C;
     a.c_ym = c_ym;
@@ -672,6 +882,10 @@ hipError_t launch_update_lp(int mode, double *C, int c_ym, int nb, int r0, const
         hipLaunchKernelGGL((k_update_f32_dtv<8, 6, 1>), dim3((unsigned)n), dim3(NT), 0, s, a);
     else if (mode == 0)
         hipLaunchKernelGGL(k_update_f32, dim3((unsigned)n), dim3(NT), 0, s, a);
+    else if (mode == 2 && rowmap == 0)
+        hipLaunchKernelGGL(k_update_split3<0>, dim3((unsigned)n), dim3(NT), 0, s, a);
+    else if (mode == 2)
+        hipLaunchKernelGGL(k_update_split3<1>, dim3((unsigned)n), dim3(NT), 0, s, a);
     else if (rowmap == 0)
         hipLaunchKernelGGL(k_update_bf16<0>, dim3((unsigned)n), dim3(NT), 0, s, a);
     else
@@ -686,9 +900,10 @@ int mp_variant() { return g_mp_variant.load(); }
 }  // namespace ooc
 
 // Mixed-precision factorization variant (test and tuning hook), bits:
-// 1 = LDS-staged FP32 update kernel, 2 = no overlap of the bulk update.
+// 1 = LDS-staged FP32 update kernel, 2 = no overlap of the bulk update,
+// 4 = FP32 MFMA for the FP32 route's super-column updates.
 extern "C" int oocdag_mp_set_update_variant(int variant) {
-    if (variant < 0 || variant > 3) return OOCDAG_EARG;
+    if (variant < 0 || variant > 7) return OOCDAG_EARG;
     ooc::g_mp_variant.store(variant);
-    return 4;
+    return 8;
 }

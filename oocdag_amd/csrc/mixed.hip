@@ -560,16 +560,18 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
                                  [](int) { return 0; });
         return rc ? -2 : 0;
     }
-    const int mode = precision - 1;
+    // super-column update operand: 0 FP32, 1 scaled BF16, 2 FP32-accurate
+    // three BF16 pieces (the FP32 route unless variant bit 4)
+    const int mode = precision == 2 ? 1 : (mp_variant() & 4) ? 0 : 2;
     // overlap: the bulk update of super-column J runs on s1 beside the FP64
     // factorization of super-column J+1 (else after it on s)
     const bool overlap = (mp_variant() & 2) == 0;
-    const int rowmap = mfma_rowmap(mode, s);
+    const int rowmap = mfma_rowmap(mode == 0 ? 0 : 1, s);
     const int rowmap32 = mfma_rowmap(0, s);
     if (rowmap < 0 || rowmap32 < 0) return -1;
     const int sw = mp_super_width(NB, bb);
     const int nsc = (bb + sw - 1) / sw;
-    const size_t esz = mode == 0 ? 4 : 2;
+    const size_t esz = mode == 0 ? 4 : mode == 1 ? 2 : 6;
     const size_t lp_bytes = (size_t)bb * NB * sw * NB * esz;
     void *lp[2] = {lpbuf, (char *)lpbuf + lp_bytes};
     float *sc[2] = {scale, scale + (size_t)bb * sw};   // BF16: per tile of a super-column
@@ -632,7 +634,7 @@ int factor_mixed(double *W, int N, int NB, int precision, double *dinv_all, int 
         if (j1 >= bb) break;
         const int nrt = bb - j1;
         const int K = (j1 - j0) * NB;
-        const long long ldx = mode == 0 ? (long long)nrt * NB : (long long)K;
+        const long long ldx = mode == 0 ? (long long)nrt * NB : mode == 1 ? (long long)K : 3LL * K;
         // lp[J & 1] was last read by bulk(J-2), finished above
         MP_TRY(convert_super(W, NB, bb, j0, j1 - j0, j1, nrt, mode, amax, sc[J & 1], lp[J & 1], s));
         // the next super-column also received bulk(J-1); past the first
@@ -697,7 +699,7 @@ size_t mp_factor_bytes(int N, int NB) {
     const size_t tiles = bb * (bb + 1) / 2 * NB * NB;
     const size_t sw = (size_t)mp_super_width(NB, (int)bb);
     return (tiles + bb * NB * NB + bb * NB * 64) * 8 + (tiles + bb * NB * NB) * 4 +
-           2 * bb * NB * sw * NB * 4 + bb * NB * NB * 4;
+           2 * bb * NB * sw * NB * 6 + bb * NB * NB * 4;
 }
 
 void mp_release(MpFactor &f, hipStream_t s) {
@@ -732,7 +734,8 @@ int mp_factor(MpFactor &f, const double *dA, int N, int NB, int precision, hipSt
                  !ok(salloc(&f.dinv, (size_t)Np * 64 * 8, s0))))
         return OOCDAG_ENOMEM;
     const size_t sw = (size_t)mp_super_width(NB, bb);
-    const size_t lp_bytes = precision > 0 ? 2 * (size_t)bb * NB * sw * NB * (precision == 1 ? 4 : 2) : 8;
+    // FP32 route: three BF16 pieces (6 B) per element unless variant bit 4
+    const size_t lp_bytes = precision > 0 ? 2 * (size_t)bb * NB * sw * NB * (precision == 1 ? 6 : 2) : 8;
     if (!ok(salloc(&lp, lp_bytes, s0)) || !ok(salloc(&scale, (size_t)2 * bb * sw * 4, s0)) ||
         (precision > 0 && !ok(salloc(&colbuf, (size_t)bb * NB * NB * 4, s0))) ||
         !ok(salloc(&amax, (size_t)bb * sw * 4, s0)) || !ok(salloc(&info, sizeof(int), s0)) ||

@@ -197,7 +197,7 @@ def test_mpir_update_variants_agree(gpu, prec):
     out = []
     try:
         for variant in (3, 2, 1, 0):
-            assert ooc.mp_set_update_variant(variant) == 4
+            assert ooc.mp_set_update_variant(variant) == 8
             ooc.gram_tiles(buf, n, nb, 1e-2, seed=5)
             dX = ooc.DeviceBuffer(n * nrhs * 8)
             st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, prec, tol=8 * EPS, max_iter=30)
@@ -209,6 +209,47 @@ def test_mpir_update_variants_agree(gpu, prec):
     for it, x in out[1:]:
         assert it == out[0][0]
         assert np.array_equal(x, out[0][1])
+
+
+@pytest.mark.parametrize("mu", [1e-2, 1e-6])
+def test_mpir_fp32_three_piece_updates_keep_fp32_accuracy(gpu, mu):
+    """The FP32 route's super-column updates from three BF16 pieces per
+    element (x = h + m + l, six BF16 products) against the same updates on
+    FP32 MFMA (variant bit 4) and against the scaled-BF16 route.  With tol = 1
+    the first solve is accepted, so its backward error measures the factor:
+    the pieces' must be within a small factor of FP32's, and (mu = 1e-2; at
+    1e-6 the BF16 factor is not positive definite) far below BF16's
+    (u = 2^-8); the pieces' error without l would be ~2^-16, without m and l
+    ~2^-8, and both fail.  Refined to 8 eps, the two FP32 factors take the
+    same number of steps (+-1).  n = 8192, NB = 256: four super-columns,
+    narrow and bulk updates."""
+    ooc = gpu
+    n, nb, nrhs = 8192, 256, 4
+    buf = ooc.DeviceBuffer(ooc.tiles_elems(n, nb) * 8)
+    ooc.gram_tiles(buf, n, nb, mu, seed=6)
+    dB = ooc.DeviceBuffer(n * nrhs * 8)
+    ooc.normal_fill(dB, n, nrhs, n, seed=6, stream_id=2)
+    dX = ooc.DeviceBuffer(n * nrhs * 8)
+    first, steps = {}, {}
+    try:
+        runs = [("pieces", "fp32", 0), ("fp32", "fp32", 4)]
+        for name, prec, variant in runs + ([("bf16", "bf16", 0)] if mu == 1e-2 else []):
+            ooc.mp_set_update_variant(variant)
+            st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, prec, tol=1.0, max_iter=30)
+            assert st["iterations"] == 0 and st["fell_back"] == 0, (name, st)
+            first[name] = st["backward_error"]
+            if prec == "fp32":
+                st = ooc.mpir_solve_tiles(buf, n, nb, dB, nrhs, dX, prec, tol=8 * EPS,
+                                          max_iter=30)
+                assert st["fell_back"] == 0 and st["backward_error"] <= 8 * EPS, (name, st)
+                steps[name] = st["iterations"]
+    finally:
+        ooc.mp_set_update_variant(0)
+    print(mu, first, steps)
+    assert first["pieces"] <= 3 * first["fp32"], first
+    if mu == 1e-2:
+        assert first["bf16"] >= 30 * first["pieces"], first
+    assert abs(steps["pieces"] - steps["fp32"]) <= 1, steps
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
