@@ -238,18 +238,25 @@ __global__ __launch_bounds__(256) void k_convert_super_split3(const double *W, i
         T[k][tx] = tile[(rr + tx) + (long long)(kk + k) * nb];
     }
     __syncthreads();
+    // each thread: row r, eight consecutive k -> three 16-byte stores
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int r = ty + 4 * q;
-        const double x = T[tx][r];
-        const unsigned short h = f2bf((float)x);
-        const double r1 = x - (double)bf2f(h);
-        const unsigned short m = f2bf((float)r1);
-        const unsigned short l = f2bf((float)(r1 - (double)bf2f(m)));
-        unsigned short *p = X + (long long)(r0 + r) * 3 * K + k0 + tx;
-        p[0] = h;
-        p[K] = m;
-        p[2 * K] = l;
+    for (int q = 0; q < 2; ++q) {
+        const int pi = threadIdx.x + 256 * q, r = pi >> 3, kg = 8 * (pi & 7);
+        s8 h8, m8, l8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const double x = T[kg + e][r];
+            const unsigned short h = f2bf((float)x);
+            const double r1 = x - (double)bf2f(h);
+            const unsigned short m = f2bf((float)r1);
+            h8[e] = (short)h;
+            m8[e] = (short)m;
+            l8[e] = (short)f2bf((float)(r1 - (double)bf2f(m)));
+        }
+        unsigned short *p = X + (long long)(r0 + r) * 3 * K + k0 + kg;
+        *reinterpret_cast<s8 *>(p) = h8;
+        *reinterpret_cast<s8 *>(p + K) = m8;
+        *reinterpret_cast<s8 *>(p + 2 * K) = l8;
     }
 }
 
