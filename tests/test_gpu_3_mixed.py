@@ -44,8 +44,11 @@ def test_gram_generator_properties(gpu):
 # (6144, 128): three super-columns of 16 tile columns -- narrow and bulk
 # low-precision updates, both buffer parities; (100, 128) and (129, 256):
 # a single padded tile (the second factored as two 128-column halves);
-# (300, 128): one super-column narrower than 2048 columns, ragged.
-@pytest.mark.parametrize("n,nb", [(2048, 256), (6144, 128), (100, 128), (129, 256), (300, 128)])
+# (300, 128): one super-column narrower than 2048 columns, ragged;
+# (4999, 128): 40 tile columns, the last one padded, in super-columns of
+# 16, 16 and 8 -- narrow and bulk low-precision updates over a ragged edge.
+@pytest.mark.parametrize("n,nb", [(2048, 256), (6144, 128), (100, 128), (129, 256), (300, 128),
+                                  (4999, 128)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
 @pytest.mark.parametrize("mu", [1e-2, 1e-6])
 def test_mpir_matches_fp64_oracle(gpu, prec, mu, n, nb):
